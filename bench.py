@@ -1,0 +1,175 @@
+"""PageRank throughput on one MI355X.
+
+Workload (the plain one-graph GPU method): directed R-MAT scale 24 (16.7M
+nodes, 16 x 2^24 draws, a=0.57 b=0.19 c=0.19, seed 1, self-loops and
+duplicates removed), unweighted, alpha 0.85, generated and built in HBM.
+A step is one power iteration over the whole graph (tol = 0 so every step
+runs; the tol=1e-6 solve is reported beside it).  value = GTEPS =
+arcs x steps / seconds of the timed region, which is bracketed by device
+synchronisation on both sides.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--scale 24]
+
+Prints one JSON line.  ``roofline`` uses the algorithmic bytes of one
+iteration, 4m + 16n (int32 column index per arc; row pointer, out-degree,
+x read and x' write per node), over the mean duration of the SpMV launch
+measured with hipEvents on the graph's stream.  ``cpu_baseline`` times the
+OpenMP push of pagerank.cpp:21-122 (oracle/egx_oracle.c) on this host, on
+the same graph, for a few iterations.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(kernel_name_part):
+    """HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_rmat24.json")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as fh:
+        d = json.load(fh)
+    return d.get("hbm_bytes_per_launch"), os.path.relpath(path, ROOT)
+
+
+def cpu_baseline(g, n, m, iters, threads):
+    from oracle import oracle as O
+
+    V, E, _ = g.download_csr(with_weights=False)
+    t0 = time.perf_counter()
+    _, it = O.pagerank_omp(V, E, None, 0.85, iters, 0.0, threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(m * it / dt / 1e9, 4),
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"same R-MAT scale graph (n={n}, m={m}), {it} iterations at fixed K, OpenMP push with "
+                  f"atomic scatter (pagerank.cpp:21-122 semantics), {dt:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--cpu-iters", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    import eggpu_amd
+    from eggpu_amd import native
+
+    native.lib()
+    if native.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+
+    log(f"building R-MAT scale {args.scale} on the device")
+    t0 = time.perf_counter()
+    g = native.DeviceGraph.rmat(args.scale, 16, a=0.57, b=0.19, c=0.19, seed=1, keep_csr=not args.no_cpu)
+    build_wall = time.perf_counter() - t0
+    info = g.info()
+    n, m = int(info["n"]), int(info["m"])
+    log(f"n={n} m={m} dangling={info['n_dangling']} build {build_wall:.3f}s")
+
+    # warmup
+    if args.warmup > 0:
+        g.pagerank_stats(0.85, args.warmup, 0.0, time_each_launch=False)
+
+    # timed region: K iterations, device synchronised before and after
+    K = args.steps
+    t0 = time.perf_counter()
+    _, st = g.pagerank_stats(0.85, K, 0.0, time_each_launch=False)  # returns after a stream sync
+    wall = time.perf_counter() - t0
+    assert st["iterations"] == K
+    value = m * K / wall / 1e9
+
+    # per-launch timing of the SpMV kernel for the roofline (separate run)
+    _, st_t = g.pagerank_stats(0.85, K, 0.0, time_each_launch=True)
+    spmv_mean = st_t["spmv_seconds"] / st_t["spmv_launches"]
+    reduce_mean = st_t["reduce_seconds"] / st_t["spmv_launches"]
+    bytes_iter = 4 * m + 16 * n
+    achieved = bytes_iter / spmv_mean
+    traffic, traffic_src = read_traffic("k_pr_spmv")
+
+    # tol = 1e-6 solve (reference semantics) and the FP64 variant, beside
+    pr, it_tol, _ = g.pagerank(0.85, 100, 1e-6)
+    _, st64 = g.pagerank_stats(0.85, K, 0.0, precision=native.PRECISION_FP64, time_each_launch=False)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {args.cpu_iters} iterations, {threads} threads")
+        cpu = cpu_baseline(g, n, m, args.cpu_iters, threads)
+
+    line = {
+        "metric": "PageRank GTEPS & % HBM roofline; batched solves/s; p50/p99 latency @QPS",
+        "value": round(value, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded R-MAT generated on the device)",
+        "config": {
+            "workload": f"pagerank R-MAT scale {args.scale} plain method, fixed K={K} (tol=0)",
+            "n": n,
+            "m": m,
+            "alpha": 0.85,
+            "iterate_dtype": "f32",
+            "reductions": "f64 (dangling mass, L1 residual, teleport, row sums)",
+            "bins": {k: info[k] for k in ("thread_row_max", "wave_row_max", "hub_segment", "tile_items")},
+            "hub_rows": info["rows_hub"],
+            "tiles": info["tiles"],
+            "build_seconds": round(info["build_seconds"], 4),
+            "kernel_ms_per_iter": round(st["kernel_seconds"] / K * 1e3, 4),
+            "spmv_ms_mean": round(spmv_mean * 1e3, 4),
+            "reduce_ms_mean": round(reduce_mean * 1e3, 4),
+            "tol1e-6_iterations": it_tol,
+            "fp64_ms_per_iter": round(st64["kernel_seconds"] / K * 1e3, 4),
+            "fp64_gteps": round(m * K / st64["kernel_seconds"] / 1e9, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved / 1e9, 2),
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4),
+            "traffic": traffic,
+            "kernel": "k_pr_spmv",
+            "algorithmic_bytes_per_launch": bytes_iter,
+            "traffic_source": traffic_src,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

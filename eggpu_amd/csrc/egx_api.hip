@@ -1,0 +1,476 @@
+// egx_api.hip -- the C ABI declared in include/egx.h.
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "egx_graph.h"
+
+struct egx_graph {
+    egx::Graph g;
+};
+
+namespace egx {
+
+namespace {
+thread_local std::string t_last_error;
+
+std::atomic<uint64_t> g_hits{0}, g_misses{0}, g_builds{0}, g_refreshes{0};
+
+// Restores the calling thread's current device on scope exit.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) return;
+        ok = (prev == dev) || (hipSetDevice(dev) == hipSuccess);
+    }
+    ~DeviceGuard() {
+        if (ok && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+int new_graph(std::unique_ptr<egx_graph>& out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
+        set_last_error("no HIP device is available");
+        return EGX_DEVICE_ERR;
+    }
+    std::unique_ptr<egx_graph> h(new (std::nothrow) egx_graph());
+    if (!h) return EGX_FAILED_TO_ALLOCATE_HOST_MEM;
+    Graph& g = h->g;
+    g.device = dev;
+    EGX_HIP_TRY(hipStreamCreate(&g.stream), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipEventCreate(&g.ev0), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipEventCreate(&g.ev1), EGX_DEVICE_ERR);
+    out = std::move(h);
+    return EGX_SUCC;
+}
+
+int validate_csr(const int32_t* V, const int32_t* E, int32_t n, int32_t m) {
+    if (n < 0 || m < 0) {
+        set_last_error("negative node or arc count");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    if (n > 0 && V == nullptr) {
+        set_last_error("V is NULL");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    if (m > 0 && E == nullptr) {
+        set_last_error("E is NULL");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    if (n == 0) return EGX_SUCC;
+    if (V[0] != 0 || V[n] != m) {
+        set_last_error("invalid CSR offsets: V[0] must be 0 and V[n] must equal the arc count");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        if (V[i + 1] < V[i]) {
+            set_last_error("invalid CSR offsets: offsets must be monotonic");
+            return EGX_UNSUPPORTED_GRAPH;
+        }
+    }
+    return EGX_SUCC;
+}
+
+int graph_from_host_csr(const int32_t* V, const int32_t* E, const double* W, int32_t n, int32_t m, double alpha,
+                        bool keep_csr, std::unique_ptr<egx_graph>& out) {
+    EGX_TRY(validate_csr(V, E, n, m));
+    std::unique_ptr<egx_graph> h;
+    EGX_TRY(new_graph(h));
+    Graph& g = h->g;
+    DevBuf dV, dE, dW;
+    EGX_TRY(dV.alloc(((size_t)n + 1) * 4));
+    EGX_TRY(dE.alloc((size_t)std::max(m, 1) * 4));
+    if (n > 0) EGX_HIP_TRY(hipMemcpy(dV.p, V, ((size_t)n + 1) * 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    else EGX_HIP_TRY(hipMemset(dV.p, 0, 4), EGX_DEVICE_ERR);
+    if (m > 0) EGX_HIP_TRY(hipMemcpy(dE.p, E, (size_t)m * 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    if (W) {
+        EGX_TRY(dW.alloc((size_t)std::max(m, 1) * 8));
+        if (m > 0) EGX_HIP_TRY(hipMemcpy(dW.p, W, (size_t)m * 8, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    }
+    EGX_TRY(build_from_device_csr(g, dV.as<int32_t>(), dE.as<int32_t>(), W ? dW.as<double>() : nullptr, n, m, alpha));
+    if (keep_csr) {
+        g.csr_V = std::move(dV);
+        g.csr_E = std::move(dE);
+        g.csr_W = std::move(dW);
+        g.csr_m = m;
+        g.have_csr = true;
+    }
+    out = std::move(h);
+    return EGX_SUCC;
+}
+
+int finish_generated(std::unique_ptr<egx_graph>& h, DevBuf& V, DevBuf& E, int64_t m, int32_t n, bool weighted,
+                     double wlo, double whi, uint64_t wseed, double alpha, bool keep_csr) {
+    Graph& g = h->g;
+    DevBuf W;
+    if (weighted)
+        EGX_TRY(gen_edge_weights(V.as<int32_t>(), E.as<int32_t>(), n, m, wseed, wlo, whi, W, g.stream));
+    EGX_TRY(build_from_device_csr(g, V.as<int32_t>(), E.as<int32_t>(), weighted ? W.as<double>() : nullptr, n, m,
+                                  alpha));
+    if (keep_csr) {
+        g.csr_V = std::move(V);
+        g.csr_E = std::move(E);
+        g.csr_W = std::move(W);
+        g.csr_m = m;
+        g.have_csr = true;
+    }
+    return EGX_SUCC;
+}
+
+uint32_t threshold32(double x) {
+    double t = std::floor(x * 4294967296.0);
+    if (t < 0.0) t = 0.0;
+    if (t > 4294967295.0) t = 4294967295.0;
+    return (uint32_t)t;
+}
+
+// Per-thread prepared graph for egx_pagerank, after PagerankWorkspace and
+// its signature (pagerank.cu:44-195).
+struct ThreadCache {
+    std::unique_ptr<egx_graph> h;
+    int owner_device = -1;
+    uint64_t graph_id = 0;
+    const int32_t* V = nullptr;
+    const int32_t* E = nullptr;
+    const double* W = nullptr;
+    int32_t n = -1, m = -1;
+    int32_t v0 = 0, vn = 0, e0 = 0, em = 0;
+    uint64_t w0 = 0, wm = 0;
+};
+thread_local ThreadCache t_cache;
+std::atomic<uint64_t> g_resident{0};
+
+uint64_t bits64(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    return u;
+}
+
+bool cache_matches(const ThreadCache& c, const int32_t* V, const int32_t* E, const double* W, int32_t n, int32_t m,
+                   uint64_t graph_id) {
+    if (!c.h) return false;
+    if (c.graph_id != graph_id || c.V != V || c.E != E || c.W != W || c.n != n || c.m != m) return false;
+    if (n <= 0) return true;
+    if (c.v0 != V[0] || c.vn != V[n]) return false;
+    if (m > 0) {
+        if (c.e0 != E[0] || c.em != E[m - 1]) return false;
+        if (W) {
+            if (c.w0 != bits64(W[0]) || c.wm != bits64(W[m - 1])) return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+
+Graph::~Graph() {
+    if (device >= 0) (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (host_state) (void)hipHostFree(host_state);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+size_t Graph::device_bytes() const {
+    const DevBuf* bufs[] = {&in_row, &in_col, &in_val, &w_csc,  &out_w,  &out_cnt, &deg,    &csr_V,
+                            &csr_E,  &csr_W,  &perm,   &rank,   &p_row,  &p_col,   &p_val,  &p_w,
+                            &p_outw, &p_deg,  &tiles,  &hsegs,  &hrows,  &hub_part, &x[0],  &x[1],
+                            &s[0],   &s[1],   &state,  &partials, &out};
+    size_t total = 0;
+    for (const DevBuf* b : bufs) total += b->bytes;
+    return total;
+}
+
+}  // namespace egx
+
+using egx::Graph;
+
+extern "C" {
+
+const char* egx_err_code_detail(int status) {
+    switch (status) {
+        case EGX_SUCC: return "EasyGraph GPU: success";
+        case EGX_FAILED_TO_ALLOCATE_HOST_MEM: return "EasyGraph GPU: failed to allocate host mem";
+        case EGX_FAILED_TO_ALLOCATE_DEVICE_MEM: return "EasyGraph GPU: failed to allocate gpu mem";
+        case EGX_DEVICE_ERR: return "EasyGraph GPU: gpu error occurred";
+        case EGX_UNKNOW_ERROR: return "EasyGraph GPU: gpu unkonw error";
+        case EGX_UNSUPPORTED_GRAPH: return "EasyGraph GPU: unsupported graph type";
+        case EGX_NEGATIVE_CYCLE: return "EasyGraph GPU: negative weight cycle detected";
+        default: break;
+    }
+    return "EasyGraph GPU: not a valid err_code";
+}
+
+const char* egx_last_error(void) { return egx::t_last_error.c_str(); }
+
+const char* egx_version(void) { return "egx 0.1.0 (gfx950)"; }
+
+int egx_device_count(void) {
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int egx_graph_from_csr(const int32_t* V, const int32_t* E, const double* W, int32_t n, int32_t m, double alpha,
+                       int32_t keep_csr, egx_graph** out) {
+    if (!out) return EGX_UNSUPPORTED_GRAPH;
+    *out = nullptr;
+    std::unique_ptr<egx_graph> h;
+    EGX_TRY(egx::graph_from_host_csr(V, E, W, n, m, alpha, keep_csr != 0, h));
+    *out = h.release();
+    return EGX_SUCC;
+}
+
+int egx_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                   int32_t weighted, double wlo, double whi, uint64_t wseed, double alpha, int32_t keep_csr,
+                   egx_graph** out) {
+    if (!out) return EGX_UNSUPPORTED_GRAPH;
+    *out = nullptr;
+    if (scale < 1 || scale > 30 || edge_factor < 0) {
+        egx::set_last_error("R-MAT needs 1 <= scale <= 30 and edge_factor >= 0");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    std::unique_ptr<egx_graph> h;
+    EGX_TRY(egx::new_graph(h));
+    egx::DevBuf V, E;
+    int64_t m = 0;
+    const int64_t draws = (int64_t)edge_factor << scale;
+    EGX_TRY(egx::gen_rmat_csr(scale, draws, seed, egx::threshold32(a), egx::threshold32(a + b),
+                              egx::threshold32(a + b + c), V, E, m, h->g.stream));
+    EGX_TRY(egx::finish_generated(h, V, E, m, (int32_t)(1 << scale), weighted != 0, wlo, whi, wseed, alpha,
+                                  keep_csr != 0));
+    *out = h.release();
+    return EGX_SUCC;
+}
+
+int egx_graph_er(int32_t n, int64_t nedges, uint64_t seed, int32_t weighted, double wlo, double whi, uint64_t wseed,
+                 double alpha, int32_t keep_csr, egx_graph** out) {
+    if (!out) return EGX_UNSUPPORTED_GRAPH;
+    *out = nullptr;
+    std::unique_ptr<egx_graph> h;
+    EGX_TRY(egx::new_graph(h));
+    egx::DevBuf V, E;
+    int64_t m = 0;
+    EGX_TRY(egx::gen_er_csr(n, nedges, seed, V, E, m, h->g.stream));
+    EGX_TRY(egx::finish_generated(h, V, E, m, n, weighted != 0, wlo, whi, wseed, alpha, keep_csr != 0));
+    *out = h.release();
+    return EGX_SUCC;
+}
+
+int egx_graph_info_get(const egx_graph* gh, egx_graph_info* out) {
+    if (!gh || !out) return EGX_UNSUPPORTED_GRAPH;
+    const Graph& g = gh->g;
+    std::memset(out, 0, sizeof(*out));
+    out->n = g.n;
+    out->m = g.m;
+    out->n_dangling = g.n_dangling;
+    out->weighted = g.weighted ? 1 : 0;
+    out->max_in_degree = g.max_in_degree;
+    out->max_out_degree = g.max_out_degree;
+    out->thread_row_max = g.bins.thread_row_max;
+    out->wave_row_max = g.bins.wave_row_max;
+    out->hub_segment = g.bins.hub_segment;
+    out->tile_items = g.bins.tile_items;
+    out->hub_rows_first = g.n_hub;
+    out->reordered = g.reordered ? 1 : 0;
+    out->layout_seconds = g.layout_seconds;
+    out->rows_thread = g.rows_thread;
+    out->rows_wave = g.rows_wave;
+    out->tiles = g.n_tiles;
+    out->rows_hub = g.n_hrows;
+    out->hub_segments = g.n_hseg;
+    out->device_bytes = (int64_t)g.device_bytes();
+    out->build_seconds = g.build_seconds;
+    return EGX_SUCC;
+}
+
+int egx_graph_set_bins(egx_graph* gh, const egx_bin_opts* opts) {
+    if (!gh || !opts) return EGX_UNSUPPORTED_GRAPH;
+    Graph& g = gh->g;
+    std::lock_guard<std::mutex> lk(g.mu);
+    egx::DeviceGuard dg(g.device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    egx::BinCfg cfg = g.bins;
+    if (opts->thread_row_max > 0) cfg.thread_row_max = opts->thread_row_max;
+    if (opts->wave_row_max > 0) cfg.wave_row_max = opts->wave_row_max;
+    if (opts->hub_segment > 0) cfg.hub_segment = opts->hub_segment;
+    if (opts->tile_items > 0) cfg.tile_items = opts->tile_items;
+    if (opts->reorder >= 0) cfg.reorder = opts->reorder ? 1 : 0;
+    return egx::rebuild_bins(g, cfg);
+}
+
+int egx_graph_download_csc(const egx_graph* gh, int32_t* in_row, int32_t* in_col, float* in_val,
+                           int32_t* dangling01) {
+    if (!gh) return EGX_UNSUPPORTED_GRAPH;
+    Graph& g = const_cast<egx_graph*>(gh)->g;
+    std::lock_guard<std::mutex> lk(g.mu);
+    egx::DeviceGuard dg(g.device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    return egx::download_csc(g, in_row, in_col, in_val, dangling01);
+}
+
+int egx_graph_download_csr(const egx_graph* gh, int32_t* V, int32_t* E, double* W, int64_t* m_out) {
+    if (!gh) return EGX_UNSUPPORTED_GRAPH;
+    Graph& g = const_cast<egx_graph*>(gh)->g;
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (m_out) *m_out = g.csr_m;
+    if (!g.have_csr) {
+        egx::set_last_error("graph was built without keep_csr");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    egx::DeviceGuard dg(g.device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    EGX_HIP_TRY(hipStreamSynchronize(g.stream), EGX_DEVICE_ERR);
+    if (V) EGX_HIP_TRY(hipMemcpy(V, g.csr_V.p, ((size_t)g.n + 1) * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    if (E && g.csr_m > 0) EGX_HIP_TRY(hipMemcpy(E, g.csr_E.p, (size_t)g.csr_m * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    if (W && g.csr_W.p && g.csr_m > 0)
+        EGX_HIP_TRY(hipMemcpy(W, g.csr_W.p, (size_t)g.csr_m * 8, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+int egx_graph_pagerank_ex(egx_graph* gh, const egx_solve_opts* opts, double* PR, egx_solve_stats* stats,
+                          int32_t time_each_launch) {
+    if (!gh || !opts) return EGX_UNSUPPORTED_GRAPH;
+    Graph& g = gh->g;
+    std::lock_guard<std::mutex> lk(g.mu);
+    egx::DeviceGuard dg(g.device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    egx::SolveParams p;
+    p.alpha = opts->alpha;
+    p.max_iter = opts->max_iter;
+    p.tol = opts->tol;
+    p.precision = opts->precision;
+    p.check_every = opts->check_every;
+    p.time_each_launch = time_each_launch != 0;
+    egx::SolveStats st;
+    int rc = egx::solve_pagerank(g, p, PR, &st);
+    if (rc == EGX_SUCC) {
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            egx::set_last_error(hipGetErrorString(e));
+            rc = EGX_DEVICE_ERR;
+        }
+    }
+    if (stats) {
+        stats->kernel_seconds = st.kernel_seconds;
+        stats->spmv_seconds = st.spmv_seconds;
+        stats->reduce_seconds = st.reduce_seconds;
+        stats->iterations = st.iterations;
+        stats->spmv_launches = st.spmv_launches;
+        stats->spmv_grid = st.spmv_grid;
+    }
+    return rc;
+}
+
+int egx_graph_pagerank(egx_graph* gh, const egx_solve_opts* opts, double* PR, int32_t* iterations,
+                       double* kernel_seconds) {
+    egx_solve_stats st{};
+    int rc = egx_graph_pagerank_ex(gh, opts, PR, &st, 0);
+    if (iterations) *iterations = st.iterations;
+    if (kernel_seconds) *kernel_seconds = st.kernel_seconds;
+    return rc;
+}
+
+void egx_graph_free(egx_graph* gh) { delete gh; }
+
+int egx_pagerank(const int32_t* V, const int32_t* E, const double* W, int32_t len_V, int32_t len_E, double alpha,
+                 int32_t max_iter_num, double threshold, uint64_t graph_id, double* PR, int32_t* iterations,
+                 double* kernel_seconds) {
+    if (kernel_seconds) *kernel_seconds = 0.0;
+    if (iterations) *iterations = 0;
+    const int32_t n = len_V;
+    const int32_t m = len_E;
+    if (n <= 0) return EGX_SUCC;
+    if (!PR) {
+        egx::set_last_error("PR is NULL");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
+        egx::set_last_error("no HIP device is available");
+        return EGX_DEVICE_ERR;
+    }
+    egx::ThreadCache& c = egx::t_cache;
+    if (c.owner_device >= 0 && c.owner_device != dev) {
+        // pagerank.cu:373-377: the cached graph is bound to one device per thread
+        egx::set_last_error("this thread's cached PageRank graph lives on another device");
+        return EGX_DEVICE_ERR;
+    }
+    if (!egx::cache_matches(c, V, E, W, n, m, graph_id)) {
+        egx::g_misses++;
+        if (c.h) {
+            egx::g_resident -= c.h->g.device_bytes();
+            c.h.reset();
+        }
+        std::unique_ptr<egx_graph> h;
+        int rc = egx::graph_from_host_csr(V, E, W, n, m, alpha, false, h);
+        if (rc != EGX_SUCC) return rc;
+        egx::g_builds++;
+        c.h = std::move(h);
+        c.owner_device = dev;
+        c.graph_id = graph_id;
+        c.V = V;
+        c.E = E;
+        c.W = W;
+        c.n = n;
+        c.m = m;
+        c.v0 = V[0];
+        c.vn = V[n];
+        c.e0 = (m > 0) ? E[0] : 0;
+        c.em = (m > 0) ? E[m - 1] : 0;
+        c.w0 = (W && m > 0) ? egx::bits64(W[0]) : 0;
+        c.wm = (W && m > 0) ? egx::bits64(W[m - 1]) : 0;
+    } else {
+        egx::g_hits++;
+        if (c.h->g.weighted && c.h->g.alpha_vals != alpha) egx::g_refreshes++;
+    }
+    egx_solve_opts o{};
+    o.alpha = alpha;
+    o.max_iter = max_iter_num;
+    o.tol = threshold;
+    o.precision = EGX_PRECISION_FP32;
+    o.check_every = 0;
+    int rc = egx_graph_pagerank(c.h.get(), &o, PR, iterations, kernel_seconds);
+    if (c.h) {
+        egx::g_resident = c.h->g.device_bytes();
+    }
+    if (rc != EGX_SUCC) {
+        // pagerank.cu:706-711: drop runtime state after a failure
+        c.h.reset();
+        c.owner_device = -1;
+        egx::g_resident = 0;
+    }
+    return rc;
+}
+
+void egx_pagerank_cache_reset(void) {
+    egx::ThreadCache& c = egx::t_cache;
+    c.h.reset();
+    c.owner_device = -1;
+    c.graph_id = 0;
+    c.V = nullptr;
+    c.E = nullptr;
+    c.W = nullptr;
+    c.n = c.m = -1;
+    egx::g_resident = 0;
+    egx::g_hits = egx::g_misses = egx::g_builds = egx::g_refreshes = 0;
+}
+
+int egx_pagerank_cache_stats(egx_cache_stats* out) {
+    if (!out) return EGX_UNSUPPORTED_GRAPH;
+    out->hits = egx::g_hits.load();
+    out->misses = egx::g_misses.load();
+    out->builds = egx::g_builds.load();
+    out->value_refreshes = egx::g_refreshes.load();
+    out->resident_bytes = egx::g_resident.load();
+    return EGX_SUCC;
+}
+
+}  // extern "C"

@@ -1,0 +1,554 @@
+// egx_build.hip -- COO/CSR -> pull CSC on the device.
+//
+// Reproduces build_inbound_from_csr (Easy-Graph/gpu_easygraph/functions/
+// centrality/pagerank.cu:286-347) bit for bit, without its single-threaded
+// host passes:
+//   * the stable counting pass over u ascending (:328-346) becomes a stable
+//     LSD radix sort of (dst, slot) pairs; the input is in out-CSR order
+//     (source ascending), so sources stay ascending inside each dst row and
+//     duplicate arcs keep their CSR order;
+//   * in_row (:318-319) is the lower bound of every row id in the sorted
+//     keys, i.e. the exclusive scan of the in-degree;
+//   * endpoints outside [0, n) sort into a sentinel bucket past row n-1 and
+//     are dropped (:282-284, :310); non-finite or negative weights count as 0
+//     (:311-312);
+//   * out-weights are summed per row, sequentially in CSR order, in FP64, so
+//     fl32(alpha * (w / outW)) (:344) rounds exactly as the reference does.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "egx_graph.h"
+
+namespace egx {
+
+namespace {
+
+__global__ void k_expand_src(const int32_t* __restrict__ V, int32_t n, int64_t m, int32_t* __restrict__ src) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    // last u with V[u] <= p (V is monotonic, V[n] = m > p)
+    int32_t lo = 0, hi = n + 1;
+    while (lo < hi) {
+        int32_t mid = (lo + hi) >> 1;
+        if ((int64_t)V[mid] <= p) lo = mid + 1;
+        else hi = mid;
+    }
+    src[p] = lo - 1;
+}
+
+__global__ void k_make_sort_keys(const int32_t* __restrict__ E, const int32_t* __restrict__ src, int32_t n,
+                                 int64_t m, int weighted, uint32_t* __restrict__ keys,
+                                 uint32_t* __restrict__ vals, uint32_t* __restrict__ invalid_count) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    const int32_t v = E[p];
+    const bool valid = (0 <= v) && (v < n);
+    keys[p] = valid ? (uint32_t)v : (uint32_t)n;
+    vals[p] = weighted ? (uint32_t)p : (uint32_t)src[p];
+    if (!valid) atomicAdd(invalid_count, 1u);
+}
+
+__global__ void k_outcnt_from_V(const int32_t* __restrict__ V, int32_t n, double* __restrict__ out_w,
+                                int32_t* __restrict__ outcnt) {
+    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const int32_t c = V[u + 1] - V[u];
+    outcnt[u] = c;
+    out_w[u] = (double)c;
+}
+
+// One thread per row, sequential in CSR order: the summation order of
+// pagerank.cu:308-315.
+__global__ void k_row_out(const int32_t* __restrict__ V, const int32_t* __restrict__ E,
+                          const double* __restrict__ W, int32_t n, double* __restrict__ out_w,
+                          int32_t* __restrict__ outcnt) {
+    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    double s = 0.0;
+    int32_t c = 0;
+    const int32_t end = V[u + 1];
+    for (int32_t p = V[u]; p < end; ++p) {
+        const int32_t v = E[p];
+        if (v < 0 || v >= n) continue;
+        double w = W ? W[p] : 1.0;
+        if (!isfinite(w) || w < 0.0) w = 0.0;
+        s += w;
+        ++c;
+    }
+    out_w[u] = s;
+    outcnt[u] = c;
+}
+
+// deg (0 == dangling), dangling count, max out-degree.
+__global__ void k_node_arrays(const double* __restrict__ out_w, const int32_t* __restrict__ outcnt, int32_t n,
+                              int weighted, float* __restrict__ deg, uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t sh_d, sh_max;
+    if (threadIdx.x == 0) { sh_d = 0; sh_max = 0; }
+    __syncthreads();
+    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) {
+        const bool dang = !(out_w[u] > 0.0);  // pagerank.cu:323-326: outW <= 0
+        deg[u] = weighted ? (dang ? 0.0f : 1.0f) : (dang ? 0.0f : (float)outcnt[u]);
+        if (dang) atomicAdd(&sh_d, 1u);
+        atomicMax(&sh_max, (uint32_t)outcnt[u]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (sh_d) atomicAdd(&ctr[1], sh_d);
+        atomicMax(&ctr[2], sh_max);
+    }
+}
+
+__global__ void k_fill_csc(const uint32_t* __restrict__ vals, const int32_t* __restrict__ src,
+                           const double* __restrict__ W, const double* __restrict__ out_w, int64_t m_valid,
+                           int weighted, double alpha, int32_t* __restrict__ in_col, float* __restrict__ in_val,
+                           double* __restrict__ w_csc) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m_valid) return;
+    const uint32_t val = vals[q];
+    if (weighted) {
+        const int32_t u = src[val];
+        double w = W[val];
+        if (!isfinite(w) || w < 0.0) w = 0.0;
+        const double denom = out_w[u];
+        in_col[q] = u;
+        w_csc[q] = w;
+        in_val[q] = (denom > 0.0) ? (float)(alpha * (w / denom)) : 0.0f;
+    } else {
+        in_col[q] = (int32_t)val;
+    }
+}
+
+__global__ void k_refresh_vals(const int32_t* __restrict__ in_col, const double* __restrict__ w_csc,
+                               const double* __restrict__ out_w, int64_t m, double alpha,
+                               float* __restrict__ in_val) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const double denom = out_w[in_col[q]];
+    in_val[q] = (denom > 0.0) ? (float)(alpha * (w_csc[q] / denom)) : 0.0f;
+}
+
+// Unit-weight in_val of the reference (W = ones): fl32(alpha * (1.0 / outW)).
+__global__ void k_unit_vals(const int32_t* __restrict__ in_col, const double* __restrict__ out_w, int64_t m,
+                            double alpha, float* __restrict__ in_val) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const double denom = out_w[in_col[q]];
+    in_val[q] = (denom > 0.0) ? (float)(alpha * (1.0 / denom)) : 0.0f;
+}
+
+__global__ void k_max_in_degree(const int32_t* __restrict__ in_row, int32_t n, uint32_t* __restrict__ out) {
+    __shared__ uint32_t sh;
+    if (threadIdx.x == 0) sh = 0;
+    __syncthreads();
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) atomicMax(&sh, (uint32_t)(in_row[v + 1] - in_row[v]));
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, sh);
+}
+
+// Relabelling key: hub rows (in-degree > hub) first, then descending
+// out-degree when reorder is on; the stable sort keeps ids ascending among
+// equal keys.  perm[r] = original id of row r, rank = its inverse.
+__global__ void k_layout_keys(const int32_t* __restrict__ in_row, const int32_t* __restrict__ out_cnt, int32_t n,
+                              int32_t hub, int32_t bits, uint32_t maxdeg, int reorder, uint32_t* __restrict__ keys,
+                              uint32_t* __restrict__ vals, uint32_t* __restrict__ n_hub) {
+    __shared__ uint32_t sh;
+    if (threadIdx.x == 0) sh = 0;
+    __syncthreads();
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) {
+        const bool is_hub = (in_row[v + 1] - in_row[v]) > hub;
+        const uint32_t sec = reorder ? (maxdeg - (uint32_t)out_cnt[v]) : 0u;
+        keys[v] = ((is_hub ? 0u : 1u) << bits) | sec;
+        vals[v] = (uint32_t)v;
+        if (is_hub) atomicAdd(&sh, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sh) atomicAdd(n_hub, sh);
+}
+
+// tiles[k] = first row v in [n_hub, n] whose merge-path diagonal
+// (it_row[v] - it_row[n_hub]) + (v - n_hub) reaches k * tile_items.
+__global__ void k_tile_starts(const int32_t* __restrict__ row, int32_t n_hub, int32_t n, int32_t tile_items,
+                              int64_t n_tiles, int32_t* __restrict__ tiles) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n_tiles) return;
+    if (k == n_tiles) {
+        tiles[k] = n;
+        return;
+    }
+    const int64_t target = k * (int64_t)tile_items;
+    const int64_t base = row[n_hub];
+    int32_t lo = n_hub, hi = n;
+    while (lo < hi) {
+        const int32_t mid = lo + ((hi - lo) >> 1);
+        const int64_t diag = ((int64_t)row[mid] - base) + (int64_t)(mid - n_hub);
+        if (diag < target) lo = mid + 1;
+        else hi = mid;
+    }
+    tiles[k] = lo;
+}
+
+__global__ void k_layout_nodes(const uint32_t* __restrict__ perm, const int32_t* __restrict__ in_row,
+                               const float* __restrict__ deg, const double* __restrict__ out_w, int32_t n,
+                               int32_t* __restrict__ rank, uint32_t* __restrict__ cnt, float* __restrict__ p_deg,
+                               double* __restrict__ p_outw) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int32_t v = (int32_t)perm[r];
+    rank[v] = r;
+    cnt[r] = (uint32_t)(in_row[v + 1] - in_row[v]);
+    p_deg[r] = deg[v];
+    p_outw[r] = out_w[v];
+}
+
+// One thread per relabelled CSC slot: find its row, copy the arc with its
+// source relabelled.
+__global__ void k_layout_slots(const int32_t* __restrict__ p_row, const int32_t* __restrict__ perm,
+                               const int32_t* __restrict__ rank, const int32_t* __restrict__ in_row,
+                               const int32_t* __restrict__ in_col, const float* __restrict__ in_val,
+                               const double* __restrict__ w_csc, int32_t n, int64_t m, int weighted,
+                               int32_t* __restrict__ p_col, float* __restrict__ p_val, double* __restrict__ p_w) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    int32_t lo = 0, hi = n + 1;  // last r with p_row[r] <= q
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if ((int64_t)p_row[mid] <= q) lo = mid + 1;
+        else hi = mid;
+    }
+    const int32_t r = lo - 1;
+    const int32_t v = perm[r];
+    const int64_t src_slot = (int64_t)in_row[v] + (q - p_row[r]);
+    p_col[q] = rank[in_col[src_slot]];
+    if (weighted) {
+        p_val[q] = in_val[src_slot];
+        p_w[q] = w_csc[src_slot];
+    }
+}
+
+__global__ void k_set_last(int32_t* __restrict__ arr, int64_t idx, int32_t value) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) arr[idx] = value;
+}
+
+inline unsigned nblocks(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace
+
+int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
+    BinCfg cfg = cfg_in;
+    if (cfg.thread_row_max < 0) cfg.thread_row_max = 0;
+    if (cfg.wave_row_max < 1) cfg.wave_row_max = 1;
+    if (cfg.wave_row_max > kTileLds - 256) cfg.wave_row_max = kTileLds - 256;
+    if (cfg.tile_items < 256) cfg.tile_items = 256;
+    if (cfg.tile_items > kTileLds - cfg.wave_row_max) cfg.tile_items = kTileLds - cfg.wave_row_max;
+    if (cfg.hub_segment < 256) cfg.hub_segment = 256;
+    if (cfg.reorder != 0) cfg.reorder = 1;
+    if (!g.layout_built || g.layout_hub != cfg.wave_row_max || g.layout_reorder != cfg.reorder)
+        EGX_TRY(build_layout(g, cfg));
+    g.bins = cfg;
+    hipStream_t s = g.stream;
+    const int32_t n = g.n;
+    g.n_hseg = g.n_hrows = g.n_tiles = 0;
+    g.rows_thread = g.rows_wave = 0;
+    if (n <= 0) return EGX_SUCC;
+    std::vector<int32_t> row((size_t)n + 1);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(row.data(), g.it_row, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+
+    // hub rows [0, n_hub): fixed-size segments
+    std::vector<HubSeg> segs;
+    std::vector<HubRow> hub_rows;
+    for (int32_t v = 0; v < g.n_hub; ++v) {
+        const int32_t b = row[(size_t)v], e = row[(size_t)v + 1];
+        const int32_t nseg = (int32_t)(((int64_t)(e - b) + cfg.hub_segment - 1) / cfg.hub_segment);
+        const int32_t first = (int32_t)segs.size();
+        hub_rows.push_back(HubRow{v, first, nseg, 0});
+        for (int32_t k = 0; k < nseg; ++k) {
+            HubSeg sg;
+            sg.row = v;
+            sg.beg = b + k * cfg.hub_segment;
+            sg.end = (int32_t)std::min<int64_t>((int64_t)sg.beg + cfg.hub_segment, e);
+            sg.slot = v;
+            sg.nseg = nseg;
+            sg.first = first;
+            segs.push_back(sg);
+        }
+    }
+    g.n_hrows = (int64_t)hub_rows.size();
+    g.n_hseg = (int64_t)segs.size();
+    EGX_TRY(g.hsegs.alloc(std::max<size_t>(segs.size(), 1) * sizeof(HubSeg)));
+    if (!segs.empty())
+        EGX_HIP_TRY(hipMemcpy(g.hsegs.p, segs.data(), segs.size() * sizeof(HubSeg), hipMemcpyHostToDevice),
+                    EGX_DEVICE_ERR);
+    EGX_TRY(g.hrows.alloc(std::max<size_t>(hub_rows.size(), 1) * sizeof(HubRow)));
+    if (!hub_rows.empty())
+        EGX_HIP_TRY(hipMemcpy(g.hrows.p, hub_rows.data(), hub_rows.size() * sizeof(HubRow), hipMemcpyHostToDevice),
+                    EGX_DEVICE_ERR);
+    EGX_TRY(g.hub_part.alloc(std::max<size_t>(segs.size(), 1) * 8));
+
+    // merge-path tiles over rows [n_hub, n)
+    const int64_t items = ((int64_t)row[(size_t)n] - row[(size_t)g.n_hub]) + (int64_t)(n - g.n_hub);
+    g.n_tiles = (items + cfg.tile_items - 1) / cfg.tile_items;
+    EGX_TRY(g.tiles.alloc(((size_t)g.n_tiles + 1) * 4));
+    k_tile_starts<<<nblocks(g.n_tiles + 1, 256), 256, 0, s>>>(g.it_row, g.n_hub, n, cfg.tile_items, g.n_tiles,
+                                                               g.tiles.as<int32_t>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    std::vector<int32_t> ts((size_t)g.n_tiles + 1);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(ts.data(), g.tiles.p, ts.size() * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    for (int64_t k = 0; k < g.n_tiles; ++k) {
+        const int64_t r = ts[(size_t)k + 1] - ts[(size_t)k];
+        const int64_t arcs = (int64_t)row[(size_t)ts[(size_t)k + 1]] - row[(size_t)ts[(size_t)k]];
+        if (arcs > (int64_t)cfg.thread_row_max * r) g.rows_wave += r;
+        else g.rows_thread += r;
+    }
+    return EGX_SUCC;
+}
+
+int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const double* dW, int32_t n, int64_t m,
+                          double alpha) {
+    hipStream_t s = g.stream;
+    g.n = n;
+    g.weighted = dW != nullptr;
+    g.alpha_vals = alpha;
+    g.alpha_orig = alpha;
+    const auto t_start = std::chrono::steady_clock::now();
+
+    DevBuf ctr;
+    EGX_TRY(ctr.alloc(4 * sizeof(uint32_t)));
+    EGX_HIP_TRY(hipMemsetAsync(ctr.p, 0, 4 * sizeof(uint32_t), s), EGX_DEVICE_ERR);
+
+    EGX_TRY(g.out_w.alloc((size_t)std::max(n, 1) * 8));
+    EGX_TRY(g.deg.alloc((size_t)std::max(n, 1) * 4));
+    EGX_TRY(g.out_cnt.alloc((size_t)std::max(n, 1) * 4));
+    int32_t* outcnt_p = g.out_cnt.as<int32_t>();
+
+    DevBuf src, keys, keys_alt, vals, vals_alt;
+    const size_t mm = (size_t)std::max<int64_t>(m, 1);
+    EGX_TRY(src.alloc(mm * 4));
+    EGX_TRY(keys.alloc(mm * 4));
+    EGX_TRY(keys_alt.alloc(mm * 4));
+    EGX_TRY(vals.alloc(mm * 4));
+    EGX_TRY(vals_alt.alloc(mm * 4));
+    if (m > 0) {
+        k_expand_src<<<nblocks(m, 256), 256, 0, s>>>(dV, n, m, src.as<int32_t>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        k_make_sort_keys<<<nblocks(m, 256), 256, 0, s>>>(dE, src.as<int32_t>(), n, m, g.weighted ? 1 : 0,
+                                                          keys.as<uint32_t>(), vals.as<uint32_t>(), ctr.as<uint32_t>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
+    uint32_t invalid = 0;
+    EGX_HIP_TRY(hipMemcpyAsync(&invalid, ctr.p, 4, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    if (n > 0) {
+        if (!g.weighted && invalid == 0)
+            k_outcnt_from_V<<<nblocks(n, 256), 256, 0, s>>>(dV, n, g.out_w.as<double>(), outcnt_p);
+        else
+            k_row_out<<<nblocks(n, 64), 64, 0, s>>>(dV, dE, dW, n, g.out_w.as<double>(), outcnt_p);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        k_node_arrays<<<nblocks(n, 256), 256, 0, s>>>(g.out_w.as<double>(), outcnt_p, n,
+                                                       g.weighted ? 1 : 0, g.deg.as<float>(), ctr.as<uint32_t>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
+
+    // stable sort of (dst-or-sentinel, slot-or-src) on the bits that can hold n
+    EGX_TRY(radix_sort_u32(keys.as<uint32_t>(), keys_alt.as<uint32_t>(), vals.as<uint32_t>(),
+                           vals_alt.as<uint32_t>(), m, bits_for((uint64_t)n), s));
+    keys_alt.release();
+    vals_alt.release();
+
+    EGX_TRY(g.in_row.alloc(((size_t)n + 1) * 4));
+    EGX_TRY(lower_bound_iota_u32(keys.as<uint32_t>(), m, g.in_row.as<int32_t>(), (int64_t)n + 1, s));
+    int32_t m_valid = 0;
+    EGX_HIP_TRY(hipMemcpyAsync(&m_valid, g.in_row.as<int32_t>() + n, 4, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    g.m = m_valid;
+
+    EGX_TRY(g.in_col.alloc((size_t)std::max(m_valid, 1) * 4));
+    if (g.weighted) {
+        EGX_TRY(g.in_val.alloc((size_t)std::max(m_valid, 1) * 4));
+        EGX_TRY(g.w_csc.alloc((size_t)std::max(m_valid, 1) * 8));
+    } else {
+        g.in_val.release();
+        g.w_csc.release();
+    }
+    if (m_valid > 0) {
+        k_fill_csc<<<nblocks(m_valid, 256), 256, 0, s>>>(vals.as<uint32_t>(), src.as<int32_t>(), dW,
+                                                          g.out_w.as<double>(), m_valid, g.weighted ? 1 : 0, alpha,
+                                                          g.in_col.as<int32_t>(), g.in_val.as<float>(),
+                                                          g.w_csc.as<double>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
+    if (n > 0) {
+        k_max_in_degree<<<nblocks(n, 256), 256, 0, s>>>(g.in_row.as<int32_t>(), n, ctr.as<uint32_t>() + 3);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
+    uint32_t hctr[4];
+    EGX_HIP_TRY(hipMemcpyAsync(hctr, ctr.p, sizeof(hctr), hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    g.n_dangling = hctr[1];
+    g.max_out_degree = (int32_t)hctr[2];
+    g.max_in_degree = (int32_t)hctr[3];
+
+    src.release();
+    keys.release();
+    vals.release();
+    g.layout_built = false;
+    EGX_TRY(rebuild_bins(g, g.bins));
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    g.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return EGX_SUCC;
+}
+
+int build_layout(Graph& g, const BinCfg& cfg) {
+    hipStream_t s = g.stream;
+    const int32_t n = g.n;
+    const int64_t m = g.m;
+    g.p_row.release();
+    g.p_col.release();
+    g.p_val.release();
+    g.p_w.release();
+    g.p_outw.release();
+    g.p_deg.release();
+    g.perm.release();
+    g.rank.release();
+    g.reordered = false;
+    g.layout_built = false;
+    g.n_hub = 0;
+    g.it_row = g.in_row.as<int32_t>();
+    g.it_col = g.in_col.as<int32_t>();
+    g.it_val = g.in_val.as<float>();
+    g.it_w = g.w_csc.as<double>();
+    g.it_outw = g.out_w.as<double>();
+    g.it_deg = g.deg.as<float>();
+    g.alpha_vals = g.alpha_orig;
+    g.layout_seconds = 0.0;
+    if (n <= 0) {
+        g.layout_built = true;
+        g.layout_hub = cfg.wave_row_max;
+        g.layout_reorder = cfg.reorder;
+        return EGX_SUCC;
+    }
+
+    EGX_HIP_TRY(hipEventRecord(g.ev0, s), EGX_DEVICE_ERR);
+    const uint32_t maxdeg = (uint32_t)g.max_out_degree;
+    const int bits = cfg.reorder ? bits_for(maxdeg) : 0;
+    DevBuf keys, keys_alt, vals_alt, cnt, hubs;
+    EGX_TRY(keys.alloc((size_t)n * 4));
+    EGX_TRY(keys_alt.alloc((size_t)n * 4));
+    EGX_TRY(g.perm.alloc((size_t)n * 4));
+    EGX_TRY(vals_alt.alloc((size_t)n * 4));
+    EGX_TRY(hubs.alloc(4));
+    EGX_HIP_TRY(hipMemsetAsync(hubs.p, 0, 4, s), EGX_DEVICE_ERR);
+    k_layout_keys<<<nblocks(n, 256), 256, 0, s>>>(g.in_row.as<int32_t>(), g.out_cnt.as<int32_t>(), n,
+                                                   cfg.wave_row_max, bits, maxdeg, cfg.reorder,
+                                                   keys.as<uint32_t>(), g.perm.as<uint32_t>(), hubs.as<uint32_t>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_TRY(radix_sort_u32(keys.as<uint32_t>(), keys_alt.as<uint32_t>(), g.perm.as<uint32_t>(),
+                           vals_alt.as<uint32_t>(), n, bits + 1, s));
+    uint32_t nh = 0;
+    EGX_HIP_TRY(hipMemcpy(&nh, hubs.p, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    keys.release();
+    keys_alt.release();
+    vals_alt.release();
+
+    EGX_TRY(g.rank.alloc((size_t)n * 4));
+    EGX_TRY(cnt.alloc((size_t)n * 4));
+    EGX_TRY(g.p_deg.alloc((size_t)n * 4));
+    EGX_TRY(g.p_outw.alloc((size_t)n * 8));
+    EGX_TRY(g.p_row.alloc(((size_t)n + 1) * 4));
+    k_layout_nodes<<<nblocks(n, 256), 256, 0, s>>>(g.perm.as<uint32_t>(), g.in_row.as<int32_t>(), g.deg.as<float>(),
+                                                    g.out_w.as<double>(), n, g.rank.as<int32_t>(),
+                                                    cnt.as<uint32_t>(), g.p_deg.as<float>(), g.p_outw.as<double>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_TRY(exclusive_scan_u32(cnt.as<uint32_t>(), g.p_row.as<uint32_t>(), n, s));
+    k_set_last<<<1, 64, 0, s>>>(g.p_row.as<int32_t>(), n, (int32_t)m);
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    cnt.release();
+
+    EGX_TRY(g.p_col.alloc((size_t)std::max<int64_t>(m, 1) * 4));
+    if (g.weighted) {
+        EGX_TRY(g.p_val.alloc((size_t)std::max<int64_t>(m, 1) * 4));
+        EGX_TRY(g.p_w.alloc((size_t)std::max<int64_t>(m, 1) * 8));
+    }
+    if (m > 0) {
+        k_layout_slots<<<nblocks(m, 256), 256, 0, s>>>(g.p_row.as<int32_t>(), g.perm.as<int32_t>(),
+                                                        g.rank.as<int32_t>(), g.in_row.as<int32_t>(),
+                                                        g.in_col.as<int32_t>(), g.in_val.as<float>(),
+                                                        g.w_csc.as<double>(), n, m, g.weighted ? 1 : 0,
+                                                        g.p_col.as<int32_t>(), g.p_val.as<float>(),
+                                                        g.p_w.as<double>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
+    g.reordered = true;
+    g.n_hub = (int32_t)nh;
+    g.it_row = g.p_row.as<int32_t>();
+    g.it_col = g.p_col.as<int32_t>();
+    g.it_val = g.p_val.as<float>();
+    g.it_w = g.p_w.as<double>();
+    g.it_outw = g.p_outw.as<double>();
+    g.it_deg = g.p_deg.as<float>();
+    EGX_HIP_TRY(hipEventRecord(g.ev1, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipEventSynchronize(g.ev1), EGX_DEVICE_ERR);
+    float ms = 0.0f;
+    EGX_HIP_TRY(hipEventElapsedTime(&ms, g.ev0, g.ev1), EGX_DEVICE_ERR);
+    g.layout_seconds = (double)ms * 1e-3;
+    g.layout_built = true;
+    g.layout_hub = cfg.wave_row_max;
+    g.layout_reorder = cfg.reorder;
+    return EGX_SUCC;
+}
+
+int refresh_alpha(Graph& g, double alpha) {
+    if (!g.weighted || alpha == g.alpha_vals) {
+        if (!g.weighted) g.alpha_vals = alpha;
+        return EGX_SUCC;
+    }
+    if (g.m > 0) {
+        k_refresh_vals<<<nblocks(g.m, 256), 256, 0, g.stream>>>(g.it_col, g.it_w, g.it_outw, g.m, alpha, g.it_val);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
+    g.alpha_vals = alpha;
+    if (!g.reordered) g.alpha_orig = alpha;  // in_val is the iteration array
+    return EGX_SUCC;
+}
+
+int download_csc(Graph& g, int32_t* in_row, int32_t* in_col, float* in_val, int32_t* dangling01) {
+    hipStream_t s = g.stream;
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    if (in_row)
+        EGX_HIP_TRY(hipMemcpy(in_row, g.in_row.p, ((size_t)g.n + 1) * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    if (in_col && g.m > 0)
+        EGX_HIP_TRY(hipMemcpy(in_col, g.in_col.p, (size_t)g.m * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    if (in_val && g.m > 0) {
+        // values for the alpha of the last build or solve
+        if (g.weighted && g.alpha_orig == g.alpha_vals) {
+            EGX_HIP_TRY(hipMemcpy(in_val, g.in_val.p, (size_t)g.m * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+        } else {
+            DevBuf tmp;
+            EGX_TRY(tmp.alloc((size_t)g.m * 4));
+            if (g.weighted)
+                k_refresh_vals<<<nblocks(g.m, 256), 256, 0, s>>>(g.in_col.as<int32_t>(), g.w_csc.as<double>(),
+                                                                  g.out_w.as<double>(), g.m, g.alpha_vals,
+                                                                  tmp.as<float>());
+            else
+                k_unit_vals<<<nblocks(g.m, 256), 256, 0, s>>>(g.in_col.as<int32_t>(), g.out_w.as<double>(), g.m,
+                                                               g.alpha_vals, tmp.as<float>());
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+            EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+            EGX_HIP_TRY(hipMemcpy(in_val, tmp.p, (size_t)g.m * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+        }
+    }
+    if (dangling01 && g.n > 0) {
+        std::vector<float> d((size_t)g.n);
+        EGX_HIP_TRY(hipMemcpy(d.data(), g.deg.p, (size_t)g.n * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+        for (int32_t i = 0; i < g.n; ++i) dangling01[i] = (d[(size_t)i] == 0.0f) ? 1 : 0;
+    }
+    return EGX_SUCC;
+}
+
+}  // namespace egx

@@ -1,0 +1,167 @@
+// egx_graph.h -- the device-resident graph.
+//
+// Two layouts live in HBM:
+//
+// 1. The reference layout, bit-exact with build_inbound_from_csr
+//    (Easy-Graph/gpu_easygraph/functions/centrality/pagerank.cu:286-347),
+//    n nodes and m stored inbound arcs:
+//      in_row  int32[n+1]  exclusive scan of in-degree            (:318-319)
+//      in_col  int32[m]    sources, ascending within each row     (:328-346)
+//      in_val  f32[m]      fl32(alpha*(w/outW[src])), weighted    (:344)
+//      w_csc   f64[m]      sanitised weight per slot, weighted    (:311-312)
+//      out_w   f64[n]      sanitised out-weight sum               (:311-313)
+//      out_cnt int32[n]    valid out-arcs per node
+//      deg     f32[n]      unweighted: out-degree; weighted: 1/0; 0 == dangling
+//
+// 2. The iteration layout that the SpMV streams: the same graph with
+//    vertices relabelled by descending out-degree (ties by id), so the
+//    sources that most arcs gather from sit in a short prefix of the
+//    gathered vector and stay resident in each XCD's 4 MB L2.  perm[r] is
+//    the original id of relabelled row r, rank[v] its inverse.  it_row /
+//    it_col / it_val / it_w / it_outw / it_deg are the relabelled arrays.
+//    With relabelling off, the iteration layout aliases the reference one.
+//
+// Bins index rows of the iteration layout, where hub rows (in-degree above
+// the hub threshold) come first and every other row follows:
+//      hsegs   HubSeg[]    fixed-size segments of hub rows, hrows HubRow[]
+//      tiles   int32[]     merge-path split of the other rows: tile k holds the
+//                          rows whose (row + arc) diagonal starts in
+//                          [k*tile_items, (k+1)*tile_items)
+// Solve workspace: x[2], s[2] iterate and contribution (x/deg) vectors,
+// double buffered, sized for FP64; partials (3 x grid, SoA), hub_part, out.
+#pragma once
+
+#include <mutex>
+#include <vector>
+
+#include "egx_common.h"
+
+namespace egx {
+
+struct HubSeg {
+    int32_t row;    // destination row
+    int32_t beg;    // first CSC slot of the segment
+    int32_t end;    // one past the last slot
+    int32_t slot;   // hub row index
+    int32_t nseg;   // segments of this row
+    int32_t first;  // index of the row's first segment in hub_part
+};
+
+struct HubRow {
+    int32_t row;    // destination row
+    int32_t first;  // first segment in hub_part
+    int32_t nseg;   // number of segments
+    int32_t pad;
+};
+
+struct PrState {
+    double dang;      // dangling mass of the current iterate
+    double err;       // L1(x' - x) of the last iteration
+    double sum;       // sum of the last iterate
+    int32_t iter;     // iterations performed
+    int32_t done;     // stop flag, set on the device
+    int32_t pad0;
+    int32_t pad1;
+};
+
+// LDS entries (of the iterate type) a tile may stage: tile_items + hub
+// threshold must not exceed it.
+constexpr int32_t kTileLds = 7936;
+
+struct BinCfg {
+    int32_t thread_row_max = 12;  // tiles with mean row length <= this: thread per row, else wavefront per row
+    int32_t wave_row_max = 2048;  // hub threshold: rows with more arcs go to the hub bin
+    int32_t hub_segment = 8192;   // arcs per hub segment (one workgroup)
+    int32_t tile_items = 5888;    // rows + arcs per tile (merge-path split)
+    int32_t reorder = 1;          // relabel by out-degree (hub rows always first)
+};
+
+struct Graph {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;
+
+    int32_t n = 0;
+    int64_t m = 0;          // stored inbound arcs
+    int64_t n_dangling = 0;
+    bool weighted = false;
+    double alpha_vals = 0.0;   // alpha baked into it_val
+    double alpha_orig = 0.0;   // alpha baked into in_val
+    int32_t max_in_degree = 0;
+    int32_t max_out_degree = 0;
+    double build_seconds = 0.0;
+    double layout_seconds = 0.0;
+
+    // reference layout
+    DevBuf in_row, in_col, in_val, w_csc, out_w, out_cnt, deg;
+    DevBuf csr_V, csr_E, csr_W;  // kept out-CSR (keep_csr)
+    int64_t csr_m = 0;
+    bool have_csr = false;
+
+    // iteration layout
+    bool reordered = false;
+    DevBuf perm, rank;
+    DevBuf p_row, p_col, p_val, p_w, p_outw, p_deg;
+    const int32_t* it_row = nullptr;
+    const int32_t* it_col = nullptr;
+    float* it_val = nullptr;
+    const double* it_w = nullptr;
+    const double* it_outw = nullptr;
+    const float* it_deg = nullptr;
+
+    BinCfg bins;
+    bool layout_built = false;
+    int32_t layout_hub = -1;      // hub threshold the layout was built for
+    int32_t layout_reorder = -1;
+    int32_t n_hub = 0;            // rows [0, n_hub) of the iteration layout are hub rows
+    DevBuf tiles;
+    int64_t n_tiles = 0;
+    DevBuf hsegs, hrows, hub_part;
+    int64_t n_hseg = 0, n_hrows = 0;
+    int64_t rows_thread = 0, rows_wave = 0;
+
+    // solve workspace (allocated on first use, sized for FP64)
+    DevBuf x[2], s[2], state, partials, out;
+    int64_t partial_blocks = 0;
+    void* host_state = nullptr;  // pinned PrState
+
+    ~Graph();
+    size_t device_bytes() const;
+};
+
+// egx_build.hip
+int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const double* dW, int32_t n,
+                          int64_t m, double alpha);
+int build_layout(Graph& g, const BinCfg& cfg);
+int rebuild_bins(Graph& g, const BinCfg& cfg);
+int refresh_alpha(Graph& g, double alpha);
+int download_csc(Graph& g, int32_t* in_row, int32_t* in_col, float* in_val, int32_t* dangling01);
+
+// egx_gen.hip: device generators producing an out-CSR (V, E, W) in device memory.
+int gen_rmat_csr(int32_t scale, int64_t nedges, uint64_t seed, uint32_t tA, uint32_t tAB, uint32_t tABC,
+                 DevBuf& V, DevBuf& E, int64_t& m, hipStream_t s);
+int gen_er_csr(int32_t n, int64_t nedges, uint64_t seed, DevBuf& V, DevBuf& E, int64_t& m, hipStream_t s);
+int gen_edge_weights(const int32_t* dV, const int32_t* dE, int32_t n, int64_t m, uint64_t wseed, double lo,
+                     double hi, DevBuf& W, hipStream_t s);
+
+// egx_pagerank.hip
+struct SolveParams {
+    double alpha = 0.85;
+    int32_t max_iter = 200;
+    double tol = 1e-6;
+    int32_t precision = EGX_PRECISION_FP32;
+    int32_t check_every = 0;
+    bool time_each_launch = false;  // hipEvent pair around every launch
+};
+struct SolveStats {
+    double kernel_seconds = 0.0;  // first to last iteration launch
+    double spmv_seconds = 0.0;    // sum of k_pr_spmv launch durations
+    double reduce_seconds = 0.0;  // sum of k_pr_reduce launch durations
+    int32_t iterations = 0;
+    int32_t spmv_launches = 0;
+    int64_t spmv_grid = 0;
+};
+int solve_pagerank(Graph& g, const SolveParams& p, double* host_out, SolveStats* stats);
+
+}  // namespace egx

@@ -1,0 +1,375 @@
+"""ctypes binding of ``libegx.so`` (the C ABI in ``include/egx.h``).
+
+This module is the only door to the GPU.  It raises
+:class:`NativeLibraryMissing` when the library has not been built; nothing in
+the package computes PageRank on the CPU instead.
+
+It also exposes the dict-returning shim the backend calls, mirroring
+``cpp_easygraph.cpp_gpu_pagerank_dense`` (``cpp_easygraph.cpp:120``,
+``functions/pagerank/pagerank.cpp:176-223``): ``{"values_dense": float64[n],
+"kernel_seconds": float}``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libegx.so")
+
+EGX_SUCC = 0
+EGX_FAILED_TO_ALLOCATE_HOST_MEM = 1
+EGX_FAILED_TO_ALLOCATE_DEVICE_MEM = 2
+EGX_DEVICE_ERR = 3
+EGX_UNKNOW_ERROR = 4
+EGX_UNSUPPORTED_GRAPH = 5
+EGX_NEGATIVE_CYCLE = 6
+
+PRECISION_FP32 = 0
+PRECISION_FP64 = 1
+
+
+class NativeLibraryMissing(RuntimeError):
+    """libegx.so is absent or failed to load: the HIP path cannot run."""
+
+
+class EgxError(RuntimeError):
+    """A non-zero status from the C ABI (``RuntimeError`` like the
+    reference's ``pybind11_fail``, ``pagerank.cpp:205-207``)."""
+
+    def __init__(self, status, detail, last):
+        msg = detail if not last else f"{detail} ({last})"
+        super().__init__(msg)
+        self.status = status
+
+
+class GraphInfo(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("m", ctypes.c_int64),
+        ("n_dangling", ctypes.c_int64),
+        ("weighted", ctypes.c_int32),
+        ("max_in_degree", ctypes.c_int32),
+        ("max_out_degree", ctypes.c_int32),
+        ("thread_row_max", ctypes.c_int32),
+        ("wave_row_max", ctypes.c_int32),
+        ("hub_segment", ctypes.c_int32),
+        ("tile_items", ctypes.c_int32),
+        ("reordered", ctypes.c_int32),
+        ("hub_rows_first", ctypes.c_int32),
+        ("rows_thread", ctypes.c_int64),
+        ("rows_wave", ctypes.c_int64),
+        ("rows_hub", ctypes.c_int64),
+        ("hub_segments", ctypes.c_int64),
+        ("tiles", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+        ("build_seconds", ctypes.c_double),
+        ("layout_seconds", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BinOpts(ctypes.Structure):
+    _fields_ = [
+        ("thread_row_max", ctypes.c_int32),
+        ("wave_row_max", ctypes.c_int32),
+        ("hub_segment", ctypes.c_int32),
+        ("tile_items", ctypes.c_int32),
+        ("reorder", ctypes.c_int32),
+    ]
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [
+        ("alpha", ctypes.c_double),
+        ("max_iter", ctypes.c_int32),
+        ("tol", ctypes.c_double),
+        ("precision", ctypes.c_int32),
+        ("check_every", ctypes.c_int32),
+    ]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("kernel_seconds", ctypes.c_double),
+        ("spmv_seconds", ctypes.c_double),
+        ("reduce_seconds", ctypes.c_double),
+        ("iterations", ctypes.c_int32),
+        ("spmv_launches", ctypes.c_int32),
+        ("spmv_grid", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class CacheStats(ctypes.Structure):
+    _fields_ = [
+        ("hits", ctypes.c_uint64),
+        ("misses", ctypes.c_uint64),
+        ("builds", ctypes.c_uint64),
+        ("value_refreshes", ctypes.c_uint64),
+        ("resident_bytes", ctypes.c_uint64),
+    ]
+
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); every function declared in include/egx.h
+SIGNATURES = {
+    "egx_err_code_detail": (ctypes.c_char_p, [ctypes.c_int]),
+    "egx_last_error": (ctypes.c_char_p, []),
+    "egx_version": (ctypes.c_char_p, []),
+    "egx_device_count": (ctypes.c_int, []),
+    "egx_pagerank": (ctypes.c_int, [_i32p, _i32p, _f64p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                    ctypes.c_int32, ctypes.c_double, ctypes.c_uint64, _f64p, _i32p, _f64p]),
+    "egx_pagerank_cache_reset": (None, []),
+    "egx_pagerank_cache_stats": (ctypes.c_int, [ctypes.POINTER(CacheStats)]),
+    "egx_graph_from_csr": (ctypes.c_int, [_i32p, _i32p, _f64p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                          ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "egx_graph_rmat": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int32,
+                                      ctypes.POINTER(_vp)]),
+    "egx_graph_er": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int32,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
+                                    ctypes.c_int32, ctypes.POINTER(_vp)]),
+    "egx_graph_info_get": (ctypes.c_int, [_vp, ctypes.POINTER(GraphInfo)]),
+    "egx_graph_set_bins": (ctypes.c_int, [_vp, ctypes.POINTER(BinOpts)]),
+    "egx_graph_download_csc": (ctypes.c_int, [_vp, _i32p, _i32p, _f32p, _i32p]),
+    "egx_graph_download_csr": (ctypes.c_int, [_vp, _i32p, _i32p, _f64p, _i64p]),
+    "egx_graph_pagerank": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, _i32p, _f64p]),
+    "egx_graph_pagerank_ex": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, ctypes.POINTER(SolveStats),
+                                             ctypes.c_int32]),
+    "egx_graph_free": (None, [_vp]),
+}
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    """Load libegx.so once; raise NativeLibraryMissing if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(
+                f"{LIB_PATH} is not built; run `python -m eggpu_amd.build` (hipcc, gfx950)"
+            )
+        try:
+            handle = ctypes.CDLL(LIB_PATH)
+        except OSError as exc:
+            raise NativeLibraryMissing(f"cannot load {LIB_PATH}: {exc}") from exc
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+        return _lib
+
+
+def err_code_detail(status):
+    return lib().egx_err_code_detail(int(status)).decode()
+
+
+def _check(rc):
+    if rc != EGX_SUCC:
+        L = lib()
+        raise EgxError(rc, L.egx_err_code_detail(rc).decode(), L.egx_last_error().decode())
+
+
+def device_count():
+    return int(lib().egx_device_count())
+
+
+def _ptr(arr, ptype):
+    return None if arr is None else arr.ctypes.data_as(ptype)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DeviceGraph:
+    """A graph resident in HBM: pull CSC, bins and solve workspace.
+
+    Prepared once and reused across calls, the role the reference's
+    thread-local PagerankWorkspace plays (``pagerank.cu:44-195``)."""
+
+    def __init__(self, handle, keep=None):
+        self._h = _vp(handle)
+        self._keep = keep
+        self._closed = False
+
+    # -- construction ---------------------------------------------------
+    @classmethod
+    def from_csr(cls, V, E, W=None, alpha=0.85, keep_csr=False):
+        V = _i32(V)
+        E = _i32(E)
+        W = _f64(W)
+        n = len(V) - 1
+        h = _vp()
+        _check(lib().egx_graph_from_csr(_ptr(V, _i32p), _ptr(E, _i32p), _ptr(W, _f64p), n, len(E),
+                                        float(alpha), int(bool(keep_csr)), ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, weighted=False, wlo=0.1, whi=1.0,
+             wseed=0, alpha=0.85, keep_csr=False):
+        h = _vp()
+        _check(lib().egx_graph_rmat(int(scale), int(edge_factor), float(a), float(b), float(c),
+                                    int(seed) & (2**64 - 1), int(bool(weighted)), float(wlo), float(whi),
+                                    int(wseed) & (2**64 - 1), float(alpha), int(bool(keep_csr)),
+                                    ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def er(cls, n, nedges, seed=1, weighted=False, wlo=0.1, whi=1.0, wseed=0, alpha=0.85, keep_csr=False):
+        h = _vp()
+        _check(lib().egx_graph_er(int(n), int(nedges), int(seed) & (2**64 - 1), int(bool(weighted)),
+                                  float(wlo), float(whi), int(wseed) & (2**64 - 1), float(alpha),
+                                  int(bool(keep_csr)), ctypes.byref(h)))
+        return cls(h.value)
+
+    # -- queries ----------------------------------------------------------
+    @property
+    def handle(self):
+        if self._closed:
+            raise ValueError("device graph is closed")
+        return self._h
+
+    def info(self):
+        out = GraphInfo()
+        _check(lib().egx_graph_info_get(self.handle, ctypes.byref(out)))
+        return out.as_dict()
+
+    @property
+    def n(self):
+        return int(self.info()["n"])
+
+    def set_bins(self, thread_row_max=0, wave_row_max=0, hub_segment=0, tile_items=0, reorder=-1):
+        """Re-bin rows (values <= 0 keep the current setting; reorder < 0 keeps it)."""
+        o = BinOpts(int(thread_row_max), int(wave_row_max), int(hub_segment), int(tile_items), int(reorder))
+        _check(lib().egx_graph_set_bins(self.handle, ctypes.byref(o)))
+
+    def download_csc(self):
+        inf = self.info()
+        n, m = int(inf["n"]), int(inf["m"])
+        in_row = np.empty(n + 1, dtype=np.int32)
+        in_col = np.empty(max(m, 1), dtype=np.int32)
+        in_val = np.empty(max(m, 1), dtype=np.float32)
+        dang = np.empty(max(n, 1), dtype=np.int32)
+        _check(lib().egx_graph_download_csc(self.handle, _ptr(in_row, _i32p), _ptr(in_col, _i32p),
+                                            _ptr(in_val, _f32p), _ptr(dang, _i32p)))
+        return in_row, in_col[:m], in_val[:m], dang[:n]
+
+    def download_csr(self, with_weights=True):
+        inf = self.info()
+        n = int(inf["n"])
+        m = ctypes.c_int64(0)
+        _check(lib().egx_graph_download_csr(self.handle, None, None, None, ctypes.byref(m)))
+        V = np.empty(n + 1, dtype=np.int32)
+        E = np.empty(max(m.value, 1), dtype=np.int32)
+        W = np.empty(max(m.value, 1), dtype=np.float64) if (with_weights and inf["weighted"]) else None
+        _check(lib().egx_graph_download_csr(self.handle, _ptr(V, _i32p), _ptr(E, _i32p), _ptr(W, _f64p),
+                                            ctypes.byref(m)))
+        return V, E[: m.value], (None if W is None else W[: m.value])
+
+    # -- solves -------------------------------------------------------------
+    def pagerank(self, alpha=0.85, max_iter=200, tol=1e-6, precision=PRECISION_FP32, check_every=0,
+                 download=True):
+        """Returns (float64[n] ranks or None, iterations, kernel_seconds)."""
+        n = self.n
+        out = np.empty(n, dtype=np.float64) if download else None
+        it = ctypes.c_int32(0)
+        ks = ctypes.c_double(0.0)
+        o = SolveOpts(float(alpha), int(max_iter), float(tol), int(precision), int(check_every))
+        _check(lib().egx_graph_pagerank(self.handle, ctypes.byref(o), _ptr(out, _f64p), ctypes.byref(it),
+                                        ctypes.byref(ks)))
+        return out, int(it.value), float(ks.value)
+
+    def pagerank_stats(self, alpha=0.85, max_iter=200, tol=1e-6, precision=PRECISION_FP32, time_each_launch=True,
+                       download=False):
+        """Like :meth:`pagerank` but returns (ranks or None, stats dict) with
+        per-launch kernel timings (hipEvents on the graph's stream)."""
+        n = self.n
+        out = np.empty(n, dtype=np.float64) if download else None
+        st = SolveStats()
+        o = SolveOpts(float(alpha), int(max_iter), float(tol), int(precision), 0)
+        _check(lib().egx_graph_pagerank_ex(self.handle, ctypes.byref(o), _ptr(out, _f64p), ctypes.byref(st),
+                                           int(bool(time_each_launch))))
+        return out, st.as_dict()
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self):
+        if not self._closed and self._h is not None and _lib is not None:
+            _lib.egx_graph_free(self._h)
+        self._closed = True
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def pagerank_csr(V, E, W=None, alpha=0.85, max_iter=200, tol=1e-6, graph_id=0):
+    """``egx_pagerank``: the drop-in for ``cuda_pagerank`` with the per-thread
+    prepared-graph cache.  Returns (float64[n], iterations, kernel_seconds)."""
+    V = _i32(V)
+    E = _i32(E)
+    W = _f64(W)
+    n = len(V) - 1
+    out = np.empty(max(n, 0), dtype=np.float64)
+    it = ctypes.c_int32(0)
+    ks = ctypes.c_double(0.0)
+    _check(lib().egx_pagerank(_ptr(V, _i32p), _ptr(E, _i32p), _ptr(W, _f64p), n, len(E), float(alpha),
+                              int(max_iter), float(tol), int(graph_id) & (2**64 - 1), _ptr(out, _f64p),
+                              ctypes.byref(it), ctypes.byref(ks)))
+    return out, int(it.value), float(ks.value)
+
+
+def cache_stats():
+    s = CacheStats()
+    _check(lib().egx_pagerank_cache_stats(ctypes.byref(s)))
+    return {name: int(getattr(s, name)) for name, _ in s._fields_}
+
+
+def cache_reset():
+    lib().egx_pagerank_cache_reset()
+
+
+def cpp_gpu_pagerank_dense(graph, alpha=0.85, max_iterator=200, threshold=1e-6, weight=None):
+    """Shim with the keys of ``cpp_gpu_pagerank_dense`` (``pagerank.cpp:176-223``).
+
+    ``graph`` is a :class:`DeviceGraph` prepared by the backend; ``weight`` is
+    accepted for signature compatibility (weights were fixed at build)."""
+    values, iters, ks = graph.pagerank(alpha=alpha, max_iter=max_iterator, tol=threshold)
+    return {"values_dense": values, "kernel_seconds": ks, "iterations": iters}

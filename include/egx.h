@@ -1,0 +1,188 @@
+/*
+ * egx.h -- C ABI of the MI355X-native PageRank library (libegx.so).
+ *
+ * Plain pointers and sizes only.  Every entry point returns a status code
+ * whose numeric values are those of EG_GPU_STATUS_CODE
+ * (Easy-Graph/gpu_easygraph/common/err.h:7-15); no exception crosses the ABI.
+ * Host pointers are borrowed for the duration of the call; device-resident
+ * graphs (egx_graph) are owned by the library until egx_graph_free().
+ */
+#ifndef EGX_H
+#define EGX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* EG_GPU_STATUS_CODE, Easy-Graph/gpu_easygraph/common/err.h:7-15 */
+#define EGX_SUCC 0
+#define EGX_FAILED_TO_ALLOCATE_HOST_MEM 1
+#define EGX_FAILED_TO_ALLOCATE_DEVICE_MEM 2
+#define EGX_DEVICE_ERR 3
+#define EGX_UNKNOW_ERROR 4
+#define EGX_UNSUPPORTED_GRAPH 5
+#define EGX_NEGATIVE_CYCLE 6
+
+/* Iterate storage of a solve. */
+#define EGX_PRECISION_FP32 0 /* FP32 ranks and weights, FP64 dangling/L1/teleport */
+#define EGX_PRECISION_FP64 1 /* FP64 ranks (weights stay FP32, as in the build) */
+
+typedef struct egx_graph egx_graph; /* opaque, device resident */
+
+/* Replaces gpu_easygraph::err_code_detail (Easy-Graph/gpu_easygraph/common/err.cpp:9-31):
+ * same strings, "EasyGraph GPU: ..." prefix. */
+const char *egx_err_code_detail(int status);
+
+/* Detail text of the last failure on the calling thread ("" if none). */
+const char *egx_last_error(void);
+
+/* Library version string. */
+const char *egx_version(void);
+
+/* Number of visible devices (0 when no GPU); never fails. */
+int egx_device_count(void);
+
+/*
+ * Replaces cuda_pagerank (Easy-Graph/gpu_easygraph/functions/centrality/pagerank.cuh:7-18)
+ * and the C++ wrapper gpu_easygraph::pagerank (Easy-Graph/gpu_easygraph/gpu_easygraph.h:95-104).
+ *
+ * V[len_V+1], E[len_E]: out-CSR of the graph (Graph::gen_CSR order).  W[len_E]
+ * edge weights or NULL for unit weights.  len_V is the node count (the
+ * reference wrapper passes V.size()-1).  graph_id plays the part of
+ * active_device_graph_id() (device_graph_cache.h:34-48): the prepared device
+ * graph is cached per host thread and reused while graph_id, the V/E/W
+ * pointers, n, m and the first/last entries stay the same
+ * (pagerank.cu:140-167); alpha no longer forces a rebuild, only a cheap
+ * device recomputation of the normalised weights.
+ * PR[len_V] (caller owned) receives sum-normalised FP64 ranks.  iterations and
+ * kernel_seconds may be NULL.  Offsets must be monotonic with V[0]=0 and
+ * V[len_V]=len_E (EGX_UNSUPPORTED_GRAPH otherwise); endpoints outside
+ * [0, len_V) are skipped and weights that are negative or not finite count as
+ * 0, as in pagerank.cu:282-312.  A thread that switches device while its
+ * cache lives gets EGX_DEVICE_ERR (pagerank.cu:373-377).
+ */
+int egx_pagerank(const int32_t *V, const int32_t *E, const double *W, int32_t len_V,
+                 int32_t len_E, double alpha, int32_t max_iter_num, double threshold,
+                 uint64_t graph_id, double *PR, int32_t *iterations, double *kernel_seconds);
+
+/* Device-CSR cache control, after cpp_gpu_device_csr_cache_reset/_stats
+ * (Easy-Graph/cpp_easygraph/cpp_easygraph.cpp:97-113, device_graph_cache.cu:519-527). */
+typedef struct {
+    uint64_t hits;
+    uint64_t misses;
+    uint64_t builds;
+    uint64_t value_refreshes; /* alpha changes served without a rebuild */
+    uint64_t resident_bytes;
+} egx_cache_stats;
+void egx_pagerank_cache_reset(void);
+int egx_pagerank_cache_stats(egx_cache_stats *out);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident graphs.  No counterpart in the reference, which keeps one  */
+/* thread-local workspace (pagerank.cu:44-132); these give the serving pool,  */
+/* the batch engines and benchmarks an explicit handle.                       */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int64_t n;               /* nodes */
+    int64_t m;               /* stored inbound arcs (valid endpoints) */
+    int64_t n_dangling;      /* nodes whose sanitised out-weight is 0 */
+    int32_t weighted;        /* 1 when explicit weights were given */
+    int32_t max_in_degree;
+    int32_t max_out_degree;
+    int32_t thread_row_max;  /* bin settings in use (egx_bin_opts) */
+    int32_t wave_row_max;
+    int32_t hub_segment;
+    int32_t tile_items;
+    int32_t reordered;       /* 1: iteration layout relabelled by out-degree */
+    int32_t hub_rows_first;  /* hub rows, stored first in the iteration layout */
+    int64_t rows_thread;     /* rows reduced one thread per row */
+    int64_t rows_wave;       /* rows reduced one wavefront per row */
+    int64_t rows_hub;        /* rows of the hub bin (segmented, one workgroup per segment) */
+    int64_t hub_segments;
+    int64_t tiles;           /* LDS-staged tiles of non-hub rows */
+    int64_t device_bytes;
+    double build_seconds;    /* whole device build, host clock */
+    double layout_seconds;   /* relabelling part of it (hipEvents) */
+} egx_graph_info;
+
+/* Degree bins of the SpMV.  Rows with in-degree above wave_row_max form the
+ * hub bin (cut into hub_segment-arc segments, one workgroup each); the other
+ * rows are split into tiles of tile_items rows+arcs whose gathered terms are
+ * staged in LDS and reduced one thread per row when the tile's mean row
+ * length is at most thread_row_max, else one wavefront per row.
+ * tile_items + wave_row_max <= 7936. */
+typedef struct {
+    int32_t thread_row_max;
+    int32_t wave_row_max;
+    int32_t hub_segment;
+    int32_t tile_items;
+    int32_t reorder;        /* 1: relabel vertices by out-degree, 0: keep ids, <0: keep setting */
+} egx_bin_opts;
+
+typedef struct {
+    double alpha;
+    int32_t max_iter;       /* clamped to >= 1, as pagerank.py:45-48 */
+    double tol;             /* stop when L1(x' - x) < tol * n, pagerank.py:131 */
+    int32_t precision;      /* EGX_PRECISION_* */
+    int32_t check_every;    /* 0: adaptive host checks of the device stop flag */
+} egx_solve_opts;
+
+/* Build from a host out-CSR (same contract as egx_pagerank). keep_csr keeps
+ * the out-CSR on the device so egx_graph_download_csr can return it. */
+int egx_graph_from_csr(const int32_t *V, const int32_t *E, const double *W, int32_t n,
+                       int32_t m, double alpha, int32_t keep_csr, egx_graph **out);
+
+/* Seeded directed R-MAT generated and built on the device: 2^scale nodes,
+ * edge_factor * 2^scale draws, self-loops and duplicates removed.  When
+ * weighted, w = fl32(wlo + (whi - wlo) * u24 * 2^-24) per arc. */
+int egx_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                   uint64_t seed, int32_t weighted, double wlo, double whi, uint64_t wseed,
+                   double alpha, int32_t keep_csr, egx_graph **out);
+
+/* Seeded Erdos-Renyi G(n, M) draw (M pairs, self-loops and duplicates removed). */
+int egx_graph_er(int32_t n, int64_t nedges, uint64_t seed, int32_t weighted, double wlo,
+                 double whi, uint64_t wseed, double alpha, int32_t keep_csr, egx_graph **out);
+
+int egx_graph_info_get(const egx_graph *g, egx_graph_info *out);
+
+/* Re-bin rows for the SpMV kernels (autotuner hook). Fields <= 0 keep the current value. */
+int egx_graph_set_bins(egx_graph *g, const egx_bin_opts *opts);
+
+/* Pull CSR (the reference's in_row/in_col/in_val/dangling01, pagerank.cu:286-347)
+ * for the alpha of the last build/solve. Arrays sized n+1, m, m, n; any may be NULL. */
+int egx_graph_download_csc(const egx_graph *g, int32_t *in_row, int32_t *in_col, float *in_val,
+                           int32_t *dangling01);
+
+/* Out-CSR kept by keep_csr (V[n+1], E[m_out], W[m_out] or NULL). */
+int egx_graph_download_csr(const egx_graph *g, int32_t *V, int32_t *E, double *W,
+                           int64_t *m_out);
+
+/* Global PageRank on a device-resident graph. PR[n] may be NULL (keep the
+ * result on the device).  kernel_seconds covers the iteration launches only
+ * (hipEvents, as pagerank.cu:578,666-676). */
+int egx_graph_pagerank(egx_graph *g, const egx_solve_opts *opts, double *PR, int32_t *iterations,
+                       double *kernel_seconds);
+
+/* Solve statistics; spmv/reduce sums are filled when time_each_launch != 0
+ * (a hipEvent pair around every launch on the graph's stream). */
+typedef struct {
+    double kernel_seconds;  /* first to last iteration launch */
+    double spmv_seconds;    /* sum of the SpMV launch durations (iterations performed) */
+    double reduce_seconds;  /* sum of the reduce launch durations */
+    int32_t iterations;
+    int32_t spmv_launches;
+    int64_t spmv_grid;      /* workgroups per SpMV launch */
+} egx_solve_stats;
+
+int egx_graph_pagerank_ex(egx_graph *g, const egx_solve_opts *opts, double *PR, egx_solve_stats *stats,
+                          int32_t time_each_launch);
+
+void egx_graph_free(egx_graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EGX_H */

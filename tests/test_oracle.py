@@ -1,0 +1,287 @@
+"""Pins the CPU oracle (oracle/egx_oracle.c) before anything is checked against it.
+
+Each oracle function is checked against something that does not share its
+code: networkx 3.4.2 (golden fixtures made by tests/golden/make_golden.py),
+the EasyGraph README known-answer vector, closed-form linear solves of the
+PageRank fixed point, scipy's CSR transpose, and invariants.  A perturbed
+oracle call shows the checks are tight enough to catch a wrong constant.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from eggpu_amd.classes import DiGraph, Graph
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _load_small():
+    with open(os.path.join(GOLDEN, "golden_small.json")) as fh:
+        return json.load(fh)
+
+
+def _graph_from_case(case):
+    G = DiGraph() if case["directed"] else Graph()
+    if case["nodes"] is not None:
+        G.add_nodes_from(case["nodes"])
+    for e in case["edges"]:
+        if len(e) == 3:
+            G.add_edge(e[0], e[1], **{case["weight"]: e[2]})
+        else:
+            G.add_edge(e[0], e[1])
+    return G
+
+
+def _fixed_point(n, src, dst, w, alpha, p=None):
+    """Exact solution of x = alpha*(P^T x + dang(x) p) + (1-alpha) p, sum x = 1,
+    by a dense linear solve (P row-normalised, dangling rows spread by p)."""
+    p = np.full(n, 1.0 / n) if p is None else np.asarray(p, float)
+    A = np.zeros((n, n))
+    wts = np.ones(len(src)) if w is None else np.asarray(w, float)
+    for s_, d_, w_ in zip(src, dst, wts):
+        A[s_, d_] += w_
+    out = A.sum(axis=1)
+    dang = out <= 1e-15
+    P = np.zeros_like(A)
+    P[~dang] = A[~dang] / out[~dang, None]
+    # G x = x with G = alpha P^T + alpha p d^T + (1-alpha) p 1^T (uses sum x = 1)
+    Gm = alpha * P.T + alpha * np.outer(p, dang.astype(float)) + (1 - alpha) * np.outer(p, np.ones(n))
+    M = np.eye(n) - Gm
+    M[-1, :] = 1.0
+    rhs = np.zeros(n)
+    rhs[-1] = 1.0
+    return np.linalg.solve(M, rhs)
+
+
+# -- pagerank.py:75-139 ------------------------------------------------------
+def test_readme_known_answer_vector():
+    """Easy-Graph/README.md:168-173 holds the converged fixed point."""
+    d = _load_small()
+    G = Graph()
+    G.add_edges_from([tuple(e) for e in d["readme_edges"]])
+    vals, it = O.pagerank_graph(G, alpha=0.85, max_iter=10000, tol=1e-15)
+    kat = {int(k): v for k, v in d["readme_kat"].items()}
+    assert max(abs(vals[k] - kat[k]) for k in kat) < 1e-12
+    # the default tol=1e-6 result is the unconverged iterate, 23 iterations
+    vals6, it6 = O.pagerank_graph(G)
+    assert it6 == 23
+    assert max(abs(vals6[k] - kat[k]) for k in kat) < 2e-6
+
+
+def test_readme_planted_mistake_is_caught():
+    """A wrong damping factor (0.8499 for 0.85) must fail the KAT check."""
+    d = _load_small()
+    G = Graph()
+    G.add_edges_from([tuple(e) for e in d["readme_edges"]])
+    vals, _ = O.pagerank_graph(G, alpha=0.8499, max_iter=10000, tol=1e-15)
+    kat = {int(k): v for k, v in d["readme_kat"].items()}
+    assert max(abs(vals[k] - kat[k]) for k in kat) > 1e-7
+
+
+@pytest.mark.parametrize("case_idx", range(11))
+def test_networkx_golden_cases(case_idx):
+    case = _load_small()["cases"][case_idx]
+    G = _graph_from_case(case)
+    vals, it = O.pagerank_graph(G, alpha=case["alpha"], weight=case["weight"], max_iter=case["max_iter"],
+                                tol=case["tol"])
+    nxv = np.asarray(case["nx_values"])
+    nxv = nxv / nxv.sum()  # networkx skips the final sum-normalisation
+    ours = np.asarray([vals[v] for v in case["nodes"]])
+    assert np.abs(ours - nxv).max() < 1e-12, case["name"]
+    assert it == case["nx_iterations"], case["name"]
+
+
+@pytest.mark.parametrize("seed,weighted", [(1, False), (2, True), (3, False), (4, True)])
+def test_fixed_point_closed_form(seed, weighted):
+    rng = np.random.default_rng(seed)
+    n = 40
+    m = 150
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    keep = src != dst
+    pairs = np.unique(np.stack([src[keep], dst[keep]], 1), axis=0)
+    src, dst = pairs[:, 0].astype(np.int32), pairs[:, 1].astype(np.int32)
+    w = rng.uniform(0.1, 3.0, len(src)) if weighted else None
+    ours, it, _ = O.pagerank_rows(n, src, dst, w, alpha=0.85, max_iter=100000, tol=1e-16)
+    exact = _fixed_point(n, src, dst, w, 0.85)
+    assert np.abs(ours - exact).max() < 1e-12
+
+
+def test_special_cases():
+    assert O.pagerank_graph(DiGraph())[0] == {}
+    G = DiGraph()
+    G.add_node("solo")
+    assert O.pagerank_graph(G)[0] == {"solo": 1.0}  # test_pagerank.py:75-77
+    G = DiGraph()
+    G.add_nodes_from(range(5))
+    vals, it = O.pagerank_graph(G)
+    assert all(v == 0.2 for v in vals.values()) and it == 0  # pagerank.py:93-95
+    G = DiGraph()
+    G.add_edges_from([(i, (i + 1) % 7) for i in range(7)])  # directed cycle: uniform, one iteration
+    vals, it = O.pagerank_graph(G)
+    assert it == 1 and max(abs(v - 1 / 7) for v in vals.values()) < 1e-15
+    G = DiGraph()
+    G.add_edge(0, 0)
+    assert O.pagerank_graph(G)[0] == {0: 1.0}
+
+
+def test_mixed_node_types_follow_insertion_order():
+    # test_pagerank.py:10-18 node types; the oracle keys results by node
+    G = DiGraph()
+    G.add_edges_from([(1, 2), (2, 3), ("String", "Bool"), (2, 1), (0, 0), ((None, None), (None, None))])
+    vals, _ = O.pagerank_graph(G)
+    assert list(vals.keys()) == list(G.nodes.keys())
+    assert abs(sum(vals.values()) - 1.0) < 1e-12
+
+
+# -- generators ----------------------------------------------------------------
+def test_rmat14_reproduces_fixture_and_networkx():
+    z = np.load(os.path.join(GOLDEN, "rmat14.npz"))
+    V, E = O.rmat_csr(14, 16, seed=1)
+    assert len(E) == int(z["m"])
+    assert hashlib.sha256(V.tobytes()).hexdigest() == str(z["V_sha256"])
+    assert hashlib.sha256(E.tobytes()).hexdigest() == str(z["E_sha256"])
+    pr, it, _ = O.pagerank_csr(V, E, None, 0.85, 100, 1e-6)
+    nxv = z["nx_values"] / z["nx_values"].sum()
+    assert np.abs(pr - nxv).max() < 1e-12
+    assert it == int(z["nx_iterations"]) == 5
+
+
+def test_rmat_quadrant_statistics():
+    """Top-bit pair frequencies of the raw draws follow (a, b, c, d)."""
+    src, dst = O.rmat_coo(16, 400000, seed=9)
+    q = (src >> 15) * 2 + (dst >> 15)
+    freq = np.bincount(q, minlength=4) / len(q)
+    assert np.allclose(freq, [0.57, 0.19, 0.19, 0.05], atol=0.005)
+
+
+def test_rmat_csr_is_simple_and_sorted():
+    V, E = O.rmat_csr(10, 16, seed=3)
+    src, dst = O.csr_rows(V, E)
+    assert (src != dst).all()
+    key = src.astype(np.int64) * 1024 + dst
+    assert (np.diff(key) > 0).all()
+
+
+def test_edge_weights_range_and_determinism():
+    V, E = O.rmat_csr(10, 16, seed=3)
+    w1 = O.edge_weights(V, E, wseed=5)
+    w2 = O.edge_weights(V, E, wseed=5)
+    assert np.array_equal(w1, w2)
+    assert (w1 >= 0.1).all() and (w1 <= 1.0).all()
+    assert np.array_equal(w1, w1.astype(np.float32).astype(np.float64))
+
+
+# -- pagerank.cu:286-347 -------------------------------------------------------
+def _random_csr(rng, n, m, invalid=False, bad_weights=False, dups=True):
+    src = np.sort(rng.integers(0, n, m))
+    dst = rng.integers(0, n, m)
+    if not dups:
+        pairs = np.unique(np.stack([src, dst], 1), axis=0)
+        src, dst = pairs[:, 0], pairs[:, 1]
+    if invalid and len(dst):
+        idx = rng.choice(len(dst), max(1, len(dst) // 10), replace=False)
+        dst[idx] = rng.choice([-1, -5, n, n + 3], len(idx))
+    V = np.zeros(n + 1, np.int32)
+    V[1:] = np.cumsum(np.bincount(src, minlength=n))
+    W = rng.uniform(0.1, 2.0, len(dst))
+    if bad_weights and len(W):
+        W[rng.choice(len(W), max(1, len(W) // 8), replace=False)] = rng.choice([-1.0, np.nan, np.inf, 0.0],
+                                                                                 max(1, len(W) // 8))
+    return V, dst.astype(np.int32), W
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_build_inbound_matches_scipy_transpose(seed):
+    """in_row/in_col equal scipy's CSR->CSC conversion of the valid arcs
+    (a stable counting sort: sources ascending, duplicates in CSR order);
+    in_val sums to alpha over each non-dangling source."""
+    rng = np.random.default_rng(seed)
+    n, m = 50, 400
+    V, E, W = _random_csr(rng, n, m, invalid=(seed % 2 == 1), bad_weights=(seed >= 3))
+    in_row, in_col, in_val, dang, outw = O.build_inbound(V, E, W, alpha=0.85)
+    src, dst = O.csr_rows(V, E)
+    ok = (dst >= 0) & (dst < n)
+    w = np.where(np.isfinite(W) & (W >= 0), W, 0.0)[ok]
+    A = sp.csr_matrix((np.ones(ok.sum()), (src[ok], dst[ok])), shape=(n, n))
+    A.sum_duplicates = lambda: None  # keep duplicate arcs as separate slots
+    T = sp.csr_matrix((np.arange(ok.sum(), dtype=float) + 1, E[ok], V_valid(V, ok)), shape=(n, n)).T.tocsr()
+    assert np.array_equal(in_row, T.indptr.astype(np.int32))
+    assert np.array_equal(in_col, T.indices.astype(np.int32))
+    # values: the slot of each CSC entry is recovered from the stored index
+    slot = T.data.astype(np.int64) - 1
+    vsrc = src[ok][slot]
+    wsum = np.bincount(src[ok], weights=w, minlength=n)
+    assert np.array_equal(dang, (wsum <= 0).astype(np.int32))
+    per_src = np.bincount(vsrc, weights=in_val.astype(np.float64), minlength=n)
+    live = wsum > 0
+    assert np.allclose(per_src[live], 0.85, rtol=1e-5)
+    assert np.abs(outw - wsum).max() < 1e-12
+    assert in_val.dtype == np.float32
+
+
+def V_valid(V, ok):
+    """Row pointer of the CSR restricted to the arcs flagged ok."""
+    n = len(V) - 1
+    src = np.repeat(np.arange(n), np.diff(V))
+    Vv = np.zeros(n + 1, np.int64)
+    Vv[1:] = np.cumsum(np.bincount(src[ok], minlength=n))
+    return Vv
+
+
+def test_build_inbound_clamps_malformed_offsets():
+    """pagerank.cu:302-306 clamps each row's [V[u], V[u+1]) into [0, m]."""
+    V = np.array([0, 5, 3, 6], np.int32)
+    E = np.array([1, 2, 0, 1, 2, 0], np.int32)
+    in_row, in_col, in_val, dang, _ = O.build_inbound(V, E, None, 0.85)
+    # row 0 covers slots 0..4, row 1 is empty (5 > 3), row 2 covers 3..5
+    arcs = sorted([(0, e) for e in E[0:5]] + [(2, e) for e in E[3:6]], key=lambda a: (a[1], a[0]))
+    assert in_row[-1] == 8
+    assert list(in_col) == [a[0] for a in arcs]
+    assert list(dang) == [0, 1, 0]
+
+
+# -- personalised PageRank -----------------------------------------------------
+def test_ppr_matches_networkx_fixture():
+    z = np.load(os.path.join(GOLDEN, "ppr_rmat10.npz"))
+    V, E, W = z["V"], z["E"], z["W"]
+    n = len(V) - 1
+    src, dst = O.csr_rows(V, E)
+    for j, S in enumerate(z["seed_sets"]):
+        S = [int(s) for s in S if s >= 0]
+        p = np.zeros(n)
+        p[S] = 1.0 / len(S)
+        ours, it, _ = O.ppr_rows(n, src, dst, W, p, 0.85, 200, 1e-6)
+        nxv = z["nx_values"][j] / z["nx_values"][j].sum()
+        assert np.abs(ours - nxv).max() < 1e-12
+        assert it == int(z["nx_iterations"][j])
+
+
+def test_ppr_closed_form():
+    rng = np.random.default_rng(11)
+    n = 30
+    src = rng.integers(0, n, 120)
+    dst = rng.integers(0, n, 120)
+    keep = src != dst
+    pairs = np.unique(np.stack([src[keep], dst[keep]], 1), axis=0)
+    src, dst = pairs[:, 0].astype(np.int32), pairs[:, 1].astype(np.int32)
+    p = np.zeros(n)
+    p[[3, 7]] = 0.5
+    ours, _, _ = O.ppr_rows(n, src, dst, None, p, 0.9, 100000, 1e-16)
+    assert np.abs(ours - _fixed_point(n, src, dst, None, 0.9, p)).max() < 1e-12
+
+
+# -- pagerank.cpp:21-122 (CPU baseline) ---------------------------------------
+@pytest.mark.parametrize("weighted", [False, True])
+def test_openmp_baseline_agrees_with_oracle(weighted):
+    V, E = O.rmat_csr(12, 16, seed=4)
+    W = O.edge_weights(V, E, 2) if weighted else None
+    ref, it_ref, _ = O.pagerank_csr(V, E, W, 0.85, 100, 1e-6)
+    omp, it = O.pagerank_omp(V, E, W, 0.85, 100, 1e-6, threads=4)
+    assert it == it_ref
+    assert np.abs(omp / omp.sum() - ref).max() < 1e-12
