@@ -23,7 +23,9 @@ SOURCES = [
     "egx_build.hip",
     "egx_gen.hip",
     "egx_pagerank.hip",
+    "egx_spmm.hip",
     "egx_api.hip",
+    "egx_probe.hip",
 ]
 
 # -packed-fp32-ops: no packed FP32 instructions are generated.

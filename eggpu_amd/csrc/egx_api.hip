@@ -7,10 +7,6 @@
 
 #include "egx_graph.h"
 
-struct egx_graph {
-    egx::Graph g;
-};
-
 namespace egx {
 
 namespace {
@@ -181,7 +177,7 @@ Graph::~Graph() {
 size_t Graph::device_bytes() const {
     const DevBuf* bufs[] = {&in_row, &in_col, &in_val, &w_csc,  &out_w,  &out_cnt, &deg,    &csr_V,
                             &csr_E,  &csr_W,  &perm,   &rank,   &p_row,  &p_col,   &p_val,  &p_w,
-                            &p_outw, &p_deg,  &tiles,  &hsegs,  &hrows,  &hub_part, &x[0],  &x[1],
+                            &p_outw, &p_deg,  &p_nval, &tiles,  &hsegs,  &hrows,  &hub_part, &x[0],  &x[1],
                             &s[0],   &s[1],   &state,  &partials, &out};
     size_t total = 0;
     for (const DevBuf* b : bufs) total += b->bytes;
@@ -301,6 +297,10 @@ int egx_graph_set_bins(egx_graph* gh, const egx_bin_opts* opts) {
     if (opts->thread_row_max > 0) cfg.thread_row_max = opts->thread_row_max;
     if (opts->wave_row_max > 0) cfg.wave_row_max = opts->wave_row_max;
     if (opts->hub_segment > 0) cfg.hub_segment = opts->hub_segment;
+    if (g.open_batches > 0) {
+        egx::set_last_error("cannot re-bin a graph while batch sessions are open on it");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
     if (opts->tile_items > 0) cfg.tile_items = opts->tile_items;
     if (opts->reorder >= 0) cfg.reorder = opts->reorder ? 1 : 0;
     return egx::rebuild_bins(g, cfg);

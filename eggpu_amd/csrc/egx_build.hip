@@ -413,6 +413,7 @@ int build_layout(Graph& g, const BinCfg& cfg) {
     g.p_w.release();
     g.p_outw.release();
     g.p_deg.release();
+    g.p_nval.release();
     g.perm.release();
     g.rank.release();
     g.reordered = false;

@@ -103,6 +103,8 @@ struct Graph {
     bool reordered = false;
     DevBuf perm, rank;
     DevBuf p_row, p_col, p_val, p_w, p_outw, p_deg;
+    DevBuf p_nval;                // w/outW per arc (no alpha), batched solves
+    int32_t open_batches = 0;     // batch sessions reading this graph
     const int32_t* it_row = nullptr;
     const int32_t* it_col = nullptr;
     float* it_val = nullptr;
@@ -165,3 +167,7 @@ struct SolveStats {
 int solve_pagerank(Graph& g, const SolveParams& p, double* host_out, SolveStats* stats);
 
 }  // namespace egx
+
+struct egx_graph {
+    egx::Graph g;
+};

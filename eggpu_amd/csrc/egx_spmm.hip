@@ -1,0 +1,831 @@
+// egx_spmm.hip -- many PageRank vectors on one device-resident graph (SpMM).
+//
+// Up to 64 columns -- personalised (teleport and dangling mass spread over a
+// seed set, networkx 3.4.2 semantics) or global (uniform teleport) -- are
+// iterated together:
+//
+//   x'_j[v] = alpha_j * sum_{u->v} nval[u->v] * x_j[u] + c_j * p_j[v]
+//   c_j     = (1 - alpha_j) + alpha_j * dang_j,  nval = w / outW[u] (FP32)
+//   err_j   = sum_v |x'_j[v] - x_j[v]|,  stop column j when err_j < tol_j*n
+//
+// X is stored row-major, n x 64 FP32: the 64 values of one node are one
+// 256-byte line pair, so each arc costs one coalesced row load shared by
+// all 64 columns.  Lane j of a wavefront owns column j; the wavefront walks
+// the in-arcs of a row, broadcasting each (source, value) pair from the lane
+// that loaded it with v_readlane, 8 row loads in flight.  Row sums, c_j and
+// every reduction are FP64.  Rows are split over wavefronts by a merge-path
+// split of (arcs + 2*rows); hub rows (the prefix of the iteration layout)
+// are cut into segments whose 64-wide FP64 partials the reduce launch adds
+// in segment order.  A column whose stop flag is set is copied through
+// unchanged, so its values do not depend on which other columns share the
+// launch.  Teleport vectors are sparse: tslot[v] indexes a small table of
+// per-column seed weights (-1 when v seeds no column).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "egx_graph.h"
+
+struct egx_batch;
+
+namespace egx {
+namespace {
+
+constexpr int kCols = 64;            // columns per batch (one per lane)
+constexpr int kRowThreads = 256;     // 4 wavefronts
+constexpr int kRowWaves = kRowThreads / kWave;
+constexpr int kRedThreads = 1024;    // reduce launch
+constexpr int kRedWaves = kRedThreads / kWave;
+constexpr int kSpmmHubSeg = 2048;    // arcs per hub segment (one workgroup)
+constexpr int kUnroll = 8;
+
+struct ColState {
+    double dang;     // dangling mass of the current iterate
+    double err;      // L1 change of the last iteration
+    double sum;      // sum of the current iterate
+    double alpha;
+    double tol_n;    // tol * n
+    int32_t iter;
+    int32_t max_iter;
+    int32_t active;  // slot holds a request
+    int32_t done;    // stop flag
+    int32_t uniform; // global PageRank (teleport 1/n everywhere)
+    int32_t pad;
+};
+
+struct BatchCtl {
+    int32_t any_active;  // some column is active and not done
+    int32_t cur;         // X buffer holding the current iterate; flipped by
+                         // each reduce launch that did work, so launches that
+                         // return early leave it in place
+    int32_t pad[2];
+};
+
+struct SpmmArgs {
+    const int32_t* row;
+    const int32_t* col;
+    const float* nval;
+    const float* deg;
+    const int32_t* tslot;
+    const float* ptab;
+    const int32_t* wave_rows;  // merge-path split of rows [n_hub, n) over wavefronts
+    const HubSeg* hsegs;
+    const HubRow* hrows;
+    double* hub_part;          // [nseg][64]
+    float* xbuf[2];            // the two X buffers; ctl->cur selects the input
+    const float* x_in;         // set per launch from ctl->cur
+    float* x_out;
+    ColState* cols;
+    BatchCtl* ctl;
+    double* partials;          // [grid][3][64]
+    int64_t n_hrows;
+    int32_t n;
+    uint32_t nblk_h;
+    uint32_t grid;
+    uint32_t n_waves;
+};
+
+__device__ __forceinline__ int32_t bcast_i(int32_t v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ __forceinline__ float bcast_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// Sum over arcs [beg, end) of nval * x_in[src][lane], in arc order.
+__device__ __forceinline__ double row_dot(const SpmmArgs& a, int32_t beg, int32_t end) {
+    const int lane = lane_id();
+    double acc = 0.0;
+    for (int32_t p0 = beg; p0 < end; p0 += kWave) {
+        const int32_t p = p0 + lane;
+        const bool ok = p < end;
+        const int32_t u_l = ok ? __builtin_nontemporal_load(&a.col[p]) : 0;
+        const float w_l = ok ? __builtin_nontemporal_load(&a.nval[p]) : 0.0f;
+        const int32_t cnt = min(kWave, end - p0);
+        int32_t i = 0;
+        for (; i + kUnroll <= cnt; i += kUnroll) {
+            float g[kUnroll];
+            float w[kUnroll];
+#pragma unroll
+            for (int q = 0; q < kUnroll; ++q) {
+                const int32_t u = bcast_i(u_l, i + q);
+                w[q] = bcast_f(w_l, i + q);
+                g[q] = a.x_in[(size_t)u * kCols + lane];
+            }
+#pragma unroll
+            for (int q = 0; q < kUnroll; ++q) acc += (double)w[q] * (double)g[q];
+        }
+        for (; i < cnt; ++i) {
+            const int32_t u = bcast_i(u_l, i);
+            const float w = bcast_f(w_l, i);
+            acc += (double)w * (double)a.x_in[(size_t)u * kCols + lane];
+        }
+    }
+    return acc;
+}
+
+__device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, double c, int32_t v, double acc,
+                                         double inv_n, double& err, double& sum, double& dang) {
+    const int lane = lane_id();
+    const size_t at = (size_t)v * kCols + lane;
+    const float xo = a.x_in[at];
+    if (!cs.active || cs.done) {
+        a.x_out[at] = xo;
+        return;
+    }
+    double tele;
+    if (cs.uniform) {
+        tele = inv_n;
+    } else {
+        const int32_t ts = a.tslot[v];
+        tele = (ts >= 0) ? (double)a.ptab[(size_t)ts * kCols + lane] : 0.0;
+    }
+    const float xn = (float)(cs.alpha * acc + c * tele);
+    err += fabs((double)xn - (double)xo);
+    sum += (double)xn;
+    if (a.deg[v] == 0.0f) dang += (double)xn;
+    a.x_out[at] = xn;
+}
+
+__global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
+    __shared__ double sh[3][kRowWaves][kCols];
+    if (!a.ctl->any_active) return;
+    const int cur = a.ctl->cur;
+    a.x_in = a.xbuf[cur];
+    a.x_out = a.xbuf[cur ^ 1];
+    const int lane = lane_id();
+    const int w = wave_id();
+    const uint32_t b = blockIdx.x;
+    const ColState cs = a.cols[lane];
+    const double c = (1.0 - cs.alpha) + cs.alpha * cs.dang;
+    const double inv_n = 1.0 / (double)a.n;
+
+    if (b < a.nblk_h) {
+        // hub segment: four wavefronts take contiguous quarters, combined in order
+        const HubSeg sg = a.hsegs[b];
+        const int32_t len = sg.end - sg.beg;
+        const int32_t q0 = sg.beg + (int32_t)(((int64_t)len * w) / kRowWaves);
+        const int32_t q1 = sg.beg + (int32_t)(((int64_t)len * (w + 1)) / kRowWaves);
+        sh[0][w][lane] = row_dot(a, q0, q1);
+        __syncthreads();
+        if (w == 0) {
+            double tot = 0.0;
+            for (int k = 0; k < kRowWaves; ++k) tot += sh[0][k][lane];
+            a.hub_part[(size_t)b * kCols + lane] = tot;
+            for (int q = 0; q < 3; ++q) a.partials[((size_t)b * 3 + q) * kCols + lane] = 0.0;
+        }
+        return;
+    }
+    double err = 0.0, sum = 0.0, dang = 0.0;
+    const uint32_t gw = (b - a.nblk_h) * kRowWaves + w;
+    if (gw < a.n_waves) {
+        const int32_t r0 = a.wave_rows[gw], r1 = a.wave_rows[gw + 1];
+        for (int32_t v = r0; v < r1; ++v) {
+            const double acc = row_dot(a, a.row[v], a.row[v + 1]);
+            finalize(a, cs, c, v, acc, inv_n, err, sum, dang);
+        }
+    }
+    sh[0][w][lane] = err;
+    sh[1][w][lane] = sum;
+    sh[2][w][lane] = dang;
+    __syncthreads();
+    if (w == 0) {
+        for (int q = 0; q < 3; ++q) {
+            double t = 0.0;
+            for (int k = 0; k < kRowWaves; ++k) t += sh[q][k][lane];
+            a.partials[((size_t)b * 3 + q) * kCols + lane] = t;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
+    __shared__ double sh[3][kRedWaves][kCols];
+    __shared__ int sh_any;
+    if (!a.ctl->any_active) return;
+    const int cur = a.ctl->cur;
+    a.x_in = a.xbuf[cur];
+    a.x_out = a.xbuf[cur ^ 1];
+    const int lane = lane_id();
+    const int w = wave_id();
+    const ColState cs = a.cols[lane];
+    const double c = (1.0 - cs.alpha) + cs.alpha * cs.dang;
+    const double inv_n = 1.0 / (double)a.n;
+    double err = 0.0, sum = 0.0, dang = 0.0;
+    for (int64_t h = w; h < a.n_hrows; h += kRedWaves) {
+        const HubRow hr = a.hrows[h];
+        double acc = 0.0;
+        for (int32_t k = 0; k < hr.nseg; ++k) acc += a.hub_part[(size_t)(hr.first + k) * kCols + lane];
+        finalize(a, cs, c, hr.row, acc, inv_n, err, sum, dang);
+    }
+    for (uint32_t b = w; b < a.grid; b += kRedWaves) {
+        err += a.partials[((size_t)b * 3 + 0) * kCols + lane];
+        sum += a.partials[((size_t)b * 3 + 1) * kCols + lane];
+        dang += a.partials[((size_t)b * 3 + 2) * kCols + lane];
+    }
+    sh[0][w][lane] = err;
+    sh[1][w][lane] = sum;
+    sh[2][w][lane] = dang;
+    if (threadIdx.x == 0) sh_any = 0;
+    __syncthreads();
+    if (w == 0) {
+        double e = 0.0, s = 0.0, d = 0.0;
+        for (int k = 0; k < kRedWaves; ++k) {
+            e += sh[0][k][lane];
+            s += sh[1][k][lane];
+            d += sh[2][k][lane];
+        }
+        ColState* st = &a.cols[lane];
+        if (st->active && !st->done) {
+            const int32_t it = st->iter + 1;
+            st->err = e;
+            st->sum = s;
+            st->dang = d;
+            st->iter = it;
+            st->done = (e < st->tol_n || it >= st->max_iter) ? 1 : 0;
+            if (!st->done) atomicOr(&sh_any, 1);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.ctl->any_active = sh_any;
+        a.ctl->cur = cur ^ 1;
+    }
+}
+
+// Column j of x := 1/n (networkx starts personalised runs from uniform).
+__global__ void k_spmm_init_col(float* __restrict__ x, int32_t n, int32_t j) {
+    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < n) x[(size_t)v * kCols + j] = (float)(1.0 / (double)n);
+}
+
+// out[perm[r]] = x[r][j] / sum_j
+__global__ void k_spmm_output(const float* __restrict__ x, const ColState* __restrict__ cols, const int32_t* perm,
+                              int32_t n, int32_t j, double* __restrict__ out) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const double s = cols[j].sum;
+    const int32_t v = perm ? perm[r] : r;
+    out[v] = (s > 0.0) ? (double)x[(size_t)r * kCols + j] * (1.0 / s) : 1.0 / (double)n;
+}
+
+__global__ void k_set_tslot(int32_t* __restrict__ tslot, const int32_t* __restrict__ rows,
+                            const int32_t* __restrict__ slots, int32_t count) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) tslot[rows[i]] = slots[i];
+}
+
+__global__ void k_fill_i32(int32_t* __restrict__ a, int64_t n, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// nval[q] = fl32(w[q] / outW[src]) (weighted) or fl32(1 / outdeg[src]).
+__global__ void k_nval(const int32_t* __restrict__ col, const double* __restrict__ w, const double* __restrict__ outw,
+                       int64_t m, int weighted, float* __restrict__ nval) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    const double d = outw[col[q]];
+    const double num = weighted ? w[q] : 1.0;
+    nval[q] = (d > 0.0) ? (float)(num / d) : 0.0f;
+}
+
+// wave_rows[k] = first row v in [n_hub, n] with (row[v]-row[n_hub]) + 2*(v-n_hub) >= k*per_wave
+__global__ void k_wave_rows(const int32_t* __restrict__ row, int32_t n_hub, int32_t n, int64_t per_wave,
+                            int64_t n_waves, int32_t* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > n_waves) return;
+    if (k == n_waves) {
+        out[k] = n;
+        return;
+    }
+    const int64_t target = k * per_wave;
+    const int64_t base = row[n_hub];
+    int32_t lo = n_hub, hi = n;
+    while (lo < hi) {
+        const int32_t mid = lo + ((hi - lo) >> 1);
+        const int64_t d = ((int64_t)row[mid] - base) + 2 * (int64_t)(mid - n_hub);
+        if (d < target) lo = mid + 1;
+        else hi = mid;
+    }
+    out[k] = lo;
+}
+
+inline unsigned nblk(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace
+
+// A batch session: up to 64 columns resident on one graph, own stream.
+struct Batch {
+    Graph* g = nullptr;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf X[2], nval, tslot, ptab, cols, ctl, partials, hub_part, hsegs, hrows, wave_rows, out;
+    int cur = 0;
+    int32_t n = 0;
+    uint32_t nblk_h = 0, grid = 0, n_waves = 0;
+    int64_t n_hrows = 0;
+    int32_t ptab_cap = 0;
+    // host mirror of the teleport table
+    std::vector<std::vector<int32_t>> seeds;  // per column
+    std::vector<int32_t> host_tslot_rows;     // rows that own a table slot
+    std::vector<std::vector<float>> host_ptab;
+    std::vector<int32_t> row_to_slot_keys, row_to_slot_vals;
+    bool ptab_dirty = false;
+    std::vector<ColState> host_cols;
+    void* pinned = nullptr;  // ColState[64] + BatchCtl
+    double last_kernel_seconds = 0.0;
+
+    ~Batch() {
+        if (device >= 0) (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (pinned) (void)hipHostFree(pinned);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int32_t slot_of_row(int32_t v) const {
+        for (size_t i = 0; i < host_tslot_rows.size(); ++i)
+            if (host_tslot_rows[i] == v) return (int32_t)i;
+        return -1;
+    }
+};
+
+int ensure_nval(Graph& g) {
+    if (g.p_nval.p && g.p_nval.bytes >= (size_t)std::max<int64_t>(g.m, 1) * 4) return EGX_SUCC;
+    EGX_TRY(g.p_nval.alloc((size_t)std::max<int64_t>(g.m, 1) * 4));
+    if (g.m > 0) {
+        k_nval<<<nblk(g.m, 256), 256, 0, g.stream>>>(g.it_col, g.it_w, g.it_outw, g.m, g.weighted ? 1 : 0,
+                                                      g.p_nval.as<float>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
+    EGX_HIP_TRY(hipStreamSynchronize(g.stream), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
+    std::unique_ptr<Batch> b(new (std::nothrow) Batch());
+    if (!b) return EGX_FAILED_TO_ALLOCATE_HOST_MEM;
+    b->g = &g;
+    b->device = g.device;
+    b->n = g.n;
+    EGX_HIP_TRY(hipStreamCreate(&b->stream), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipEventCreate(&b->ev0), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipEventCreate(&b->ev1), EGX_DEVICE_ERR);
+    const int32_t n = g.n;
+    if (n <= 0 || g.m <= 0) {
+        set_last_error("batched PageRank needs a graph with nodes and arcs");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    EGX_TRY(ensure_nval(g));
+    hipStream_t s = b->stream;
+    for (int i = 0; i < 2; ++i) {
+        EGX_TRY(b->X[i].alloc((size_t)n * kCols * 4));
+        EGX_HIP_TRY(hipMemsetAsync(b->X[i].p, 0, (size_t)n * kCols * 4, s), EGX_DEVICE_ERR);
+    }
+    EGX_TRY(b->tslot.alloc((size_t)n * 4));
+    k_fill_i32<<<nblk(n, 256), 256, 0, s>>>(b->tslot.as<int32_t>(), n, -1);
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    b->ptab_cap = 64 * 16;
+    EGX_TRY(b->ptab.alloc((size_t)b->ptab_cap * kCols * 4));
+    EGX_HIP_TRY(hipMemsetAsync(b->ptab.p, 0, (size_t)b->ptab_cap * kCols * 4, s), EGX_DEVICE_ERR);
+    EGX_TRY(b->cols.alloc(sizeof(ColState) * kCols));
+    EGX_HIP_TRY(hipMemsetAsync(b->cols.p, 0, sizeof(ColState) * kCols, s), EGX_DEVICE_ERR);
+    EGX_TRY(b->ctl.alloc(sizeof(BatchCtl)));
+    EGX_HIP_TRY(hipMemsetAsync(b->ctl.p, 0, sizeof(BatchCtl), s), EGX_DEVICE_ERR);
+    b->host_cols.assign(kCols, ColState{});
+    b->seeds.assign(kCols, {});
+    EGX_HIP_TRY(hipHostMalloc(&b->pinned, sizeof(ColState) * kCols + sizeof(BatchCtl)),
+                EGX_FAILED_TO_ALLOCATE_HOST_MEM);
+
+    // hub segments of the SpMM (hub rows are the iteration-layout prefix)
+    std::vector<int32_t> row((size_t)g.n_hub + 1);
+    EGX_HIP_TRY(hipMemcpy(row.data(), g.it_row, row.size() * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    std::vector<HubSeg> segs;
+    std::vector<HubRow> hrows;
+    for (int32_t v = 0; v < g.n_hub; ++v) {
+        const int32_t bg = row[(size_t)v], en = row[(size_t)v + 1];
+        const int32_t nseg = (int32_t)(((int64_t)(en - bg) + kSpmmHubSeg - 1) / kSpmmHubSeg);
+        const int32_t first = (int32_t)segs.size();
+        hrows.push_back(HubRow{v, first, nseg, 0});
+        for (int32_t k = 0; k < nseg; ++k) {
+            HubSeg sg{};
+            sg.row = v;
+            sg.beg = bg + k * kSpmmHubSeg;
+            sg.end = (int32_t)std::min<int64_t>((int64_t)sg.beg + kSpmmHubSeg, en);
+            sg.slot = v;
+            sg.nseg = nseg;
+            sg.first = first;
+            segs.push_back(sg);
+        }
+    }
+    b->nblk_h = (uint32_t)segs.size();
+    b->n_hrows = (int64_t)hrows.size();
+    EGX_TRY(b->hsegs.alloc(std::max<size_t>(segs.size(), 1) * sizeof(HubSeg)));
+    EGX_TRY(b->hrows.alloc(std::max<size_t>(hrows.size(), 1) * sizeof(HubRow)));
+    if (!segs.empty())
+        EGX_HIP_TRY(hipMemcpy(b->hsegs.p, segs.data(), segs.size() * sizeof(HubSeg), hipMemcpyHostToDevice),
+                    EGX_DEVICE_ERR);
+    if (!hrows.empty())
+        EGX_HIP_TRY(hipMemcpy(b->hrows.p, hrows.data(), hrows.size() * sizeof(HubRow), hipMemcpyHostToDevice),
+                    EGX_DEVICE_ERR);
+    EGX_TRY(b->hub_part.alloc(std::max<size_t>(segs.size(), 1) * kCols * 8));
+
+    // merge-path split of the other rows: about 8 wavefronts per CU-slot
+    int32_t row_end_hub = 0, row_end = 0;
+    EGX_HIP_TRY(hipMemcpy(&row_end_hub, g.it_row + g.n_hub, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(&row_end, g.it_row + n, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    const int64_t work = (int64_t)(row_end - row_end_hub) + 2 * (int64_t)(n - g.n_hub);
+    int64_t waves = std::min<int64_t>(8192, std::max<int64_t>(1, (work + 511) / 512));
+    const int64_t per_wave = std::max<int64_t>(1, (work + waves - 1) / waves);
+    waves = std::max<int64_t>(1, (work + per_wave - 1) / per_wave);
+    b->n_waves = (uint32_t)waves;
+    EGX_TRY(b->wave_rows.alloc(((size_t)waves + 1) * 4));
+    k_wave_rows<<<nblk(waves + 1, 256), 256, 0, s>>>(g.it_row, g.n_hub, n, per_wave, waves,
+                                                      b->wave_rows.as<int32_t>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    b->grid = b->nblk_h + (uint32_t)((waves + kRowWaves - 1) / kRowWaves);
+    EGX_TRY(b->partials.alloc((size_t)b->grid * 3 * kCols * 8));
+    EGX_TRY(b->out.alloc((size_t)n * 8));
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    out = std::move(b);
+    return EGX_SUCC;
+}
+
+int batch_poll(Batch& b);
+
+// Upload the host column states and the teleport table (between launches).
+int batch_sync_state(Batch& b) {
+    hipStream_t s = b.stream;
+    if (b.ptab_dirty) {
+        std::vector<float> flat((size_t)b.ptab_cap * kCols, 0.0f);
+        for (size_t t = 0; t < b.host_ptab.size(); ++t)
+            std::memcpy(&flat[t * kCols], b.host_ptab[t].data(), kCols * 4);
+        EGX_HIP_TRY(hipMemcpyAsync(b.ptab.p, flat.data(), flat.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+        if (!b.host_tslot_rows.empty()) {
+            std::vector<int32_t> rows = b.host_tslot_rows, slots(rows.size());
+            for (size_t i = 0; i < rows.size(); ++i) slots[i] = (int32_t)i;
+            DevBuf dr, ds;
+            EGX_TRY(dr.alloc(rows.size() * 4));
+            EGX_TRY(ds.alloc(rows.size() * 4));
+            EGX_HIP_TRY(hipMemcpyAsync(dr.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+            EGX_HIP_TRY(hipMemcpyAsync(ds.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+            k_set_tslot<<<nblk((int64_t)rows.size(), 256), 256, 0, s>>>(b.tslot.as<int32_t>(), dr.as<int32_t>(),
+                                                                          ds.as<int32_t>(), (int32_t)rows.size());
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+            EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+        }
+        b.ptab_dirty = false;
+    }
+    return EGX_SUCC;
+}
+
+// Assign column j: seeds (original node ids) or none (global PageRank).
+int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, double alpha, int32_t max_iter,
+                 double tol) {
+    if (j < 0 || j >= kCols) {
+        set_last_error("batch slot out of range [0, 64)");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    Graph& g = *b.g;
+    hipStream_t s = b.stream;
+    // pull the device column states and buffer parity so host edits do not
+    // lose device progress
+    EGX_TRY(batch_poll(b));
+    // drop the old seeds of column j from the table
+    for (int32_t v : b.seeds[(size_t)j]) {
+        const int32_t t = b.slot_of_row(v);
+        if (t >= 0) b.host_ptab[(size_t)t][(size_t)j] = 0.0f;
+    }
+    b.seeds[(size_t)j].clear();
+    std::vector<int32_t> rseeds;
+    if (nseeds > 0) {
+        if (!seeds) return EGX_UNSUPPORTED_GRAPH;
+        // seeds arrive as caller ids; the table is indexed by iteration rows
+        for (int32_t i = 0; i < nseeds; ++i) {
+            if (seeds[i] < 0 || seeds[i] >= g.n) {
+                set_last_error("seed node out of range");
+                return EGX_UNSUPPORTED_GRAPH;
+            }
+        }
+        std::vector<int32_t> uniq(seeds, seeds + nseeds);
+        std::sort(uniq.begin(), uniq.end());
+        uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+        for (int32_t v : uniq) {
+            int32_t r = v;
+            if (g.reordered) EGX_HIP_TRY(hipMemcpy(&r, g.rank.as<int32_t>() + v, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+            rseeds.push_back(r);
+        }
+        const float wgt = (float)(1.0 / (double)rseeds.size());
+        for (int32_t r : rseeds) {
+            int32_t t = b.slot_of_row(r);
+            if (t < 0) {
+                // reuse an all-zero slot before growing
+                for (size_t i = 0; i < b.host_ptab.size() && t < 0; ++i) {
+                    bool empty = true;
+                    for (int q = 0; q < kCols; ++q) empty = empty && b.host_ptab[i][(size_t)q] == 0.0f;
+                    if (empty) {
+                        t = (int32_t)i;
+                        const int32_t old = b.host_tslot_rows[i];
+                        b.host_tslot_rows[i] = r;
+                        // the old row no longer owns a slot
+                        int32_t m1 = -1;
+                        DevBuf dr, ds;
+                        EGX_TRY(dr.alloc(4));
+                        EGX_TRY(ds.alloc(4));
+                        EGX_HIP_TRY(hipMemcpy(dr.p, &old, 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+                        EGX_HIP_TRY(hipMemcpy(ds.p, &m1, 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+                        k_set_tslot<<<1, 64, 0, s>>>(b.tslot.as<int32_t>(), dr.as<int32_t>(), ds.as<int32_t>(), 1);
+                        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+                        EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+                    }
+                }
+                if (t < 0) {
+                    if ((int32_t)b.host_ptab.size() >= b.ptab_cap) {
+                        set_last_error("teleport table full (more than 1024 distinct seed nodes in one batch)");
+                        return EGX_UNSUPPORTED_GRAPH;
+                    }
+                    t = (int32_t)b.host_ptab.size();
+                    b.host_ptab.emplace_back(kCols, 0.0f);
+                    b.host_tslot_rows.push_back(r);
+                }
+            }
+            b.host_ptab[(size_t)t][(size_t)j] = wgt;
+        }
+        b.ptab_dirty = true;
+    }
+    b.seeds[(size_t)j] = rseeds;
+    ColState cs{};
+    cs.alpha = alpha;
+    cs.tol_n = std::max(0.0, tol) * (double)g.n;
+    cs.max_iter = std::max(1, max_iter);
+    cs.active = 1;
+    cs.uniform = rseeds.empty() ? 1 : 0;
+    cs.sum = 1.0;
+    const float x0 = (float)(1.0 / (double)g.n);
+    cs.dang = (double)g.n_dangling * (double)x0;
+    b.host_cols[(size_t)j] = cs;
+    if (nseeds > 0) b.ptab_dirty = true;
+    EGX_TRY(batch_sync_state(b));
+    k_spmm_init_col<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), g.n, j);
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(b.cols.p, b.host_cols.data(), sizeof(ColState) * kCols, hipMemcpyHostToDevice, s),
+                EGX_DEVICE_ERR);
+    BatchCtl ctl{};
+    ctl.any_active = 1;
+    ctl.cur = b.cur;
+    EGX_HIP_TRY(hipMemcpyAsync(b.ctl.p, &ctl, sizeof(ctl), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+int batch_release(Batch& b, int32_t j) {
+    if (j < 0 || j >= kCols) return EGX_UNSUPPORTED_GRAPH;
+    hipStream_t s = b.stream;
+    EGX_TRY(batch_poll(b));
+    for (int32_t v : b.seeds[(size_t)j]) {
+        const int32_t t = b.slot_of_row(v);
+        if (t >= 0) b.host_ptab[(size_t)t][(size_t)j] = 0.0f;
+    }
+    b.seeds[(size_t)j].clear();
+    b.host_cols[(size_t)j].active = 0;
+    b.ptab_dirty = true;
+    EGX_TRY(batch_sync_state(b));
+    EGX_HIP_TRY(hipMemcpyAsync(b.cols.p, b.host_cols.data(), sizeof(ColState) * kCols, hipMemcpyHostToDevice, s),
+                EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+// Enqueue `iters` iterations (launches after every column stops return at
+// once).  Asynchronous: returns before the work completes.
+int batch_step(Batch& b, int32_t iters, bool record_time) {
+    Graph& g = *b.g;
+    hipStream_t s = b.stream;
+    SpmmArgs a{};
+    a.row = g.it_row;
+    a.col = g.it_col;
+    a.nval = g.p_nval.as<float>();
+    a.deg = g.it_deg;
+    a.tslot = b.tslot.as<int32_t>();
+    a.ptab = b.ptab.as<float>();
+    a.wave_rows = b.wave_rows.as<int32_t>();
+    a.hsegs = b.hsegs.as<HubSeg>();
+    a.hrows = b.hrows.as<HubRow>();
+    a.hub_part = b.hub_part.as<double>();
+    a.cols = b.cols.as<ColState>();
+    a.ctl = b.ctl.as<BatchCtl>();
+    a.partials = b.partials.as<double>();
+    a.n_hrows = b.n_hrows;
+    a.n = g.n;
+    a.nblk_h = b.nblk_h;
+    a.grid = b.grid;
+    a.n_waves = b.n_waves;
+    if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev0, s), EGX_DEVICE_ERR);
+    a.xbuf[0] = b.X[0].as<float>();
+    a.xbuf[1] = b.X[1].as<float>();
+    for (int32_t i = 0; i < iters; ++i) {
+        k_spmm_rows<<<b.grid, kRowThreads, 0, s>>>(a);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        k_spmm_reduce<<<1, kRedThreads, 0, s>>>(a);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
+    if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev1, s), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+// Copy the column states to the host (synchronises the batch stream).
+int batch_poll(Batch& b) {
+    hipStream_t s = b.stream;
+    char* p = static_cast<char*>(b.pinned);
+    EGX_HIP_TRY(hipMemcpyAsync(p, b.cols.p, sizeof(ColState) * kCols, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(p + sizeof(ColState) * kCols, b.ctl.p, sizeof(BatchCtl), hipMemcpyDeviceToHost, s),
+                EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    std::memcpy(b.host_cols.data(), p, sizeof(ColState) * kCols);
+    b.cur = reinterpret_cast<const BatchCtl*>(p + sizeof(ColState) * kCols)->cur;
+    return EGX_SUCC;
+}
+
+int batch_any_active(Batch& b, int32_t* any) {
+    EGX_TRY(batch_poll(b));
+    const BatchCtl* c = reinterpret_cast<const BatchCtl*>(static_cast<char*>(b.pinned) + sizeof(ColState) * kCols);
+    *any = c->any_active;
+    return EGX_SUCC;
+}
+
+int batch_result(Batch& b, int32_t j, double* host_out) {
+    if (j < 0 || j >= kCols) return EGX_UNSUPPORTED_GRAPH;
+    Graph& g = *b.g;
+    hipStream_t s = b.stream;
+    EGX_TRY(batch_poll(b));  // the device knows which buffer is current
+    k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), b.cols.as<ColState>(),
+                                                  g.reordered ? g.perm.as<int32_t>() : nullptr, g.n, j,
+                                                  b.out.as<double>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    if (host_out)
+        EGX_HIP_TRY(hipMemcpyAsync(host_out, b.out.p, (size_t)g.n * 8, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+int batch_elapsed(Batch& b, double* seconds) {
+    EGX_HIP_TRY(hipEventSynchronize(b.ev1), EGX_DEVICE_ERR);
+    float ms = 0.0f;
+    EGX_HIP_TRY(hipEventElapsedTime(&ms, b.ev0, b.ev1), EGX_DEVICE_ERR);
+    *seconds = (double)ms * 1e-3;
+    return EGX_SUCC;
+}
+
+}  // namespace egx
+
+struct egx_batch {
+    std::unique_ptr<egx::Batch> b;
+};
+
+namespace {
+
+struct BatchDevice {
+    int prev = -1;
+    bool ok = false;
+    explicit BatchDevice(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) return;
+        ok = (prev == dev) || (hipSetDevice(dev) == hipSuccess);
+    }
+    ~BatchDevice() {
+        if (ok && prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Runs the batch until no column is active or max_launches iterations were
+// enqueued, checking the device stop flags every check_every launches.
+int run_until_done(egx::Batch& b, int32_t max_launches, int32_t check_every, double* kernel_seconds) {
+    if (check_every < 1) check_every = 4;
+    double total = 0.0;
+    int32_t launched = 0;
+    while (launched < max_launches) {
+        const int32_t todo = std::min(check_every, max_launches - launched);
+        EGX_TRY(egx::batch_step(b, todo, true));
+        launched += todo;
+        int32_t any = 0;
+        EGX_TRY(egx::batch_any_active(b, &any));
+        double sec = 0.0;
+        EGX_TRY(egx::batch_elapsed(b, &sec));
+        total += sec;
+        if (!any) break;
+    }
+    if (kernel_seconds) *kernel_seconds = total;
+    return EGX_SUCC;
+}
+
+}  // namespace
+
+extern "C" {
+
+int egx_batch_open(egx_graph* gh, egx_batch** out) {
+    if (!gh || !out) return EGX_UNSUPPORTED_GRAPH;
+    *out = nullptr;
+    egx::Graph& g = gh->g;
+    std::lock_guard<std::mutex> lk(g.mu);
+    BatchDevice dg(g.device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    std::unique_ptr<egx::Batch> b;
+    EGX_TRY(egx::batch_open(g, b));
+    g.open_batches += 1;
+    egx_batch* h = new (std::nothrow) egx_batch();
+    if (!h) return EGX_FAILED_TO_ALLOCATE_HOST_MEM;
+    h->b = std::move(b);
+    *out = h;
+    return EGX_SUCC;
+}
+
+int egx_batch_assign(egx_batch* bh, int32_t slot, const int32_t* seeds, int32_t nseeds, double alpha,
+                     int32_t max_iter, double tol) {
+    if (!bh || !bh->b) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    return egx::batch_assign(*bh->b, slot, seeds, nseeds, alpha, max_iter, tol);
+}
+
+int egx_batch_release(egx_batch* bh, int32_t slot) {
+    if (!bh || !bh->b) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    return egx::batch_release(*bh->b, slot);
+}
+
+int egx_batch_step(egx_batch* bh, int32_t iterations) {
+    if (!bh || !bh->b || iterations < 0) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    return egx::batch_step(*bh->b, iterations, false);
+}
+
+int egx_batch_poll(egx_batch* bh, int32_t* active, int32_t* done, int32_t* iterations, double* err) {
+    if (!bh || !bh->b) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    EGX_TRY(egx::batch_poll(*bh->b));
+    for (int j = 0; j < egx::kCols; ++j) {
+        const egx::ColState& c = bh->b->host_cols[(size_t)j];
+        if (active) active[j] = c.active;
+        if (done) done[j] = c.done;
+        if (iterations) iterations[j] = c.iter;
+        if (err) err[j] = c.err;
+    }
+    return EGX_SUCC;
+}
+
+int egx_batch_run(egx_batch* bh, int32_t max_launches, int32_t check_every, double* kernel_seconds) {
+    if (!bh || !bh->b) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    return run_until_done(*bh->b, max_launches, check_every, kernel_seconds);
+}
+
+int egx_batch_result(egx_batch* bh, int32_t slot, double* PR) {
+    if (!bh || !bh->b) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    return egx::batch_result(*bh->b, slot, PR);
+}
+
+void egx_batch_close(egx_batch* bh) {
+    if (!bh) return;
+    if (bh->b && bh->b->g) {
+        egx::Graph* g = bh->b->g;
+        bh->b.reset();
+        std::lock_guard<std::mutex> lk(g->mu);
+        g->open_batches -= 1;
+    }
+    delete bh;
+}
+
+int egx_graph_ppr(egx_graph* gh, int32_t k, const int32_t* seed_offsets, const int32_t* seeds, const double* alphas,
+                  double alpha, int32_t max_iter, double tol, double* PR, int32_t* iterations,
+                  double* kernel_seconds) {
+    if (!gh || k < 1 || k > egx::kCols || !seed_offsets) {
+        egx::set_last_error("egx_graph_ppr needs 1 <= k <= 64 columns and seed_offsets[k+1]");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    egx_batch* bh = nullptr;
+    EGX_TRY(egx_batch_open(gh, &bh));
+    std::unique_ptr<egx_batch, void (*)(egx_batch*)> guard(bh, egx_batch_close);
+    for (int32_t j = 0; j < k; ++j) {
+        const int32_t b0 = seed_offsets[j], b1 = seed_offsets[j + 1];
+        if (b1 < b0) return EGX_UNSUPPORTED_GRAPH;
+        EGX_TRY(egx_batch_assign(bh, j, seeds ? seeds + b0 : nullptr, b1 - b0, alphas ? alphas[j] : alpha,
+                                 max_iter, tol));
+    }
+    EGX_TRY(egx_batch_run(bh, std::max(1, max_iter), 0, kernel_seconds));
+    std::vector<int32_t> it(egx::kCols);
+    EGX_TRY(egx_batch_poll(bh, nullptr, nullptr, it.data(), nullptr));
+    for (int32_t j = 0; j < k; ++j) {
+        if (iterations) iterations[j] = it[(size_t)j];
+        if (PR) EGX_TRY(egx_batch_result(bh, j, PR + (size_t)j * gh->g.n));
+    }
+    return EGX_SUCC;
+}
+
+}  // extern "C"

@@ -151,6 +151,19 @@ SIGNATURES = {
     "egx_graph_pagerank_ex": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, ctypes.POINTER(SolveStats),
                                              ctypes.c_int32]),
     "egx_graph_free": (None, [_vp]),
+    "egx_batch_open": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "egx_batch_assign": (ctypes.c_int, [_vp, ctypes.c_int32, _i32p, ctypes.c_int32, ctypes.c_double,
+                                        ctypes.c_int32, ctypes.c_double]),
+    "egx_batch_release": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "egx_batch_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "egx_batch_poll": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _f64p]),
+    "egx_batch_run": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _f64p]),
+    "egx_batch_result": (ctypes.c_int, [_vp, ctypes.c_int32, _f64p]),
+    "egx_batch_close": (None, [_vp]),
+    "egx_graph_ppr": (ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _i32p, _f64p, ctypes.c_double, ctypes.c_int32,
+                                     ctypes.c_double, _f64p, _i32p, _f64p]),
+    "egx_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _f64p]),
+    "egx_probe_gather": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _f64p]),
 }
 
 _lib = None
@@ -321,10 +334,97 @@ class DeviceGraph:
                                            int(bool(time_each_launch))))
         return out, st.as_dict()
 
+    def ppr(self, seed_sets, alpha=0.85, max_iter=200, tol=1e-6, alphas=None, download=True):
+        """Batched PageRank: one column per entry of ``seed_sets`` (a list of
+        node-id lists; an empty list is a global PageRank), at most 64.
+        Returns (float64[k, n] or None, iterations int32[k], kernel_seconds)."""
+        k = len(seed_sets)
+        if not 1 <= k <= 64:
+            raise ValueError("1..64 seed sets per call")
+        offs = np.zeros(k + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([len(s) for s in seed_sets])
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(s, dtype=np.int32) for s in seed_sets])
+                                    if offs[-1] else np.zeros(1, np.int32), dtype=np.int32)
+        al = None if alphas is None else np.ascontiguousarray(alphas, dtype=np.float64)
+        n = self.n
+        out = np.empty((k, n), dtype=np.float64) if download else None
+        it = np.zeros(k, dtype=np.int32)
+        ks = ctypes.c_double(0.0)
+        _check(lib().egx_graph_ppr(self.handle, k, _ptr(offs, _i32p), _ptr(flat, _i32p), _ptr(al, _f64p),
+                                   float(alpha), int(max_iter), float(tol), _ptr(out, _f64p), _ptr(it, _i32p),
+                                   ctypes.byref(ks)))
+        return out, it, float(ks.value)
+
+    def batch(self):
+        """Open a :class:`Batch` session (64 column slots) on this graph."""
+        return Batch(self)
+
     # -- lifetime ---------------------------------------------------------
     def close(self):
         if not self._closed and self._h is not None and _lib is not None:
             _lib.egx_graph_free(self._h)
+        self._closed = True
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """64 PageRank column slots resident on one :class:`DeviceGraph`, with
+    their own HIP stream: assign / step / poll / result, slots re-assignable
+    while other columns keep iterating (continuous batching)."""
+
+    SLOTS = 64
+
+    def __init__(self, graph):
+        self.graph = graph
+        h = _vp()
+        _check(lib().egx_batch_open(graph.handle, ctypes.byref(h)))
+        self._h = h
+        self._closed = False
+
+    def assign(self, slot, seeds=(), alpha=0.85, max_iter=200, tol=1e-6):
+        s = np.ascontiguousarray(np.asarray(seeds, dtype=np.int32).reshape(-1))
+        _check(lib().egx_batch_assign(self._h, int(slot), _ptr(s, _i32p) if len(s) else None, len(s),
+                                      float(alpha), int(max_iter), float(tol)))
+
+    def release(self, slot):
+        _check(lib().egx_batch_release(self._h, int(slot)))
+
+    def step(self, iterations=1):
+        _check(lib().egx_batch_step(self._h, int(iterations)))
+
+    def poll(self):
+        """dict of int32[64] 'active', 'done', 'iterations' and float64[64] 'err'."""
+        a = np.zeros(64, np.int32)
+        d = np.zeros(64, np.int32)
+        it = np.zeros(64, np.int32)
+        e = np.zeros(64, np.float64)
+        _check(lib().egx_batch_poll(self._h, _ptr(a, _i32p), _ptr(d, _i32p), _ptr(it, _i32p), _ptr(e, _f64p)))
+        return {"active": a, "done": d, "iterations": it, "err": e}
+
+    def run(self, max_launches=200, check_every=0):
+        ks = ctypes.c_double(0.0)
+        _check(lib().egx_batch_run(self._h, int(max_launches), int(check_every), ctypes.byref(ks)))
+        return float(ks.value)
+
+    def result(self, slot):
+        out = np.empty(self.graph.n, dtype=np.float64)
+        _check(lib().egx_batch_result(self._h, int(slot), _ptr(out, _f64p)))
+        return out
+
+    def close(self):
+        if not self._closed and _lib is not None:
+            _lib.egx_batch_close(self._h)
         self._closed = True
 
     def __del__(self):

@@ -1,0 +1,195 @@
+"""Measurement commands behind the committed profiles.
+
+    python -m eggpu_amd.probe copy   [--bytes B] [--reps R]
+    python -m eggpu_amd.probe gather [--n N] [--reps R]
+    python -m eggpu_amd.probe calib  [--scale S] [--iters K] [--bytes B]
+        one plain copy of B bytes, then K PageRank iterations on R-MAT
+        scale S: the workload rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
+        use, so the copy's known bytes calibrate the SpMV's counters
+    python -m eggpu_amd.probe stats   DB [--out FILE]
+        per-kernel duration summary of a rocprofv3 --kernel-trace database
+    python -m eggpu_amd.probe traffic --fetch DB --write DB --bytes B [--kernel K] [--out FILE]
+        HBM bytes per launch of kernel K from the two --pmc databases of
+        `calib`, scaled by what the counters read for the B-byte copy
+
+Each command prints JSON.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import glob
+import json
+import os
+import sqlite3
+import sys
+
+from . import native
+
+
+def _copy(nbytes, reps):
+    sec = ctypes.c_double(0.0)
+    native._check(native.lib().egx_probe_copy(int(nbytes), int(reps), ctypes.byref(sec)))
+    return sec.value
+
+
+def _gather(ws, n, reps):
+    sec = ctypes.c_double(0.0)
+    native._check(native.lib().egx_probe_gather(int(ws), int(n), int(reps), ctypes.byref(sec)))
+    return sec.value
+
+
+def cmd_copy(a):
+    s = _copy(a.bytes, a.reps)
+    print(json.dumps({"probe": "copy", "bytes": a.bytes, "seconds": s,
+                      "read_plus_write_TBps": 2 * a.bytes / s / 1e12}))
+
+
+def cmd_gather(a):
+    rows = []
+    for ws in (1 << 12, 1 << 14, 1 << 16, 1 << 18, 1 << 20, 1 << 22, 1 << 24, 1 << 26):
+        s = _gather(ws, a.n, a.reps)
+        rows.append({"working_set_bytes": ws * 4, "gathers": a.n, "seconds": s,
+                     "Ggathers_per_s": a.n / s / 1e9})
+    print(json.dumps({"probe": "gather", "rows": rows}))
+
+
+def cmd_calib(a):
+    _copy(a.bytes, 1)
+    g = native.DeviceGraph.rmat(a.scale, 16, seed=1)
+    inf = g.info()
+    _, st = g.pagerank_stats(0.85, a.iters, 0.0, time_each_launch=False)
+    print(json.dumps({"probe": "calib", "copy_bytes": a.bytes, "n": inf["n"], "m": inf["m"],
+                      "iterations": st["iterations"]}))
+
+
+def _dbs(path):
+    if os.path.isfile(path):
+        return [path]
+    return sorted(glob.glob(os.path.join(path, "**", "*.db"), recursive=True))
+
+
+def _kernel_rows(db):
+    c = sqlite3.connect(db)
+    cols = [r[1] for r in c.execute("pragma table_info(kernels)")]
+    rows = [dict(zip(cols, r)) for r in c.execute("select * from kernels")]
+    c.close()
+    return rows
+
+
+def cmd_stats(a):
+    per = {}
+    for db in _dbs(a.db):
+        for r in _kernel_rows(db):
+            d = per.setdefault(r["name"], [])
+            d.append(r["duration"])
+    out = []
+    for name, durs in per.items():
+        out.append({"kernel": name, "calls": len(durs), "total_us": sum(durs) / 1e3,
+                    "mean_us": sum(durs) / len(durs) / 1e3, "min_us": min(durs) / 1e3, "max_us": max(durs) / 1e3})
+    out.sort(key=lambda r: -r["total_us"])
+    text = json.dumps({"source": a.db, "kernels": out}, indent=1)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(text + "\n")
+    print(text)
+
+
+def _pmc(db, counter):
+    """{kernel name: [counter value per dispatch]} from a --pmc database."""
+    c = sqlite3.connect(db)
+    tabs = [r[0] for r in c.execute("select name from sqlite_master where type in ('table','view')")]
+    res = {}
+    if "counters_collection" in tabs:
+        cols = [r[1] for r in c.execute("pragma table_info(counters_collection)")]
+        name_col = "kernel_name" if "kernel_name" in cols else "name"
+        val_col = "value" if "value" in cols else "counter_value"
+        q = f"select {name_col}, dispatch_id, counter_name, {val_col} from counters_collection"
+        acc = {}
+        for name, disp, cname, val in c.execute(q):
+            if cname != counter:
+                continue
+            acc[(name, disp)] = acc.get((name, disp), 0.0) + float(val)
+        for (name, _), v in acc.items():
+            res.setdefault(name, []).append(v)
+    c.close()
+    return res
+
+
+def cmd_traffic(a):
+    fetch = {}
+    write = {}
+    for db in _dbs(a.fetch):
+        for k, v in _pmc(db, "FETCH_SIZE").items():
+            fetch.setdefault(k, []).extend(v)
+    for db in _dbs(a.write):
+        for k, v in _pmc(db, "WRITE_SIZE").items():
+            write.setdefault(k, []).extend(v)
+
+    def pick(d, part):
+        for k, v in d.items():
+            if part in k and v:
+                return sum(v) / len(v), len(v), k
+        return None, 0, None
+
+    cf, _, _ = pick(fetch, "k_probe_copy")
+    cw, _, _ = pick(write, "k_probe_copy")
+    kf, nf, kname = pick(fetch, a.kernel)
+    kw, nw, _ = pick(write, a.kernel)
+    if None in (cf, cw, kf, kw):
+        raise SystemExit(f"missing counters: copy fetch {cf} write {cw}; {a.kernel} fetch {kf} write {kw}")
+    f_scale = a.bytes / cf  # bytes per counter unit, from the copy's known read
+    w_scale = a.bytes / cw
+    out = {
+        "kernel": kname,
+        "dispatches": {"fetch": nf, "write": nw},
+        "copy_bytes": a.bytes,
+        "copy_FETCH_SIZE": cf,
+        "copy_WRITE_SIZE": cw,
+        "fetch_scale_bytes_per_unit": f_scale,
+        "write_scale_bytes_per_unit": w_scale,
+        "kernel_FETCH_SIZE_mean": kf,
+        "kernel_WRITE_SIZE_mean": kw,
+        "hbm_read_bytes_per_launch": kf * f_scale,
+        "hbm_write_bytes_per_launch": kw * w_scale,
+        "hbm_bytes_per_launch": kf * f_scale + kw * w_scale,
+        "fetch_db": a.fetch,
+        "write_db": a.write,
+    }
+    text = json.dumps(out, indent=1)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(text + "\n")
+    print(text)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m eggpu_amd.probe")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    p = sub.add_parser("copy")
+    p.add_argument("--bytes", type=int, default=1 << 30)
+    p.add_argument("--reps", type=int, default=10)
+    p = sub.add_parser("gather")
+    p.add_argument("--n", type=int, default=1 << 27)
+    p.add_argument("--reps", type=int, default=3)
+    p = sub.add_parser("calib")
+    p.add_argument("--scale", type=int, default=24)
+    p.add_argument("--iters", type=int, default=3)
+    p.add_argument("--bytes", type=int, default=1 << 30)
+    p = sub.add_parser("stats")
+    p.add_argument("db")
+    p.add_argument("--out")
+    p = sub.add_parser("traffic")
+    p.add_argument("--fetch", required=True)
+    p.add_argument("--write", required=True)
+    p.add_argument("--bytes", type=int, default=1 << 30)
+    p.add_argument("--kernel", default="k_pr_spmv")
+    p.add_argument("--out")
+    a = ap.parse_args(argv)
+    {"copy": cmd_copy, "gather": cmd_gather, "calib": cmd_calib, "stats": cmd_stats,
+     "traffic": cmd_traffic}[a.cmd](a)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -181,6 +181,62 @@ int egx_graph_pagerank_ex(egx_graph *g, const egx_solve_opts *opts, double *PR, 
 
 void egx_graph_free(egx_graph *g);
 
+/* ------------------------------------------------------------------------ */
+/* Batched and personalised PageRank (no counterpart in the reference, whose  */
+/* pagerank.py:14 has no personalisation).  Up to 64 columns share one graph: */
+/* each is a global PageRank (no seeds) or a personalised one whose teleport  */
+/* and dangling mass are spread uniformly over its seed nodes (networkx 3.4.2 */
+/* pagerank(personalization=...) semantics), with its own alpha, tol,         */
+/* max_iter, stop flag and iteration count.  Results of a column do not       */
+/* depend on the other columns of the batch.                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct egx_batch egx_batch; /* opaque, owns its stream and HBM state */
+
+/* Open a batch session on g (g must outlive it and is not re-binned meanwhile). */
+int egx_batch_open(egx_graph *g, egx_batch **out);
+
+/* (Re)start column slot in [0, 64): x0 = 1/n, seeds are node ids (nseeds = 0:
+ * global PageRank).  Safe between steps; other columns keep running. */
+int egx_batch_assign(egx_batch *b, int32_t slot, const int32_t *seeds, int32_t nseeds, double alpha,
+                     int32_t max_iter, double tol);
+
+/* Free column slot (it stops taking part in later iterations). */
+int egx_batch_release(egx_batch *b, int32_t slot);
+
+/* Enqueue `iterations` iterations of every active column (asynchronous). */
+int egx_batch_step(egx_batch *b, int32_t iterations);
+
+/* Wait for the batch stream and read the per-column state; each array holds
+ * 64 entries and may be NULL. */
+int egx_batch_poll(egx_batch *b, int32_t *active, int32_t *done, int32_t *iterations, double *err);
+
+/* Step until no active column is left or max_launches iterations ran,
+ * polling every check_every iterations (0: 4).  kernel_seconds: hipEvents. */
+int egx_batch_run(egx_batch *b, int32_t max_launches, int32_t check_every, double *kernel_seconds);
+
+/* Sum-normalised FP64 ranks of column slot, in node id order (PR[n]). */
+int egx_batch_result(egx_batch *b, int32_t slot, double *PR);
+
+void egx_batch_close(egx_batch *b);
+
+/* One-shot: k columns, column j seeded by seeds[seed_offsets[j] .. seed_offsets[j+1])
+ * (an empty range = global PageRank), alpha_j = alphas ? alphas[j] : alpha.
+ * PR (k x n, row j = column j) and iterations (k) may be NULL. */
+int egx_graph_ppr(egx_graph *g, int32_t k, const int32_t *seed_offsets, const int32_t *seeds,
+                  const double *alphas, double alpha, int32_t max_iter, double tol, double *PR,
+                  int32_t *iterations, double *kernel_seconds);
+
+/* ------------------------------------------------------------------------ */
+/* Measurement probes (no counterpart in the reference): calibration of the   */
+/* roofline figures on the device the library runs on.                       */
+/* ------------------------------------------------------------------------ */
+/* Plain streaming device copy of `bytes`; mean seconds per copy over reps. */
+int egx_probe_copy(int64_t bytes, int32_t reps, double *seconds_per_copy);
+
+/* n_gathers random 4-byte loads from a working set of working_set_floats
+ * floats; mean seconds per pass over reps. */
+int egx_probe_gather(int64_t working_set_floats, int64_t n_gathers, int32_t reps, double *seconds_per_pass);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,148 @@
+"""GPU tests of batched / personalised PageRank (SpMM engine).
+
+Semantics pinned by networkx 3.4.2 pagerank(personalization=...) fixtures
+and the oracle's orc_ppr_push; a global column must equal plain PageRank.
+A column's result must be the same alone and in a mixed batch (here:
+bit-identical), also when other slots are re-assigned while it runs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from eggpu_amd import native
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _say(msg):
+    print(f"[gpu-batch] {msg}", flush=True)
+
+
+def _close(ours, ref, tag, tol=1e-6, l1=1e-5):
+    e = np.abs(np.asarray(ours) - np.asarray(ref))
+    _say(f"{tag}: max-abs {e.max():.3e} L1 {e.sum():.3e}")
+    assert e.max() <= tol and e.sum() <= l1, tag
+
+
+def _p(n, seeds):
+    p = np.zeros(n)
+    p[list(seeds)] = 1.0 / len(seeds)
+    return p
+
+
+def test_ppr_matches_networkx_fixture(gpu_ready):
+    z = np.load(os.path.join(GOLDEN, "ppr_rmat10.npz"))
+    V, E, W = z["V"], z["E"], z["W"]
+    sets = [[int(s) for s in S if s >= 0] for S in z["seed_sets"]]
+    g = native.DeviceGraph.from_csr(V, E, W)
+    pr, it, _ = g.ppr(sets, alpha=0.85, max_iter=200, tol=1e-6)
+    for j in range(len(sets)):
+        ref = z["nx_values"][j] / z["nx_values"][j].sum()
+        _close(pr[j], ref, f"nx ppr set {j}")
+        assert it[j] == int(z["nx_iterations"][j])
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_mixed_columns_vs_oracle(gpu_ready, weighted):
+    V, E = O.rmat_csr(12, 16, seed=31)
+    W = O.edge_weights(V, E, 8) if weighted else None
+    n = len(V) - 1
+    src, dst = O.csr_rows(V, E)
+    rng = np.random.default_rng(5)
+    sets, alphas = [], []
+    for j in range(20):
+        k = int(rng.integers(0, 17))  # 0 -> global column
+        sets.append(sorted(rng.choice(n, k, replace=False).tolist()) if k else [])
+        alphas.append(0.85 if j % 3 else 0.9)
+    g = native.DeviceGraph.from_csr(V, E, W)
+    pr, it, _ = g.ppr(sets, alphas=alphas, max_iter=200, tol=1e-6)
+    for j, S in enumerate(sets):
+        if S:
+            ref, it_ref, _ = O.ppr_rows(n, src, dst, W, _p(n, S), alphas[j], 200, 1e-6)
+        else:
+            ref, it_ref, _ = O.pagerank_rows(n, src, dst, W, alphas[j], 200, 1e-6)
+        _close(pr[j], ref, f"col {j} seeds={len(S)} alpha={alphas[j]}")
+        assert it[j] == it_ref, f"col {j}"
+
+
+def test_full_batch_of_64_fixed_iterations(gpu_ready):
+    V, E = O.rmat_csr(13, 16, seed=2)
+    n = len(V) - 1
+    src, dst = O.csr_rows(V, E)
+    rng = np.random.default_rng(9)
+    sets = [sorted(rng.choice(n, int(rng.integers(1, 17)), replace=False).tolist()) for _ in range(64)]
+    g = native.DeviceGraph.from_csr(V, E)
+    pr, it, _ = g.ppr(sets, max_iter=30, tol=0.0)
+    assert (it == 30).all()
+    for j in (0, 17, 40, 63):
+        ref, _, _ = O.ppr_rows(n, src, dst, None, _p(n, sets[j]), 0.85, 30, 0.0)
+        _close(pr[j], ref, f"64-batch col {j}")
+
+
+def test_column_alone_equals_column_in_mix(gpu_ready):
+    V, E = O.rmat_csr(12, 16, seed=77)
+    W = O.edge_weights(V, E, 3)
+    n = len(V) - 1
+    g = native.DeviceGraph.from_csr(V, E, W)
+    target = [5, 99, 1000]
+    alone, it_alone, _ = g.ppr([target], alpha=0.9, max_iter=100, tol=1e-7)
+    rng = np.random.default_rng(1)
+    others = [sorted(rng.choice(n, 4, replace=False).tolist()) for _ in range(40)]
+    mix_sets = others[:23] + [target] + others[23:]
+    alphas = [0.85] * 23 + [0.9] + [0.85] * 17
+    mixed, it_mix, _ = g.ppr(mix_sets, alphas=alphas, max_iter=100, tol=1e-7)
+    assert it_mix[23] == it_alone[0]
+    assert np.array_equal(mixed[23], alone[0])
+
+
+def test_continuous_reassignment(gpu_ready):
+    """Slots re-assigned while others run give the alone results."""
+    V, E = O.rmat_csr(12, 16, seed=4)
+    n = len(V) - 1
+    g = native.DeviceGraph.from_csr(V, E)
+    A, B, C = [3], [10, 20, 30], [2048, 7]
+    ref = {}
+    for name, S in (("A", A), ("B", B), ("C", C)):
+        out, it, _ = g.ppr([S], max_iter=200, tol=1e-6)
+        ref[name] = (out[0], it[0])
+    with g.batch() as b:
+        b.assign(0, A)
+        b.assign(5, B)
+        b.step(3)
+        st = b.poll()
+        assert st["iterations"][0] == 3 and st["iterations"][5] == 3
+        b.assign(5, C)  # slot 5 restarts with a new request mid-flight
+        b.run(max_launches=300)
+        st = b.poll()
+        assert st["done"][0] and st["done"][5]
+        assert st["iterations"][0] == ref["A"][1] and st["iterations"][5] == ref["C"][1]
+        assert np.array_equal(b.result(0), ref["A"][0])
+        assert np.array_equal(b.result(5), ref["C"][0])
+        b.release(5)
+        b.assign(9, B)
+        b.run(max_launches=300)
+        assert np.array_equal(b.result(9), ref["B"][0])
+
+
+def test_global_column_equals_plain_pagerank(gpu_ready):
+    V, E = O.rmat_csr(14, 16, seed=1)
+    g = native.DeviceGraph.from_csr(V, E)
+    pr, it, _ = g.ppr([[]], max_iter=100, tol=1e-6)
+    plain, it_plain, _ = g.pagerank(0.85, 100, 1e-6)
+    assert it[0] == it_plain == 5
+    _close(pr[0], plain, "global column vs plain", tol=1e-9, l1=1e-7)
+
+
+def test_scale20_ppr_properties(gpu_ready):
+    _say("scale 20 weighted ppr, 64 columns")
+    g = native.DeviceGraph.rmat(20, 16, seed=3, weighted=True, wseed=4)
+    rng = np.random.default_rng(2)
+    n = g.n
+    sets = [sorted(rng.choice(n, int(rng.integers(1, 17)), replace=False).tolist()) for _ in range(64)]
+    pr, it, _ = g.ppr(sets, max_iter=50, tol=0.0)
+    assert (it == 50).all()
+    assert np.abs(pr.sum(axis=1) - 1.0).max() < 1e-9
+    assert (pr >= 0).all()
