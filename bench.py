@@ -73,6 +73,8 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sections", default="batched_ppr,multi_graph,serving",
+                    help="secondary workloads reported beside the headline line ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -124,6 +126,19 @@ def main():
         log(f"cpu baseline: {args.cpu_iters} iterations, {threads} threads")
         cpu = cpu_baseline(g, n, m, args.cpu_iters, threads)
 
+    secondary = {}
+    wanted = [x for x in args.sections.split(",") if x]
+    if wanted:
+        from eggpu_amd import benchmarks
+
+        g.close()  # the secondary workloads build their own graphs
+        for name in wanted:
+            log(f"secondary workload: {name}")
+            t1 = time.perf_counter()
+            secondary[name] = getattr(benchmarks, name)()
+            secondary[name]["section_wall_s"] = round(time.perf_counter() - t1, 1)
+            log(f"{name}: {json.dumps(secondary[name])}")
+
     line = {
         "metric": "PageRank GTEPS & % HBM roofline; batched solves/s; p50/p99 latency @QPS",
         "value": round(value, 3),
@@ -167,6 +182,7 @@ def main():
             "traffic_source": traffic_src,
         },
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
 

@@ -1,0 +1,211 @@
+"""Batched and served PageRank workloads measured beside the plain method.
+
+Each function returns a JSON-ready dict.  Throughput is reported at a fixed
+iteration count (tol = 0) so both sides do the same work; the served form
+uses the reference's stopping rule (tol = 1e-6) because requests finish
+independently.  "plain" is the one-solve-per-call GPU path of this library
+on the same inputs: egx_graph_pagerank for a global solve, egx_graph_ppr with
+one column for a personalised one.
+"""
+from __future__ import annotations
+
+import sys
+import threading
+import time
+
+import numpy as np
+
+from . import native, workloads
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _seed_sets(rng, n, count, lo=1, hi=16):
+    return [sorted(rng.choice(n, int(rng.integers(lo, hi + 1)), replace=False).tolist()) for _ in range(count)]
+
+
+def batched_ppr(scale=22, columns=64, K=20, seed=1, plain_sample=64):
+    """64 personalised vectors on a weighted R-MAT graph in one batch, against
+    the same vectors solved one per call."""
+    g = native.DeviceGraph.rmat(scale, 16, seed=seed, weighted=True, wlo=0.1, whi=1.0, wseed=seed + 100)
+    n, m = g.n, int(g.info()["m"])
+    rng = np.random.default_rng(seed)
+    sets = _seed_sets(rng, n, columns)
+    g.ppr(sets[:2], max_iter=2, tol=0.0, download=False)  # warm
+    t0 = time.perf_counter()
+    _, it, ks = g.ppr(sets, max_iter=K, tol=0.0, download=False)
+    wall = time.perf_counter() - t0
+    assert (it == K).all()
+    sample = sets[: max(1, min(plain_sample, columns))]
+    with g.batch() as b:  # one resident single-column session: no per-call setup
+        b.assign(0, sample[0], max_iter=2, tol=0.0)
+        b.run(max_launches=4)
+        t0 = time.perf_counter()
+        for S in sample:
+            b.assign(0, S, max_iter=K, tol=0.0)
+            b.run(max_launches=K)
+        plain_wall = time.perf_counter() - t0
+    batch_rate = columns / wall
+    plain_rate = len(sample) / plain_wall
+    # bytes per iteration of the 64-wide launch: arcs (index + value), row
+    # pointer and degree per row, 64 columns read and written per row
+    bytes_iter = m * 8 + n * 8 + columns * n * 8
+    res = {
+        "workload": f"personalised PageRank, R-MAT scale {scale} weighted U(0.1,1), {columns} seed sets of 1-16 "
+                    f"nodes, fixed K={K}",
+        "n": n, "m": m,
+        "batch_solves_per_s": round(batch_rate, 3),
+        "batch_wall_s": round(wall, 4),
+        "batch_kernel_ms_per_iter": round(ks / K * 1e3, 4),
+        "batch_algorithmic_GBps": round(bytes_iter * K / ks / 1e9, 2),
+        "plain_solves_per_s": round(plain_rate, 3),
+        "plain_calls_timed": len(sample),
+        "speedup_vs_plain": round(batch_rate / plain_rate, 2),
+    }
+    g.close()
+    return res
+
+
+def multi_graph(count=512, K=20, seed=2024, plain_sample=512):
+    """count heterogeneous graphs in shared launches against one call per graph."""
+    specs = workloads.hetero_specs(count, seed=seed)
+    t0 = time.perf_counter()
+    graphs = [s.build(native) for s in specs]
+    build = time.perf_counter() - t0
+    arcs = sum(int(g.info()["m"]) for g in graphs)
+    _log(f"{count} graphs built in {build:.1f}s, {arcs} arcs")
+    native.multi_pagerank(graphs[:4], max_iter=2, tol=0.0, download=False)  # warm
+    t0 = time.perf_counter()
+    _, iters, st = native.multi_pagerank(graphs, max_iter=K, tol=0.0, download=False)
+    wall = time.perf_counter() - t0
+    assert (iters == K).all()
+    sample = graphs[: max(1, min(plain_sample, count))]
+    t0 = time.perf_counter()
+    for g in sample:
+        g.pagerank(0.85, K, 0.0, download=False)
+    plain_wall = time.perf_counter() - t0
+    # reference stopping rule, every graph stops on its own
+    t0 = time.perf_counter()
+    _, it_tol, _ = native.multi_pagerank(graphs, max_iter=100, tol=1e-6, download=False)
+    tol_wall = time.perf_counter() - t0
+    batch_rate = count / wall
+    plain_rate = len(sample) / plain_wall
+    res = {
+        "workload": f"{count} graphs, n log-uniform [1e3,1e6], mean out-degree U[4,32], half ER half R-MAT, "
+                    f"unweighted, fixed K={K}",
+        "arcs": arcs,
+        "batch_solves_per_s": round(batch_rate, 2),
+        "batch_wall_s": round(wall, 4),
+        "batch_gteps": round(arcs * K / wall / 1e9, 2),
+        "launch_workgroups": int(st["spmv_grid"]),
+        "plain_solves_per_s": round(plain_rate, 2),
+        "plain_calls_timed": len(sample),
+        "speedup_vs_plain": round(batch_rate / plain_rate, 2),
+        "tol1e-6_wall_s": round(tol_wall, 4),
+        "tol1e-6_iterations_range": [int(it_tol.min()), int(it_tol.max())],
+        "build_s": round(build, 2),
+    }
+    for g in graphs:
+        g.close()
+    return res
+
+
+def _serving_requests(graphs, count, seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(count):
+        gi = int(rng.integers(0, len(graphs)))
+        seeds = [] if rng.random() < 0.7 else _seed_sets(rng, graphs[gi].n, 1)[0]
+        reqs.append((gi, seeds, 0.85 if rng.random() < 0.5 else 0.9))
+    return reqs
+
+
+def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, step_iters=4):
+    """Poisson arrivals against a device-resident pool; p50/p99 latency at a
+    fixed QPS beside the closed-loop solve rate and the one-call-per-solve
+    rate on the same request stream."""
+    specs = workloads.serving_pool_specs(pool_size, seed=seed)
+    graphs = [s.build(native) for s in specs]
+    arcs = [int(g.info()["m"]) for g in graphs]
+    reqs = _serving_requests(graphs, requests, seed + 1)
+
+    sessions = {}
+
+    def plain(r):
+        gi, seeds, a = r
+        if seeds:  # a resident single-column session per graph: no per-call setup
+            b = sessions.get(gi) or sessions.setdefault(gi, graphs[gi].batch())
+            b.assign(0, seeds, alpha=a, max_iter=100, tol=1e-6)
+            b.run(max_launches=100)
+            b.result(0)
+        else:
+            graphs[gi].pagerank(a, 100, 1e-6, download=True)
+
+    for r in reqs[:8]:
+        plain(r)  # warm
+    sample = reqs[:plain_sample]
+    t0 = time.perf_counter()
+    for r in sample:
+        plain(r)
+    plain_rate = len(sample) / (time.perf_counter() - t0)
+    for b in sessions.values():
+        b.close()
+
+    def run(rate):
+        with native.Server(graphs, step_iters=step_iters) as srv:
+            ids = [None] * len(reqs)
+            gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=len(reqs)) if rate else None
+
+            def feed():
+                t = time.perf_counter()
+                for k, (gi, seeds, a) in enumerate(reqs):
+                    if gaps is not None:
+                        t += gaps[k]
+                        while True:
+                            d = t - time.perf_counter()
+                            if d <= 0:
+                                break
+                            if d > 3e-4:
+                                time.sleep(d - 2e-4)
+                    ids[k] = srv.submit(gi, seeds, alpha=a, max_iter=100, tol=1e-6)
+
+            t0 = time.perf_counter()
+            th = threading.Thread(target=feed)
+            th.start()
+            th.join()
+            lat, widths = [], []
+            for rid in ids:
+                _, info = srv.wait(rid)
+                lat.append(info["latency_seconds"])
+                widths.append(info["width"])
+            wall = time.perf_counter() - t0
+            st = srv.stats()
+        return len(reqs) / wall, np.array(lat), st
+
+    # warm the server path once (sessions, code objects)
+    with native.Server(graphs) as srv:
+        srv.wait(srv.submit(0))
+    closed_rate, closed_lat, closed_st = run(None)
+    target = qps if qps else 0.5 * closed_rate
+    rate, lat, st = run(target)
+    res = {
+        "workload": f"pool of {pool_size} R-MAT graphs ({min(arcs)}-{max(arcs)} arcs, every other weighted), "
+                    f"{requests} requests: 70% global, 30% personalised (1-16 seeds), alpha 0.85/0.9, tol 1e-6",
+        "qps_offered": round(target, 1),
+        "solves_per_s": round(rate, 1),
+        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+        "closed_loop_solves_per_s": round(closed_rate, 1),
+        "closed_loop_p50_ms": round(float(np.percentile(closed_lat, 50)) * 1e3, 3),
+        "closed_loop_p99_ms": round(float(np.percentile(closed_lat, 99)) * 1e3, 3),
+        "plain_solves_per_s": round(plain_rate, 1),
+        "plain_calls_timed": len(sample),
+        "speedup_vs_plain": round(closed_rate / plain_rate, 2),
+        "peak_active_columns": st["peak_active_columns"],
+        "sessions_opened": st["sessions_opened"],
+    }
+    for g in graphs:
+        g.close()
+    return res

@@ -1,4 +1,5 @@
 // egx_api.hip -- the C ABI declared in include/egx.h.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -233,15 +234,21 @@ int egx_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, doubl
         egx::set_last_error("R-MAT needs 1 <= scale <= 30 and edge_factor >= 0");
         return EGX_UNSUPPORTED_GRAPH;
     }
+    return egx_graph_rmat_n((int32_t)(1 << scale), (int64_t)edge_factor << scale, a, b, c, seed, weighted, wlo, whi,
+                            wseed, alpha, keep_csr, out);
+}
+
+int egx_graph_rmat_n(int32_t n, int64_t nedges, double a, double b, double c, uint64_t seed, int32_t weighted,
+                     double wlo, double whi, uint64_t wseed, double alpha, int32_t keep_csr, egx_graph** out) {
+    if (!out) return EGX_UNSUPPORTED_GRAPH;
+    *out = nullptr;
     std::unique_ptr<egx_graph> h;
     EGX_TRY(egx::new_graph(h));
     egx::DevBuf V, E;
     int64_t m = 0;
-    const int64_t draws = (int64_t)edge_factor << scale;
-    EGX_TRY(egx::gen_rmat_csr(scale, draws, seed, egx::threshold32(a), egx::threshold32(a + b),
+    EGX_TRY(egx::gen_rmat_csr(n, nedges, seed, egx::threshold32(a), egx::threshold32(a + b),
                               egx::threshold32(a + b + c), V, E, m, h->g.stream));
-    EGX_TRY(egx::finish_generated(h, V, E, m, (int32_t)(1 << scale), weighted != 0, wlo, whi, wseed, alpha,
-                                  keep_csr != 0));
+    EGX_TRY(egx::finish_generated(h, V, E, m, n, weighted != 0, wlo, whi, wseed, alpha, keep_csr != 0));
     *out = h.release();
     return EGX_SUCC;
 }
@@ -335,22 +342,20 @@ int egx_graph_download_csr(const egx_graph* gh, int32_t* V, int32_t* E, double* 
     return EGX_SUCC;
 }
 
-int egx_graph_pagerank_ex(egx_graph* gh, const egx_solve_opts* opts, double* PR, egx_solve_stats* stats,
-                          int32_t time_each_launch) {
-    if (!gh || !opts) return EGX_UNSUPPORTED_GRAPH;
-    Graph& g = gh->g;
-    std::lock_guard<std::mutex> lk(g.mu);
-    egx::DeviceGuard dg(g.device);
-    if (!dg.ok) return EGX_DEVICE_ERR;
+namespace {
+egx::SolveParams solve_params(const egx_solve_opts* opts, int32_t flags) {
     egx::SolveParams p;
     p.alpha = opts->alpha;
     p.max_iter = opts->max_iter;
     p.tol = opts->tol;
     p.precision = opts->precision;
     p.check_every = opts->check_every;
-    p.time_each_launch = time_each_launch != 0;
-    egx::SolveStats st;
-    int rc = egx::solve_pagerank(g, p, PR, &st);
+    p.time_each_launch = (flags & EGX_SOLVE_TIME_LAUNCHES) != 0;
+    p.use_graph = (flags & EGX_SOLVE_HIP_GRAPH) != 0 && !p.time_each_launch;
+    return p;
+}
+
+int after_launches(int rc) {
     if (rc == EGX_SUCC) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) {
@@ -358,14 +363,82 @@ int egx_graph_pagerank_ex(egx_graph* gh, const egx_solve_opts* opts, double* PR,
             rc = EGX_DEVICE_ERR;
         }
     }
-    if (stats) {
-        stats->kernel_seconds = st.kernel_seconds;
-        stats->spmv_seconds = st.spmv_seconds;
-        stats->reduce_seconds = st.reduce_seconds;
-        stats->iterations = st.iterations;
-        stats->spmv_launches = st.spmv_launches;
-        stats->spmv_grid = st.spmv_grid;
+    return rc;
+}
+
+void copy_stats(const egx::SolveStats& st, egx_solve_stats* stats) {
+    if (!stats) return;
+    stats->kernel_seconds = st.kernel_seconds;
+    stats->spmv_seconds = st.spmv_seconds;
+    stats->reduce_seconds = st.reduce_seconds;
+    stats->iterations = st.iterations;
+    stats->spmv_launches = st.spmv_launches;
+    stats->spmv_grid = st.spmv_grid;
+}
+}  // namespace
+
+int egx_graph_pagerank_ex(egx_graph* gh, const egx_solve_opts* opts, double* PR, egx_solve_stats* stats,
+                          int32_t flags) {
+    if (!gh || !opts) return EGX_UNSUPPORTED_GRAPH;
+    Graph& g = gh->g;
+    std::lock_guard<std::mutex> lk(g.mu);
+    egx::DeviceGuard dg(g.device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    egx::SolveStats st;
+    int rc = after_launches(egx::solve_pagerank(g, solve_params(opts, flags), PR, &st));
+    copy_stats(st, stats);
+    return rc;
+}
+
+int egx_multi_pagerank(egx_graph* const* graphs, int32_t count, const egx_solve_opts* opts, double* const* PR,
+                       int32_t* iterations, egx_solve_stats* stats, int32_t flags) {
+    if (!opts || count < 0 || (count > 0 && !graphs)) {
+        egx::set_last_error("egx_multi_pagerank: bad arguments");
+        return EGX_UNSUPPORTED_GRAPH;
     }
+    if (count == 0) return EGX_SUCC;
+    std::vector<Graph*> gs((size_t)count);
+    for (int32_t i = 0; i < count; ++i) {
+        if (!graphs[i]) {
+            egx::set_last_error("egx_multi_pagerank: NULL graph handle");
+            return EGX_UNSUPPORTED_GRAPH;
+        }
+        gs[(size_t)i] = &graphs[i]->g;
+        if (gs[(size_t)i]->device != gs[0]->device) {
+            egx::set_last_error("egx_multi_pagerank: graphs live on different devices");
+            return EGX_DEVICE_ERR;
+        }
+    }
+    // lock each distinct graph once, in address order (no lock-order inversion
+    // between concurrent multi-graph calls)
+    std::vector<Graph*> order(gs);
+    std::sort(order.begin(), order.end());
+    order.erase(std::unique(order.begin(), order.end()), order.end());
+    if (order.size() != gs.size()) {
+        egx::set_last_error("egx_multi_pagerank: a graph appears twice in the set");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    std::vector<std::unique_lock<std::mutex>> locks;
+    locks.reserve(order.size());
+    for (Graph* g : order) locks.emplace_back(g->mu);
+    egx::DeviceGuard dg(gs[0]->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    hipStream_t s = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int rc = EGX_SUCC;
+    if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&ev0) != hipSuccess ||
+        hipEventCreate(&ev1) != hipSuccess) {
+        egx::set_last_error("egx_multi_pagerank: stream/event creation failed");
+        rc = EGX_DEVICE_ERR;
+    }
+    egx::SolveStats st;
+    if (rc == EGX_SUCC)
+        rc = after_launches(egx::solve_pagerank_multi(gs, solve_params(opts, flags), PR, iterations, &st, s, ev0, ev1));
+    if (s) (void)hipStreamSynchronize(s);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (s) (void)hipStreamDestroy(s);
+    copy_stats(st, stats);
     return rc;
 }
 

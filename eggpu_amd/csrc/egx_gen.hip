@@ -30,7 +30,9 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 
 constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
 
-__global__ void k_rmat_keys(int32_t scale, int64_t nedges, uint64_t seed, uint32_t tA, uint32_t tAB,
+// Draw e descends `scale` quadrant levels; a draw that lands on a self-loop
+// or on an endpoint >= n (n not a power of two) becomes the sentinel key.
+__global__ void k_rmat_keys(int32_t scale, int32_t n, int64_t nedges, uint64_t seed, uint32_t tA, uint32_t tAB,
                             uint32_t tABC, uint64_t* __restrict__ keys) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nedges) return;
@@ -44,7 +46,8 @@ __global__ void k_rmat_keys(int32_t scale, int64_t nedges, uint64_t seed, uint32
         d = (d << 1) | (q & 1u);
     }
     const uint64_t sentinel = (2 * scale >= 64) ? ~0ull : ((1ull << (2 * scale)) - 1ull);
-    keys[e] = (s == d) ? sentinel : (((uint64_t)s << scale) | (uint64_t)d);
+    const bool drop = (s == d) || s >= (uint32_t)n || d >= (uint32_t)n;
+    keys[e] = drop ? sentinel : (((uint64_t)s << scale) | (uint64_t)d);
 }
 
 __global__ void k_er_keys(int32_t n, int32_t bits, int64_t nedges, uint64_t seed, uint64_t* __restrict__ keys) {
@@ -151,12 +154,13 @@ int keys_to_csr(DevBuf& keys, int64_t nedges, int32_t n, int32_t bits, DevBuf& V
 
 }  // namespace
 
-int gen_rmat_csr(int32_t scale, int64_t nedges, uint64_t seed, uint32_t tA, uint32_t tAB, uint32_t tABC, DevBuf& V,
+int gen_rmat_csr(int32_t n, int64_t nedges, uint64_t seed, uint32_t tA, uint32_t tAB, uint32_t tABC, DevBuf& V,
                  DevBuf& E, int64_t& m, hipStream_t s) {
-    if (scale < 1 || scale > 30) {
-        set_last_error("R-MAT scale must be in [1, 30]");
+    if (n < 2 || n > (1 << 30)) {
+        set_last_error("R-MAT node count must be in [2, 2^30]");
         return EGX_UNSUPPORTED_GRAPH;
     }
+    const int32_t scale = bits_for((uint64_t)n - 1);
     if (nedges < 0 || nedges > (int64_t)0xFFFFFFFFll) {
         set_last_error("R-MAT draw count must be in [0, 2^32)");
         return EGX_UNSUPPORTED_GRAPH;
@@ -164,10 +168,10 @@ int gen_rmat_csr(int32_t scale, int64_t nedges, uint64_t seed, uint32_t tA, uint
     DevBuf keys;
     EGX_TRY(keys.alloc((size_t)std::max<int64_t>(nedges, 1) * 8));
     if (nedges > 0) {
-        k_rmat_keys<<<nblk(nedges, 256), 256, 0, s>>>(scale, nedges, seed, tA, tAB, tABC, keys.as<uint64_t>());
+        k_rmat_keys<<<nblk(nedges, 256), 256, 0, s>>>(scale, n, nedges, seed, tA, tAB, tABC, keys.as<uint64_t>());
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     }
-    return keys_to_csr(keys, nedges, (int32_t)(1 << scale), scale, V, E, m, s);
+    return keys_to_csr(keys, nedges, n, scale, V, E, m, s);
 }
 
 int gen_er_csr(int32_t n, int64_t nedges, uint64_t seed, DevBuf& V, DevBuf& E, int64_t& m, hipStream_t s) {

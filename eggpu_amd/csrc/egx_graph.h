@@ -141,7 +141,7 @@ int refresh_alpha(Graph& g, double alpha);
 int download_csc(Graph& g, int32_t* in_row, int32_t* in_col, float* in_val, int32_t* dangling01);
 
 // egx_gen.hip: device generators producing an out-CSR (V, E, W) in device memory.
-int gen_rmat_csr(int32_t scale, int64_t nedges, uint64_t seed, uint32_t tA, uint32_t tAB, uint32_t tABC,
+int gen_rmat_csr(int32_t n, int64_t nedges, uint64_t seed, uint32_t tA, uint32_t tAB, uint32_t tABC,
                  DevBuf& V, DevBuf& E, int64_t& m, hipStream_t s);
 int gen_er_csr(int32_t n, int64_t nedges, uint64_t seed, DevBuf& V, DevBuf& E, int64_t& m, hipStream_t s);
 int gen_edge_weights(const int32_t* dV, const int32_t* dE, int32_t n, int64_t m, uint64_t wseed, double lo,
@@ -155,6 +155,7 @@ struct SolveParams {
     int32_t precision = EGX_PRECISION_FP32;
     int32_t check_every = 0;
     bool time_each_launch = false;  // hipEvent pair around every launch
+    bool use_graph = false;         // replay captured runs of check_every iterations
 };
 struct SolveStats {
     double kernel_seconds = 0.0;  // first to last iteration launch
@@ -165,6 +166,10 @@ struct SolveStats {
     int64_t spmv_grid = 0;
 };
 int solve_pagerank(Graph& g, const SolveParams& p, double* host_out, SolveStats* stats);
+// Many graphs (same device, all weighted or all unweighted) in one launch per
+// iteration on stream s; each graph stops on its own.
+int solve_pagerank_multi(std::vector<Graph*>& gs, const SolveParams& p, double* const* host_out, int32_t* iters,
+                         SolveStats* st, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1);
 
 }  // namespace egx
 

@@ -10,7 +10,7 @@
 //   err   = sum_v |x'[v] - x[v]|,  dang' = sum_{v dangling} x'[v]
 //   stop when err < tol*n (pagerank.py:131), decided on the device.
 //
-// k_pr_spmv, one workgroup per work item:
+// SpMV, one workgroup per work item:
 //   * hub segment (rows with in-degree above the hub threshold): a fixed
 //     slice of one row, 8 gathers in flight per thread, FP64 block sum
 //     written to hub_part;
@@ -20,15 +20,26 @@
 //     per thread), then reduces rows from LDS -- one thread per row when the
 //     tile's mean row length is at most thread_row_max, one wavefront per
 //     row with a wave64 shuffle reduction otherwise -- and finalises them.
-// k_pr_reduce (one workgroup) finalises hub rows from their segment partials
-// in segment order and folds the per-block (err, sum, dang) slots in index
-// order: results are run-to-run deterministic and no launch uses a global
-// atomic (device-scope atomics on one address serialise across the 8 XCDs).
-// Row sums, the teleport term, the dangling mass and the L1 residual are
-// FP64; iterates are stored as T (float or double).  Unweighted graphs keep
-// s = x/outdeg beside x, so each arc costs one 4-byte index and one gather.
+// Reduce (one workgroup per graph) finalises hub rows from their segment
+// partials in segment order and folds the per-block (err, sum, dang) slots
+// in index order: results are run-to-run deterministic and no launch uses a
+// global atomic (device-scope atomics on one address serialise across the 8
+// XCDs).  Row sums, the teleport term, the dangling mass and the L1 residual
+// are FP64; iterates are stored as T (float or double).  Unweighted graphs
+// keep s = x/outdeg beside x, so each arc costs one 4-byte index and one
+// gather.
+//
+// The ping-pong buffer of an iteration is chosen on the device from the
+// graph's own iteration counter, so kernel arguments never change between
+// iterations: a run of iterations can be captured once in a hipGraph and
+// replayed, and many graphs -- each stopping on its own -- can share one
+// launch through a table of per-graph arguments (k_pr_spmv_multi /
+// k_pr_reduce_multi run the same block code as the one-graph kernels, so a
+// graph's result is bit-identical alone and in a batch).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "egx_graph.h"
@@ -52,7 +63,9 @@ struct IterArgs {
     const HubSeg* hsegs;
     const HubRow* hrows;
     double* hub_part;
-    const T* x_in;
+    T* xb[2];          // iterate buffers; state->iter parity selects the input
+    T* sb[2];          // contribution buffers (unweighted)
+    const T* x_in;     // bound per launch
     const T* s_in;
     T* x_out;
     T* s_out;
@@ -67,6 +80,16 @@ struct IterArgs {
     double tol_n;
     int32_t max_iter;
 };
+
+template <typename T>
+__device__ __forceinline__ void bind(IterArgs<T>& a) {
+    // selects, not an index: a runtime index would move the argument block to scratch
+    const bool odd = (a.state->iter & 1) != 0;
+    a.x_in = odd ? a.xb[1] : a.xb[0];
+    a.x_out = odd ? a.xb[0] : a.xb[1];
+    a.s_in = odd ? a.sb[1] : a.sb[0];
+    a.s_out = odd ? a.sb[0] : a.sb[1];
+}
 
 // The gathered term of an arc from u: x[u] * in_val (weighted) or
 // s[u] = x[u]/outdeg[u].
@@ -96,15 +119,11 @@ __device__ __forceinline__ double jump_of(const IterArgs<T>& a) {
     return (1.0 - a.alpha) / n_d + a.alpha * a.state->dang / n_d;  // pagerank.py:125
 }
 
-// col / val are streamed once per iteration: non-temporal loads keep them
-// from evicting the gathered vector from L2.
+// One SpMV work item.  col / val are streamed once per iteration:
+// non-temporal loads keep them from evicting the gathered vector from L2.
 template <typename T, bool W>
-__global__ __launch_bounds__(kThreads) void k_pr_spmv(IterArgs<T> a) {
-    __shared__ T lds[kTileLds];
-    __shared__ double sh_red[3 * kWavesPerBlock];
-    if (a.state->done) return;  // uniform: written by an earlier launch
+__device__ __forceinline__ void spmv_block(const IterArgs<T>& a, uint32_t b, T* lds, double* sh_red) {
     const double jump = jump_of(a);
-    const uint32_t b = blockIdx.x;
     const int t = threadIdx.x;
 
     if (b < a.nblk_h) {
@@ -195,14 +214,11 @@ __global__ __launch_bounds__(kThreads) void k_pr_spmv(IterArgs<T> a) {
     }
 }
 
-// Launch 2 of an iteration (one workgroup): finalise hub rows from their
-// segment partials (segment order), fold every partial slot in index order,
-// then update the state and take the stop decision err < tol*n
-// (pagerank.py:131) on the device.
+// Finalise hub rows from their segment partials (segment order), fold every
+// partial slot in index order, update the state and take the stop decision
+// err < tol*n (pagerank.py:131) on the device.  One workgroup per graph.
 template <typename T, bool W>
-__global__ __launch_bounds__(kReduceThreads) void k_pr_reduce(IterArgs<T> a) {
-    __shared__ double sh_red[3 * (kReduceThreads / kWave)];
-    if (a.state->done) return;
+__device__ __forceinline__ void reduce_block(const IterArgs<T>& a, double* sh_red) {
     const double jump = jump_of(a);
     const int t = threadIdx.x;
     double e = 0.0, su = 0.0, dg = 0.0;
@@ -233,6 +249,47 @@ __global__ __launch_bounds__(kReduceThreads) void k_pr_reduce(IterArgs<T> a) {
     }
 }
 
+template <typename T, bool W>
+__global__ __launch_bounds__(kThreads) void k_pr_spmv(IterArgs<T> a) {
+    __shared__ T lds[kTileLds];
+    __shared__ double sh_red[3 * kWavesPerBlock];
+    if (a.state->done) return;  // uniform: written by an earlier launch
+    bind(a);
+    spmv_block<T, W>(a, blockIdx.x, lds, sh_red);
+}
+
+template <typename T, bool W>
+__global__ __launch_bounds__(kReduceThreads) void k_pr_reduce(IterArgs<T> a) {
+    __shared__ double sh_red[3 * (kReduceThreads / kWave)];
+    if (a.state->done) return;
+    bind(a);
+    reduce_block<T, W>(a, sh_red);
+}
+
+// Many graphs, one launch: block b works for graph blk_graph[b], as its
+// local block b - blk_base[graph].
+template <typename T, bool W>
+__global__ __launch_bounds__(kThreads) void k_pr_spmv_multi(const IterArgs<T>* __restrict__ table,
+                                                            const int32_t* __restrict__ blk_graph,
+                                                            const uint32_t* __restrict__ blk_base) {
+    __shared__ T lds[kTileLds];
+    __shared__ double sh_red[3 * kWavesPerBlock];
+    const int32_t gi = blk_graph[blockIdx.x];
+    IterArgs<T> a = table[gi];
+    if (a.state->done) return;
+    bind(a);
+    spmv_block<T, W>(a, blockIdx.x - blk_base[gi], lds, sh_red);
+}
+
+template <typename T, bool W>
+__global__ __launch_bounds__(kReduceThreads) void k_pr_reduce_multi(const IterArgs<T>* __restrict__ table) {
+    __shared__ double sh_red[3 * (kReduceThreads / kWave)];
+    IterArgs<T> a = table[blockIdx.x];
+    if (a.state->done) return;
+    bind(a);
+    reduce_block<T, W>(a, sh_red);
+}
+
 // x0 = 1/n (as T), s0 = x0/deg.
 template <typename T>
 __global__ void k_pr_init(const float* __restrict__ deg, int32_t n, T* __restrict__ x, T* __restrict__ s) {
@@ -247,11 +304,12 @@ __global__ void k_pr_init(const float* __restrict__ deg, int32_t n, T* __restric
 // out[perm[r]] = x[r] * (1/sum) in FP64, uniform when sum <= 0
 // (pagerank.py:134-139); perm maps iteration rows to caller node ids.
 template <typename T>
-__global__ void k_pr_output(const T* __restrict__ x, const PrState* __restrict__ st, const int32_t* __restrict__ perm,
-                            int32_t n, double* __restrict__ out) {
+__global__ void k_pr_output(const T* __restrict__ x0, const T* __restrict__ x1, const PrState* __restrict__ st,
+                            const int32_t* __restrict__ perm, int32_t n, double* __restrict__ out) {
     const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const double norm = st->sum;
+    const T* x = (st->iter & 1) ? x1 : x0;
     const int32_t v = perm ? perm[r] : r;
     out[v] = (norm > 0.0) ? (double)x[r] * (1.0 / norm) : 1.0 / (double)n;
 }
@@ -290,28 +348,36 @@ struct LaunchTimer {
     }
 };
 
-template <typename T, bool W>
-int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st) {
-    hipStream_t s = g.stream;
-    const int32_t n = g.n;
-    const uint32_t nblk_h = (uint32_t)g.n_hseg;
-    const uint32_t grid = nblk_h + (uint32_t)g.n_tiles;
+// A captured run of iterations, replayed until every graph has stopped.
+struct CapturedRun {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~CapturedRun() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+// Fills the launch arguments of graph g and resets its iterate to x0 on stream s.
+template <typename T>
+int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a) {
+    const uint32_t grid = (uint32_t)g.n_hseg + (uint32_t)g.n_tiles;
     if (grid == 0) {
         set_last_error("graph has no work items");
         return EGX_UNKNOW_ERROR;
     }
+    EGX_TRY(refresh_alpha(g, p.alpha));
     EGX_TRY(ensure_workspace(g, grid));
-
-    T* xb[2] = {g.x[0].as<T>(), g.x[1].as<T>()};
-    T* sb[2] = {g.s[0].as<T>(), g.s[1].as<T>()};
-    k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, xb[0], sb[0]);
+    const int32_t n = g.n;
+    k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, g.x[0].as<T>(), g.s[0].as<T>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-    PrState st0{};
-    st0.dang = (double)g.n_dangling * (double)(T)(1.0 / (double)n);
-    st0.sum = 1.0;
-    EGX_HIP_TRY(hipMemcpyAsync(g.state.p, &st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
-
-    IterArgs<T> a{};
+    // staged through the pinned state buffer: the copy may run after return
+    PrState* st0 = static_cast<PrState*>(g.host_state);
+    *st0 = PrState{};
+    st0->dang = (double)g.n_dangling * (double)(T)(1.0 / (double)n);
+    st0->sum = 1.0;
+    EGX_HIP_TRY(hipMemcpyAsync(g.state.p, st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    a = IterArgs<T>{};
     a.row = g.it_row;
     a.col = g.it_col;
     a.val = g.it_val;
@@ -320,19 +386,45 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
     a.hsegs = g.hsegs.as<HubSeg>();
     a.hrows = g.hrows.as<HubRow>();
     a.hub_part = g.hub_part.as<double>();
+    a.xb[0] = g.x[0].as<T>();
+    a.xb[1] = g.x[1].as<T>();
+    a.sb[0] = g.s[0].as<T>();
+    a.sb[1] = g.s[1].as<T>();
     a.state = g.state.as<PrState>();
     a.partials = g.partials.as<double>();
     a.n_hrows = g.n_hrows;
     a.n = n;
     a.thread_row_max = g.bins.thread_row_max;
-    a.nblk_h = nblk_h;
+    a.nblk_h = (uint32_t)g.n_hseg;
     a.grid = grid;
     a.alpha = p.alpha;
     a.tol_n = std::max(0.0, p.tol) * (double)n;
     a.max_iter = p.max_iter;
+    return EGX_SUCC;
+}
+
+// Writes the normalised result of graph g (and copies it to host_out).
+template <typename T>
+int emit_result(Graph& g, hipStream_t s, double* host_out) {
+    k_pr_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
+                                                   g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
+                                                   g.out.as<double>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    if (host_out)
+        EGX_HIP_TRY(hipMemcpyAsync(host_out, g.out.p, (size_t)g.n * 8, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+template <typename T, bool W>
+int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st) {
+    hipStream_t s = g.stream;
+    IterArgs<T> a;
+    EGX_TRY(prepare_graph<T>(g, p, s, a));
+    const uint32_t grid = a.grid;
 
     LaunchTimer timer;
     if (p.time_each_launch) EGX_TRY(timer.reserve((size_t)4 * p.max_iter));
+    CapturedRun cap;
     EGX_HIP_TRY(hipEventRecord(g.ev0, s), EGX_DEVICE_ERR);
     int32_t launched = 0;
     int32_t chunk = p.check_every > 0 ? p.check_every : 4;
@@ -340,20 +432,32 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
     PrState* hs = static_cast<PrState*>(g.host_state);
     while (launched < p.max_iter) {
         const int32_t todo = can_stop_early ? std::min(chunk, p.max_iter - launched) : (p.max_iter - launched);
-        for (int32_t q = 0; q < todo; ++q, ++launched) {
-            const int cur = launched & 1;
-            a.x_in = xb[cur];
-            a.s_in = sb[cur];
-            a.x_out = xb[cur ^ 1];
-            a.s_out = sb[cur ^ 1];
-            if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched], s), EGX_DEVICE_ERR);
-            k_pr_spmv<T, W><<<grid, kThreads, 0, s>>>(a);
-            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-            if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 1], s), EGX_DEVICE_ERR);
-            if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 2], s), EGX_DEVICE_ERR);
-            k_pr_reduce<T, W><<<1, kReduceThreads, 0, s>>>(a);
-            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-            if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 3], s), EGX_DEVICE_ERR);
+        if (p.use_graph && !p.time_each_launch && can_stop_early && todo == p.check_every) {
+            // replay one captured chunk of `check_every` iterations
+            if (!cap.exec) {
+                EGX_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
+                for (int32_t q = 0; q < todo; ++q) {
+                    k_pr_spmv<T, W><<<grid, kThreads, 0, s>>>(a);
+                    k_pr_reduce<T, W><<<1, kReduceThreads, 0, s>>>(a);
+                }
+                EGX_HIP_TRY(hipStreamEndCapture(s, &cap.graph), EGX_DEVICE_ERR);
+                EGX_HIP_TRY(hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0), EGX_DEVICE_ERR);
+            }
+            EGX_HIP_TRY(hipGraphLaunch(cap.exec, s), EGX_DEVICE_ERR);
+            launched += todo;
+        } else {
+            for (int32_t q = 0; q < todo; ++q, ++launched) {
+                if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched], s), EGX_DEVICE_ERR);
+                k_pr_spmv<T, W><<<grid, kThreads, 0, s>>>(a);
+                EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+                if (p.time_each_launch) {
+                    EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 1], s), EGX_DEVICE_ERR);
+                    EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 2], s), EGX_DEVICE_ERR);
+                }
+                k_pr_reduce<T, W><<<1, kReduceThreads, 0, s>>>(a);
+                EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+                if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 3], s), EGX_DEVICE_ERR);
+            }
         }
         if (launched >= p.max_iter) break;
         EGX_HIP_TRY(hipMemcpyAsync(hs, g.state.p, sizeof(PrState), hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
@@ -384,14 +488,107 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
             st->spmv_launches = iters;
         }
     }
-    // the final iterate sits in the buffer written by launch iters-1
-    T* xf = xb[iters & 1];
-    k_pr_output<T><<<nblk(n, 256), 256, 0, s>>>(xf, g.state.as<PrState>(),
-                                                 g.reordered ? g.perm.as<int32_t>() : nullptr, n,
-                                                 g.out.as<double>());
-    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-    if (host_out)
-        EGX_HIP_TRY(hipMemcpyAsync(host_out, g.out.p, (size_t)n * 8, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    EGX_TRY(emit_result<T>(g, s, host_out));
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+template <typename T, bool W>
+int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* const* host_out, int32_t* iters,
+                      SolveStats* st, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    const int32_t count = (int32_t)gs.size();
+    std::vector<IterArgs<T>> table((size_t)count);
+    std::vector<uint32_t> base((size_t)count);
+    uint64_t total = 0;
+    for (Graph* g : gs) EGX_HIP_TRY(hipStreamSynchronize(g->stream), EGX_DEVICE_ERR);
+    for (int32_t i = 0; i < count; ++i) {
+        EGX_TRY(prepare_graph<T>(*gs[(size_t)i], p, s, table[(size_t)i]));
+        base[(size_t)i] = (uint32_t)total;
+        total += table[(size_t)i].grid;
+    }
+    if (total > 0x7FFFFFFFull) {
+        set_last_error("too many work items for one launch");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    std::vector<int32_t> blk((size_t)total);
+    for (int32_t i = 0; i < count; ++i)
+        std::fill(blk.begin() + base[(size_t)i], blk.begin() + base[(size_t)i] + table[(size_t)i].grid, i);
+    DevBuf d_table, d_blk, d_base;
+    EGX_TRY(d_table.alloc(sizeof(IterArgs<T>) * (size_t)count));
+    EGX_TRY(d_blk.alloc(blk.size() * 4));
+    EGX_TRY(d_base.alloc(base.size() * 4));
+    EGX_HIP_TRY(hipMemcpyAsync(d_table.p, table.data(), sizeof(IterArgs<T>) * (size_t)count, hipMemcpyHostToDevice, s),
+                EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(d_blk.p, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(d_base.p, base.data(), base.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    // per-graph states are gathered into one pinned copy for polling
+    std::vector<PrState> hstates((size_t)count);
+    void* pinned = nullptr;
+    EGX_HIP_TRY(hipHostMalloc(&pinned, sizeof(PrState) * (size_t)count), EGX_FAILED_TO_ALLOCATE_HOST_MEM);
+    struct PinnedGuard {
+        void* p;
+        ~PinnedGuard() { (void)hipHostFree(p); }
+    } pg{pinned};
+    PrState* hs = static_cast<PrState*>(pinned);
+
+    const int32_t chunk = p.check_every > 0 ? p.check_every : 8;
+    CapturedRun cap;
+    // alpha refreshes run on each graph's own stream
+    for (Graph* g : gs) EGX_HIP_TRY(hipStreamSynchronize(g->stream), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipEventRecord(ev0, s), EGX_DEVICE_ERR);
+    int32_t launched = 0;
+    while (launched < p.max_iter) {
+        const int32_t todo = std::min(chunk, p.max_iter - launched);
+        if (p.use_graph && todo == chunk) {
+            if (!cap.exec) {
+                EGX_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
+                for (int32_t q = 0; q < chunk; ++q) {
+                    k_pr_spmv_multi<T, W><<<(unsigned)total, kThreads, 0, s>>>(
+                        d_table.as<IterArgs<T>>(), d_blk.as<int32_t>(), d_base.as<uint32_t>());
+                    k_pr_reduce_multi<T, W><<<(unsigned)count, kReduceThreads, 0, s>>>(d_table.as<IterArgs<T>>());
+                }
+                EGX_HIP_TRY(hipStreamEndCapture(s, &cap.graph), EGX_DEVICE_ERR);
+                EGX_HIP_TRY(hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0), EGX_DEVICE_ERR);
+            }
+            EGX_HIP_TRY(hipGraphLaunch(cap.exec, s), EGX_DEVICE_ERR);
+        } else {
+            for (int32_t q = 0; q < todo; ++q) {
+                k_pr_spmv_multi<T, W><<<(unsigned)total, kThreads, 0, s>>>(d_table.as<IterArgs<T>>(),
+                                                                          d_blk.as<int32_t>(), d_base.as<uint32_t>());
+                EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+                k_pr_reduce_multi<T, W><<<(unsigned)count, kReduceThreads, 0, s>>>(d_table.as<IterArgs<T>>());
+                EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+            }
+        }
+        launched += todo;
+        if (launched >= p.max_iter) break;
+        for (int32_t i = 0; i < count; ++i)
+            EGX_HIP_TRY(hipMemcpyAsync(hs + i, gs[(size_t)i]->state.p, sizeof(PrState), hipMemcpyDeviceToHost, s),
+                        EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+        bool all_done = true;
+        for (int32_t i = 0; i < count && all_done; ++i) all_done = hs[i].done != 0;
+        if (all_done) break;
+    }
+    EGX_HIP_TRY(hipEventRecord(ev1, s), EGX_DEVICE_ERR);
+    for (int32_t i = 0; i < count; ++i)
+        EGX_HIP_TRY(hipMemcpyAsync(hs + i, gs[(size_t)i]->state.p, sizeof(PrState), hipMemcpyDeviceToHost, s),
+                    EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    if (st) {
+        float ms = 0.0f;
+        EGX_HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1), EGX_DEVICE_ERR);
+        st->kernel_seconds = (double)ms * 1e-3;
+        st->spmv_grid = (int64_t)total;
+        st->spmv_launches = launched;
+        st->iterations = 0;
+        for (int32_t i = 0; i < count; ++i) st->iterations = std::max(st->iterations, hs[i].iter);
+    }
+    for (int32_t i = 0; i < count; ++i) {
+        if (iters) iters[i] = hs[i].iter;
+        EGX_TRY(emit_result<T>(*gs[(size_t)i], s, host_out ? host_out[i] : nullptr));
+    }
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
@@ -410,11 +607,54 @@ int solve_pagerank(Graph& g, const SolveParams& p_in, double* host_out, SolveSta
             for (int32_t i = 0; i < g.n; ++i) host_out[i] = 1.0 / (double)g.n;
         return EGX_SUCC;
     }
-    EGX_TRY(refresh_alpha(g, p.alpha));
     if (p.precision == EGX_PRECISION_FP64)
         return g.weighted ? solve_typed<double, true>(g, p, host_out, st)
                           : solve_typed<double, false>(g, p, host_out, st);
     return g.weighted ? solve_typed<float, true>(g, p, host_out, st) : solve_typed<float, false>(g, p, host_out, st);
+}
+
+int solve_pagerank_multi(std::vector<Graph*>& gs, const SolveParams& p_in, double* const* host_out, int32_t* iters,
+                         SolveStats* st, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+    SolveParams p = p_in;
+    if (p.max_iter < 1) p.max_iter = 1;
+    if (!(p.tol >= 0.0)) p.tol = 0.0;
+    if (st) *st = SolveStats{};
+    // graphs without arcs (or nodes) are answered here and not launched
+    std::vector<Graph*> work;
+    std::vector<double*> outs;
+    std::vector<int32_t> idx;
+    bool weighted = false;
+    for (size_t i = 0; i < gs.size(); ++i) {
+        Graph& g = *gs[i];
+        if (iters) iters[i] = 0;
+        if (g.n <= 0) continue;
+        if (g.m == 0) {
+            if (host_out && host_out[i])
+                for (int32_t v = 0; v < g.n; ++v) host_out[i][v] = 1.0 / (double)g.n;
+            continue;
+        }
+        if (!work.empty() && g.weighted != weighted) {
+            set_last_error("one multi-graph launch takes either weighted or unweighted graphs");
+            return EGX_UNSUPPORTED_GRAPH;
+        }
+        weighted = g.weighted;
+        work.push_back(&g);
+        outs.push_back(host_out ? host_out[i] : nullptr);
+        idx.push_back((int32_t)i);
+    }
+    if (work.empty()) return EGX_SUCC;
+    std::vector<int32_t> it(work.size(), 0);
+    int rc;
+    if (p.precision == EGX_PRECISION_FP64)
+        rc = weighted ? solve_multi_typed<double, true>(work, p, outs.data(), it.data(), st, s, ev0, ev1)
+                      : solve_multi_typed<double, false>(work, p, outs.data(), it.data(), st, s, ev0, ev1);
+    else
+        rc = weighted ? solve_multi_typed<float, true>(work, p, outs.data(), it.data(), st, s, ev0, ev1)
+                      : solve_multi_typed<float, false>(work, p, outs.data(), it.data(), st, s, ev0, ev1);
+    if (rc != EGX_SUCC) return rc;
+    if (iters)
+        for (size_t k = 0; k < work.size(); ++k) iters[idx[k]] = it[k];
+    return EGX_SUCC;
 }
 
 }  // namespace egx

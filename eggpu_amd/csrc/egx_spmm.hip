@@ -40,6 +40,7 @@ constexpr int kRedThreads = 1024;    // reduce launch
 constexpr int kRedWaves = kRedThreads / kWave;
 constexpr int kSpmmHubSeg = 2048;    // arcs per hub segment (one workgroup)
 constexpr int kUnroll = 8;
+constexpr int kMinWidth = 8;         // narrowest X row (columns)
 
 struct ColState {
     double dang;     // dangling mass of the current iterate
@@ -74,7 +75,7 @@ struct SpmmArgs {
     const HubSeg* hsegs;
     const HubRow* hrows;
     double* hub_part;          // [nseg][64]
-    float* xbuf[2];            // the two X buffers; ctl->cur selects the input
+    float* xbuf[2];            // the two X buffers (row stride = width); ctl->cur selects the input
     const float* x_in;         // set per launch from ctl->cur
     float* x_out;
     ColState* cols;
@@ -87,47 +88,72 @@ struct SpmmArgs {
     uint32_t n_waves;
 };
 
-__device__ __forceinline__ int32_t bcast_i(int32_t v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
-__device__ __forceinline__ float bcast_f(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-// Sum over arcs [beg, end) of nval * x_in[src][lane], in arc order.
+// Width-C launches: a wavefront holds G = 64/C lane groups of C columns;
+// lane = group*C + column.  Arc k of a row range (k counted from its first
+// arc) belongs to residue class q = k mod 8; every lane keeps R = 8/G FP64
+// accumulators (classes q = group + G*r), each summing its class in arc
+// order, and the 8 classes are combined in one fixed tree,
+// ((q0+q1)+(q2+q3))+((q4+q5)+(q6+q7)): a column's sums -- and so its
+// result -- are bit-identical at every width.
+template <int C>
 __device__ __forceinline__ double row_dot(const SpmmArgs& a, int32_t beg, int32_t end) {
+    constexpr int G = kWave / C;
+    constexpr int R = 8 / G;
     const int lane = lane_id();
-    double acc = 0.0;
+    const int grp = lane / C;
+    const int c = lane % C;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
     for (int32_t p0 = beg; p0 < end; p0 += kWave) {
         const int32_t p = p0 + lane;
         const bool ok = p < end;
         const int32_t u_l = ok ? __builtin_nontemporal_load(&a.col[p]) : 0;
         const float w_l = ok ? __builtin_nontemporal_load(&a.nval[p]) : 0.0f;
         const int32_t cnt = min(kWave, end - p0);
-        int32_t i = 0;
-        for (; i + kUnroll <= cnt; i += kUnroll) {
+        // lane takes chunk arcs k = grp + G*t, t = 0 .. C-1, eight loads in flight
+        for (int t0 = 0; t0 < C && G * t0 < cnt; t0 += kUnroll) {
             float g[kUnroll];
             float w[kUnroll];
 #pragma unroll
             for (int q = 0; q < kUnroll; ++q) {
-                const int32_t u = bcast_i(u_l, i + q);
-                w[q] = bcast_f(w_l, i + q);
-                g[q] = a.x_in[(size_t)u * kCols + lane];
+                const int k = grp + G * (t0 + q);
+                int32_t u;
+                if constexpr (G == 1) {
+                    u = __builtin_amdgcn_readlane(u_l, t0 + q);
+                    w[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), t0 + q));
+                } else {
+                    u = __shfl(u_l, k & (kWave - 1), kWave);
+                    w[q] = __shfl(w_l, k & (kWave - 1), kWave);
+                }
+                g[q] = (k < cnt) ? a.x_in[(size_t)u * C + c] : 0.0f;
+                if (k >= cnt) w[q] = 0.0f;
             }
 #pragma unroll
-            for (int q = 0; q < kUnroll; ++q) acc += (double)w[q] * (double)g[q];
-        }
-        for (; i < cnt; ++i) {
-            const int32_t u = bcast_i(u_l, i);
-            const float w = bcast_f(w_l, i);
-            acc += (double)w * (double)a.x_in[(size_t)u * kCols + lane];
+            for (int q = 0; q < kUnroll; ++q) acc[q % R] += (double)w[q] * (double)g[q];  // t0 % R == 0
         }
     }
-    return acc;
+    // classes differing in q bit 0, 1, 2 in that order: group bits are the
+    // low bits of q (lane xor C, 2C, ...), accumulator bits the high ones
+#pragma unroll
+    for (int off = C; off < kWave; off <<= 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) acc[r] += __shfl_xor(acc[r], off, kWave);
+    }
+#pragma unroll
+    for (int span = 1; span < R; span <<= 1) {
+#pragma unroll
+        for (int r = 0; r < R; r += 2 * span) acc[r] += acc[r + span];
+    }
+    return acc[0];
 }
 
-__device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, double c, int32_t v, double acc,
+// Lanes c < C finalise column c of row v.
+template <int C>
+__device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, double cj, int32_t v, double acc,
                                          double inv_n, double& err, double& sum, double& dang) {
-    const int lane = lane_id();
-    const size_t at = (size_t)v * kCols + lane;
+    const int c = lane_id();
+    const size_t at = (size_t)v * C + c;
     const float xo = a.x_in[at];
     if (!cs.active || cs.done) {
         a.x_out[at] = xo;
@@ -138,26 +164,29 @@ __device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, 
         tele = inv_n;
     } else {
         const int32_t ts = a.tslot[v];
-        tele = (ts >= 0) ? (double)a.ptab[(size_t)ts * kCols + lane] : 0.0;
+        tele = (ts >= 0) ? (double)a.ptab[(size_t)ts * kCols + c] : 0.0;
     }
-    const float xn = (float)(cs.alpha * acc + c * tele);
+    const float xn = (float)(cs.alpha * acc + cj * tele);
     err += fabs((double)xn - (double)xo);
     sum += (double)xn;
     if (a.deg[v] == 0.0f) dang += (double)xn;
     a.x_out[at] = xn;
 }
 
+template <int C>
 __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     __shared__ double sh[3][kRowWaves][kCols];
     if (!a.ctl->any_active) return;
     const int cur = a.ctl->cur;
-    a.x_in = a.xbuf[cur];
-    a.x_out = a.xbuf[cur ^ 1];
+    // selects, not an index: a runtime index would move the argument block to scratch
+    a.x_in = cur ? a.xbuf[1] : a.xbuf[0];
+    a.x_out = cur ? a.xbuf[0] : a.xbuf[1];
     const int lane = lane_id();
     const int w = wave_id();
     const uint32_t b = blockIdx.x;
-    const ColState cs = a.cols[lane];
-    const double c = (1.0 - cs.alpha) + cs.alpha * cs.dang;
+    const bool owner = lane < C;  // lanes that finalise a column
+    const ColState cs = a.cols[lane % C];
+    const double cj = (1.0 - cs.alpha) + cs.alpha * cs.dang;
     const double inv_n = 1.0 / (double)a.n;
 
     if (b < a.nblk_h) {
@@ -166,12 +195,12 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
         const int32_t len = sg.end - sg.beg;
         const int32_t q0 = sg.beg + (int32_t)(((int64_t)len * w) / kRowWaves);
         const int32_t q1 = sg.beg + (int32_t)(((int64_t)len * (w + 1)) / kRowWaves);
-        sh[0][w][lane] = row_dot(a, q0, q1);
+        sh[0][w][lane] = row_dot<C>(a, q0, q1);
         __syncthreads();
         if (w == 0) {
             double tot = 0.0;
             for (int k = 0; k < kRowWaves; ++k) tot += sh[0][k][lane];
-            a.hub_part[(size_t)b * kCols + lane] = tot;
+            if (owner) a.hub_part[(size_t)b * kCols + lane] = tot;
             for (int q = 0; q < 3; ++q) a.partials[((size_t)b * 3 + q) * kCols + lane] = 0.0;
         }
         return;
@@ -181,8 +210,8 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     if (gw < a.n_waves) {
         const int32_t r0 = a.wave_rows[gw], r1 = a.wave_rows[gw + 1];
         for (int32_t v = r0; v < r1; ++v) {
-            const double acc = row_dot(a, a.row[v], a.row[v + 1]);
-            finalize(a, cs, c, v, acc, inv_n, err, sum, dang);
+            const double acc = row_dot<C>(a, a.row[v], a.row[v + 1]);
+            if (owner) finalize<C>(a, cs, cj, v, acc, inv_n, err, sum, dang);
         }
     }
     sh[0][w][lane] = err;
@@ -198,24 +227,28 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     }
 }
 
+template <int C>
 __global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
     __shared__ double sh[3][kRedWaves][kCols];
     __shared__ int sh_any;
     if (!a.ctl->any_active) return;
     const int cur = a.ctl->cur;
-    a.x_in = a.xbuf[cur];
-    a.x_out = a.xbuf[cur ^ 1];
+    // selects, not an index: a runtime index would move the argument block to scratch
+    a.x_in = cur ? a.xbuf[1] : a.xbuf[0];
+    a.x_out = cur ? a.xbuf[0] : a.xbuf[1];
     const int lane = lane_id();
     const int w = wave_id();
     const ColState cs = a.cols[lane];
-    const double c = (1.0 - cs.alpha) + cs.alpha * cs.dang;
+    const double cj = (1.0 - cs.alpha) + cs.alpha * cs.dang;
     const double inv_n = 1.0 / (double)a.n;
     double err = 0.0, sum = 0.0, dang = 0.0;
-    for (int64_t h = w; h < a.n_hrows; h += kRedWaves) {
-        const HubRow hr = a.hrows[h];
-        double acc = 0.0;
-        for (int32_t k = 0; k < hr.nseg; ++k) acc += a.hub_part[(size_t)(hr.first + k) * kCols + lane];
-        finalize(a, cs, c, hr.row, acc, inv_n, err, sum, dang);
+    if (lane < C) {
+        for (int64_t h = w; h < a.n_hrows; h += kRedWaves) {
+            const HubRow hr = a.hrows[h];
+            double acc = 0.0;
+            for (int32_t k = 0; k < hr.nseg; ++k) acc += a.hub_part[(size_t)(hr.first + k) * kCols + lane];
+            finalize<C>(a, cs, cj, hr.row, acc, inv_n, err, sum, dang);
+        }
     }
     for (uint32_t b = w; b < a.grid; b += kRedWaves) {
         err += a.partials[((size_t)b * 3 + 0) * kCols + lane];
@@ -235,7 +268,7 @@ __global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
             d += sh[2][k][lane];
         }
         ColState* st = &a.cols[lane];
-        if (st->active && !st->done) {
+        if (lane < C && st->active && !st->done) {
             const int32_t it = st->iter + 1;
             st->err = e;
             st->sum = s;
@@ -252,20 +285,30 @@ __global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
     }
 }
 
-// Column j of x := 1/n (networkx starts personalised runs from uniform).
-__global__ void k_spmm_init_col(float* __restrict__ x, int32_t n, int32_t j) {
+// Column j of x (row stride C) := 1/n (networkx starts personalised runs from uniform).
+__global__ void k_spmm_init_col(float* __restrict__ x, int32_t n, int32_t stride, int32_t j) {
     const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < n) x[(size_t)v * kCols + j] = (float)(1.0 / (double)n);
+    if (v < n) x[(size_t)v * stride + j] = (float)(1.0 / (double)n);
+}
+
+// dst (row stride cd) := src (row stride cs) for columns < min(cs, cd), 0 beyond.
+__global__ void k_spmm_repack(const float* __restrict__ src, int32_t cs, float* __restrict__ dst, int32_t cd,
+                              int32_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * cd) return;
+    const int64_t v = i / cd;
+    const int32_t c = (int32_t)(i - v * cd);
+    dst[i] = (c < cs) ? src[v * cs + c] : 0.0f;
 }
 
 // out[perm[r]] = x[r][j] / sum_j
-__global__ void k_spmm_output(const float* __restrict__ x, const ColState* __restrict__ cols, const int32_t* perm,
-                              int32_t n, int32_t j, double* __restrict__ out) {
+__global__ void k_spmm_output(const float* __restrict__ x, int32_t stride, const ColState* __restrict__ cols,
+                              const int32_t* perm, int32_t n, int32_t j, double* __restrict__ out) {
     const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const double s = cols[j].sum;
     const int32_t v = perm ? perm[r] : r;
-    out[v] = (s > 0.0) ? (double)x[(size_t)r * kCols + j] * (1.0 / s) : 1.0 / (double)n;
+    out[v] = (s > 0.0) ? (double)x[(size_t)r * stride + j] * (1.0 / s) : 1.0 / (double)n;
 }
 
 __global__ void k_set_tslot(int32_t* __restrict__ tslot, const int32_t* __restrict__ rows,
@@ -322,6 +365,8 @@ struct Batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf X[2], nval, tslot, ptab, cols, ctl, partials, hub_part, hsegs, hrows, wave_rows, out;
     int cur = 0;
+    int32_t width = 0;     // columns per X row (8, 16, 32 or 64): covers every assigned slot
+    int32_t cap_cols = 0;  // columns the X buffers hold
     int32_t n = 0;
     uint32_t nblk_h = 0, grid = 0, n_waves = 0;
     int64_t n_hrows = 0;
@@ -380,9 +425,10 @@ int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
     }
     EGX_TRY(ensure_nval(g));
     hipStream_t s = b->stream;
+    b->width = b->cap_cols = kMinWidth;
     for (int i = 0; i < 2; ++i) {
-        EGX_TRY(b->X[i].alloc((size_t)n * kCols * 4));
-        EGX_HIP_TRY(hipMemsetAsync(b->X[i].p, 0, (size_t)n * kCols * 4, s), EGX_DEVICE_ERR);
+        EGX_TRY(b->X[i].alloc((size_t)n * kMinWidth * 4));
+        EGX_HIP_TRY(hipMemsetAsync(b->X[i].p, 0, (size_t)n * kMinWidth * 4, s), EGX_DEVICE_ERR);
     }
     EGX_TRY(b->tslot.alloc((size_t)n * 4));
     k_fill_i32<<<nblk(n, 256), 256, 0, s>>>(b->tslot.as<int32_t>(), n, -1);
@@ -474,11 +520,60 @@ int batch_sync_state(Batch& b) {
             k_set_tslot<<<nblk((int64_t)rows.size(), 256), 256, 0, s>>>(b.tslot.as<int32_t>(), dr.as<int32_t>(),
                                                                           ds.as<int32_t>(), (int32_t)rows.size());
             EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-            EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
         }
+        EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);  // before the staging vectors go away
         b.ptab_dirty = false;
     }
     return EGX_SUCC;
+}
+
+// Width covering slots [0, need): the next power of two >= need, at least 8.
+int32_t width_for(int32_t need) {
+    int32_t w = kMinWidth;
+    while (w < need) w <<= 1;
+    return w;
+}
+
+// Re-lay X for `want` columns per row: the current iterate is repacked into
+// the other buffer (or into new, larger buffers) and becomes current.  The
+// stream is idle afterwards.
+int batch_set_width(Batch& b, int32_t want) {
+    if (want == b.width) return EGX_SUCC;
+    EGX_TRY(batch_poll(b));
+    hipStream_t s = b.stream;
+    const int64_t total = (int64_t)b.n * want;
+    int next;
+    if (want > b.cap_cols) {
+        DevBuf nx[2];
+        EGX_TRY(nx[0].alloc((size_t)total * 4));
+        EGX_TRY(nx[1].alloc((size_t)total * 4));
+        k_spmm_repack<<<nblk(total, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), b.width, nx[0].as<float>(), want, b.n);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+        b.X[0] = std::move(nx[0]);
+        b.X[1] = std::move(nx[1]);
+        b.cap_cols = want;
+        next = 0;
+    } else {
+        k_spmm_repack<<<nblk(total, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), b.width, b.X[b.cur ^ 1].as<float>(),
+                                                        want, b.n);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        next = b.cur ^ 1;
+    }
+    BatchCtl* hc = reinterpret_cast<BatchCtl*>(static_cast<char*>(b.pinned) + sizeof(ColState) * kCols);
+    hc->cur = next;
+    EGX_HIP_TRY(hipMemcpyAsync(b.ctl.p, hc, sizeof(BatchCtl), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    b.cur = next;
+    b.width = want;
+    return EGX_SUCC;
+}
+
+int32_t batch_needed_width(const Batch& b) {
+    int32_t need = 0;
+    for (int32_t j = 0; j < kCols; ++j)
+        if (b.host_cols[(size_t)j].active) need = j + 1;
+    return width_for(need);
 }
 
 // Assign column j: seeds (original node ids) or none (global PageRank).
@@ -568,7 +663,8 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
     b.host_cols[(size_t)j] = cs;
     if (nseeds > 0) b.ptab_dirty = true;
     EGX_TRY(batch_sync_state(b));
-    k_spmm_init_col<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), g.n, j);
+    if (b.width <= j) EGX_TRY(batch_set_width(b, width_for(j + 1)));
+    k_spmm_init_col<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), g.n, b.width, j);
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpyAsync(b.cols.p, b.host_cols.data(), sizeof(ColState) * kCols, hipMemcpyHostToDevice, s),
                 EGX_DEVICE_ERR);
@@ -603,6 +699,9 @@ int batch_release(Batch& b, int32_t j) {
 int batch_step(Batch& b, int32_t iters, bool record_time) {
     Graph& g = *b.g;
     hipStream_t s = b.stream;
+    // narrow the launches when the high slots have been released
+    const int32_t want = batch_needed_width(b);
+    if (want != b.width) EGX_TRY(batch_set_width(b, want));
     SpmmArgs a{};
     a.row = g.it_row;
     a.col = g.it_col;
@@ -626,9 +725,19 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
     a.xbuf[0] = b.X[0].as<float>();
     a.xbuf[1] = b.X[1].as<float>();
     for (int32_t i = 0; i < iters; ++i) {
-        k_spmm_rows<<<b.grid, kRowThreads, 0, s>>>(a);
+        switch (b.width) {
+            case 8: k_spmm_rows<8><<<b.grid, kRowThreads, 0, s>>>(a); break;
+            case 16: k_spmm_rows<16><<<b.grid, kRowThreads, 0, s>>>(a); break;
+            case 32: k_spmm_rows<32><<<b.grid, kRowThreads, 0, s>>>(a); break;
+            default: k_spmm_rows<64><<<b.grid, kRowThreads, 0, s>>>(a); break;
+        }
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-        k_spmm_reduce<<<1, kRedThreads, 0, s>>>(a);
+        switch (b.width) {
+            case 8: k_spmm_reduce<8><<<1, kRedThreads, 0, s>>>(a); break;
+            case 16: k_spmm_reduce<16><<<1, kRedThreads, 0, s>>>(a); break;
+            case 32: k_spmm_reduce<32><<<1, kRedThreads, 0, s>>>(a); break;
+            default: k_spmm_reduce<64><<<1, kRedThreads, 0, s>>>(a); break;
+        }
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     }
     if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev1, s), EGX_DEVICE_ERR);
@@ -660,7 +769,11 @@ int batch_result(Batch& b, int32_t j, double* host_out) {
     Graph& g = *b.g;
     hipStream_t s = b.stream;
     EGX_TRY(batch_poll(b));  // the device knows which buffer is current
-    k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), b.cols.as<ColState>(),
+    if (j >= b.width) {
+        set_last_error("batch slot holds no request");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), b.width, b.cols.as<ColState>(),
                                                   g.reordered ? g.perm.as<int32_t>() : nullptr, g.n, j,
                                                   b.out.as<double>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
@@ -812,6 +925,11 @@ int egx_graph_ppr(egx_graph* gh, int32_t k, const int32_t* seed_offsets, const i
     egx_batch* bh = nullptr;
     EGX_TRY(egx_batch_open(gh, &bh));
     std::unique_ptr<egx_batch, void (*)(egx_batch*)> guard(bh, egx_batch_close);
+    {
+        BatchDevice dg(bh->b->device);
+        if (!dg.ok) return EGX_DEVICE_ERR;
+        EGX_TRY(egx::batch_set_width(*bh->b, egx::width_for(k)));  // one layout for all k columns
+    }
     for (int32_t j = 0; j < k; ++j) {
         const int32_t b0 = seed_offsets[j], b1 = seed_offsets[j + 1];
         if (b1 < b0) return EGX_UNSUPPORTED_GRAPH;

@@ -30,6 +30,8 @@ EGX_NEGATIVE_CYCLE = 6
 
 PRECISION_FP32 = 0
 PRECISION_FP64 = 1
+SOLVE_TIME_LAUNCHES = 1
+SOLVE_HIP_GRAPH = 2
 
 
 class NativeLibraryMissing(RuntimeError):
@@ -108,6 +110,28 @@ class SolveStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_sessions", ctypes.c_int32),
+                ("step_iters", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("width", ctypes.c_int32), ("queue_seconds", ctypes.c_double),
+                ("service_seconds", ctypes.c_double), ("latency_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "failed", "steps", "sessions_opened",
+                                              "sessions_closed", "peak_active_columns", "peak_session_bytes",
+                                              "budget_bytes")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class CacheStats(ctypes.Structure):
     _fields_ = [
         ("hits", ctypes.c_uint64),
@@ -140,6 +164,10 @@ SIGNATURES = {
                                       ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int32,
                                       ctypes.POINTER(_vp)]),
+    "egx_graph_rmat_n": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int32,
+                                        ctypes.POINTER(_vp)]),
     "egx_graph_er": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int32,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
                                     ctypes.c_int32, ctypes.POINTER(_vp)]),
@@ -150,7 +178,16 @@ SIGNATURES = {
     "egx_graph_pagerank": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, _i32p, _f64p]),
     "egx_graph_pagerank_ex": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, ctypes.POINTER(SolveStats),
                                              ctypes.c_int32]),
+    "egx_multi_pagerank": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(SolveOpts),
+                                          ctypes.POINTER(_f64p), _i32p, ctypes.POINTER(SolveStats), ctypes.c_int32]),
     "egx_graph_free": (None, [_vp]),
+    "egx_server_open": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(ServerOpts),
+                                       ctypes.POINTER(_vp)]),
+    "egx_server_submit": (ctypes.c_int, [_vp, ctypes.c_int32, _i32p, ctypes.c_int32, ctypes.c_double,
+                                         ctypes.c_int32, ctypes.c_double, _f64p, _i64p]),
+    "egx_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(RequestInfo)]),
+    "egx_server_stats_get": (ctypes.c_int, [_vp, ctypes.POINTER(ServerStats)]),
+    "egx_server_close": (None, [_vp]),
     "egx_batch_open": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "egx_batch_assign": (ctypes.c_int, [_vp, ctypes.c_int32, _i32p, ctypes.c_int32, ctypes.c_double,
                                         ctypes.c_int32, ctypes.c_double]),
@@ -258,6 +295,17 @@ class DeviceGraph:
         return cls(h.value)
 
     @classmethod
+    def rmat_n(cls, n, nedges, a=0.57, b=0.19, c=0.19, seed=1, weighted=False, wlo=0.1, whi=1.0, wseed=0,
+               alpha=0.85, keep_csr=False):
+        """R-MAT on any n >= 2 nodes with ``nedges`` draws (endpoints >= n dropped)."""
+        h = _vp()
+        _check(lib().egx_graph_rmat_n(int(n), int(nedges), float(a), float(b), float(c),
+                                      int(seed) & (2**64 - 1), int(bool(weighted)), float(wlo), float(whi),
+                                      int(wseed) & (2**64 - 1), float(alpha), int(bool(keep_csr)),
+                                      ctypes.byref(h)))
+        return cls(h.value)
+
+    @classmethod
     def er(cls, n, nedges, seed=1, weighted=False, wlo=0.1, whi=1.0, wseed=0, alpha=0.85, keep_csr=False):
         h = _vp()
         _check(lib().egx_graph_er(int(n), int(nedges), int(seed) & (2**64 - 1), int(bool(weighted)),
@@ -323,15 +371,18 @@ class DeviceGraph:
         return out, int(it.value), float(ks.value)
 
     def pagerank_stats(self, alpha=0.85, max_iter=200, tol=1e-6, precision=PRECISION_FP32, time_each_launch=True,
-                       download=False):
+                       download=False, hip_graph=False, check_every=0):
         """Like :meth:`pagerank` but returns (ranks or None, stats dict) with
-        per-launch kernel timings (hipEvents on the graph's stream)."""
+        per-launch kernel timings (hipEvents on the graph's stream) when
+        ``time_each_launch``; ``hip_graph`` replays captured runs of
+        ``check_every`` iterations instead of launching each kernel."""
         n = self.n
         out = np.empty(n, dtype=np.float64) if download else None
         st = SolveStats()
-        o = SolveOpts(float(alpha), int(max_iter), float(tol), int(precision), 0)
+        o = SolveOpts(float(alpha), int(max_iter), float(tol), int(precision), int(check_every))
+        flags = (SOLVE_TIME_LAUNCHES if time_each_launch else 0) | (SOLVE_HIP_GRAPH if hip_graph else 0)
         _check(lib().egx_graph_pagerank_ex(self.handle, ctypes.byref(o), _ptr(out, _f64p), ctypes.byref(st),
-                                           int(bool(time_each_launch))))
+                                           flags))
         return out, st.as_dict()
 
     def ppr(self, seed_sets, alpha=0.85, max_iter=200, tol=1e-6, alphas=None, download=True):
@@ -438,6 +489,80 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Server:
+    """Continuous-batching PageRank server over a pool of DeviceGraphs
+    (egx_server_*).  submit() queues a request and returns its id; wait()
+    returns (ranks float64[n] or None, info dict)."""
+
+    def __init__(self, graphs, hbm_budget_bytes=0, max_sessions=0, step_iters=0):
+        self._graphs = list(graphs)
+        k = len(self._graphs)
+        handles = (_vp * max(k, 1))(*[g.handle for g in self._graphs])
+        o = ServerOpts(int(hbm_budget_bytes), int(max_sessions), int(step_iters))
+        h = _vp()
+        _check(lib().egx_server_open(handles, k, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h.value
+        self._outs = {}
+
+    def submit(self, graph, seeds=(), alpha=0.85, max_iter=100, tol=1e-6, download=True):
+        seeds = np.ascontiguousarray(seeds, dtype=np.int32)
+        out = np.empty(self._graphs[graph].n, dtype=np.float64) if download else None
+        rid = ctypes.c_int64(0)
+        _check(lib().egx_server_submit(self._h, int(graph), _ptr(seeds, _i32p) if len(seeds) else None, len(seeds),
+                                       float(alpha), int(max_iter), float(tol), _ptr(out, _f64p),
+                                       ctypes.byref(rid)))
+        self._outs[rid.value] = out
+        return rid.value
+
+    def wait(self, request_id, timeout=-1.0):
+        info = RequestInfo()
+        rc = lib().egx_server_wait(self._h, int(request_id), float(timeout), ctypes.byref(info))
+        if rc != 0 and timeout >= 0:
+            _check(rc)  # may still be running: its buffer stays referenced until close()
+        out = self._outs.pop(request_id, None)
+        _check(rc)
+        return out, info.as_dict()
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().egx_server_stats_get(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().egx_server_close(self._h)  # joins the worker: no request writes after this
+            self._h = None
+            self._outs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def multi_pagerank(graphs, alpha=0.85, max_iter=200, tol=1e-6, precision=PRECISION_FP32, check_every=0,
+                   hip_graph=True, download=True):
+    """Global PageRank of many device-resident graphs in shared launches.
+    Returns (list of float64[n_i] or None, iterations int32[k], stats dict)."""
+    k = len(graphs)
+    handles = (_vp * max(k, 1))(*[g.handle for g in graphs])
+    outs = [np.empty(g.n, dtype=np.float64) for g in graphs] if download else None
+    out_ptrs = (_f64p * max(k, 1))(*[_ptr(o, _f64p) for o in outs]) if download else None
+    iters = np.zeros(max(k, 1), dtype=np.int32)
+    st = SolveStats()
+    o = SolveOpts(float(alpha), int(max_iter), float(tol), int(precision), int(check_every))
+    _check(lib().egx_multi_pagerank(handles, k, ctypes.byref(o), out_ptrs, _ptr(iters, _i32p), ctypes.byref(st),
+                                    SOLVE_HIP_GRAPH if hip_graph else 0))
+    return outs, iters[:k], st.as_dict()
 
 
 def pagerank_csr(V, E, W=None, alpha=0.85, max_iter=200, tol=1e-6, graph_id=0):

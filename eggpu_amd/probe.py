@@ -6,8 +6,9 @@
         one plain copy of B bytes, then K PageRank iterations on R-MAT
         scale S: the workload rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
         use, so the copy's known bytes calibrate the SpMV's counters
-    python -m eggpu_amd.probe stats   DB [--out FILE]
+    python -m eggpu_amd.probe stats   TRACE [--out FILE] [--markdown] [--min-us U]
         per-kernel duration summary of a rocprofv3 --kernel-trace database
+        or *_kernel_trace.csv
     python -m eggpu_amd.probe traffic --fetch DB --write DB --bytes B [--kernel K] [--out FILE]
         HBM bytes per launch of kernel K from the two --pmc databases of
         `calib`, scaled by what the counters read for the B-byte copy
@@ -77,18 +78,55 @@ def _kernel_rows(db):
     return rows
 
 
-def cmd_stats(a):
+def _trace_durations(path):
+    """{kernel name: [duration ns per dispatch]} from a rocprofv3 kernel
+    trace: a --kernel-trace database or a *_kernel_trace.csv."""
     per = {}
-    for db in _dbs(a.db):
+    if path.endswith(".csv"):
+        import csv
+
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                per.setdefault(r["Kernel_Name"], []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        return per
+    for db in _dbs(path):
         for r in _kernel_rows(db):
-            d = per.setdefault(r["name"], [])
-            d.append(r["duration"])
+            per.setdefault(r["name"], []).append(r["duration"])
+    return per
+
+
+def _short(name):
+    name = name.replace("egx::(anonymous namespace)::", "").replace("void ", "")
+    return name.split("(")[0] if "(" in name else name
+
+
+def cmd_stats(a):
+    """Per-kernel durations.  Launches that find the device stop flag set
+    return in a few microseconds; --min-us separates them from the launches
+    that iterate, so the mean of the latter compares with the hipEvent
+    timings that bench.py reports."""
     out = []
-    for name, durs in per.items():
-        out.append({"kernel": name, "calls": len(durs), "total_us": sum(durs) / 1e3,
-                    "mean_us": sum(durs) / len(durs) / 1e3, "min_us": min(durs) / 1e3, "max_us": max(durs) / 1e3})
+    for name, durs in _trace_durations(a.db).items():
+        work = [d for d in durs if d >= a.min_us * 1e3]
+        out.append({"kernel": _short(name), "calls": len(durs), "total_us": sum(durs) / 1e3,
+                    "mean_us": sum(durs) / len(durs) / 1e3, "min_us": min(durs) / 1e3, "max_us": max(durs) / 1e3,
+                    "calls_ge_min": len(work), "mean_ge_min_us": (sum(work) / len(work) / 1e3) if work else None})
     out.sort(key=lambda r: -r["total_us"])
-    text = json.dumps({"source": a.db, "kernels": out}, indent=1)
+    if a.markdown:
+        lines = [f"# {a.title or 'kernel durations'}", ""]
+        if a.command:
+            lines += ["Command (from the repository root):", "", "```", a.command, "```", ""]
+        lines += [f"Source: `{a.db}`; launches shorter than {a.min_us:g} us (stop flag already set) are "
+                  "excluded from the last two columns.", "",
+                  f"| kernel | calls | total us | mean us | min us | max us | calls >= {a.min_us:g} us | mean of those us |",
+                  "|---|---:|---:|---:|---:|---:|---:|---:|"]
+        for r in out[: a.top]:
+            m = "-" if r["mean_ge_min_us"] is None else f"{r['mean_ge_min_us']:.1f}"
+            lines.append(f"| `{r['kernel']}` | {r['calls']} | {r['total_us']:.1f} | {r['mean_us']:.1f} | "
+                         f"{r['min_us']:.1f} | {r['max_us']:.1f} | {r['calls_ge_min']} | {m} |")
+        text = "\n".join(lines)
+    else:
+        text = json.dumps({"source": a.db, "kernels": out}, indent=1)
     if a.out:
         with open(a.out, "w") as fh:
             fh.write(text + "\n")
@@ -179,6 +217,11 @@ def main(argv=None):
     p = sub.add_parser("stats")
     p.add_argument("db")
     p.add_argument("--out")
+    p.add_argument("--min-us", type=float, default=100.0)
+    p.add_argument("--markdown", action="store_true")
+    p.add_argument("--title")
+    p.add_argument("--command")
+    p.add_argument("--top", type=int, default=20)
     p = sub.add_parser("traffic")
     p.add_argument("--fetch", required=True)
     p.add_argument("--write", required=True)

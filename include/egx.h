@@ -141,6 +141,13 @@ int egx_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, doubl
                    uint64_t seed, int32_t weighted, double wlo, double whi, uint64_t wseed,
                    double alpha, int32_t keep_csr, egx_graph **out);
 
+/* R-MAT on n nodes (any n >= 2): draws descend ceil(log2 n) levels and a
+ * draw with an endpoint >= n is dropped like a self-loop.  For n = 2^scale and
+ * nedges = edge_factor * n this is egx_graph_rmat. */
+int egx_graph_rmat_n(int32_t n, int64_t nedges, double a, double b, double c, uint64_t seed,
+                     int32_t weighted, double wlo, double whi, uint64_t wseed, double alpha,
+                     int32_t keep_csr, egx_graph **out);
+
 /* Seeded Erdos-Renyi G(n, M) draw (M pairs, self-loops and duplicates removed). */
 int egx_graph_er(int32_t n, int64_t nedges, uint64_t seed, int32_t weighted, double wlo,
                  double whi, uint64_t wseed, double alpha, int32_t keep_csr, egx_graph **out);
@@ -165,7 +172,7 @@ int egx_graph_download_csr(const egx_graph *g, int32_t *V, int32_t *E, double *W
 int egx_graph_pagerank(egx_graph *g, const egx_solve_opts *opts, double *PR, int32_t *iterations,
                        double *kernel_seconds);
 
-/* Solve statistics; spmv/reduce sums are filled when time_each_launch != 0
+/* Solve statistics; spmv/reduce sums are filled with EGX_SOLVE_TIME_LAUNCHES
  * (a hipEvent pair around every launch on the graph's stream). */
 typedef struct {
     double kernel_seconds;  /* first to last iteration launch */
@@ -176,8 +183,22 @@ typedef struct {
     int64_t spmv_grid;      /* workgroups per SpMV launch */
 } egx_solve_stats;
 
+/* flags for egx_graph_pagerank_ex / egx_multi_pagerank */
+#define EGX_SOLVE_TIME_LAUNCHES 1 /* time every launch (no hipGraph replay) */
+#define EGX_SOLVE_HIP_GRAPH 2     /* capture check_every iterations once, replay the hipGraph */
+
 int egx_graph_pagerank_ex(egx_graph *g, const egx_solve_opts *opts, double *PR, egx_solve_stats *stats,
-                          int32_t time_each_launch);
+                          int32_t flags);
+
+/* Global PageRank on many device-resident graphs of one device at once (no
+ * counterpart in the reference, which solves one graph per call,
+ * pagerank.cu:591-664).  All graphs must be weighted or all unweighted; each
+ * stops on its own tol test and gets the result it would get alone
+ * (bit-identical: the same block code runs over a table of per-graph
+ * arguments, one SpMV and one reduce launch per iteration for the whole set).
+ * PR[i] (n_i doubles) and iterations[i] may be NULL; stats sums the set. */
+int egx_multi_pagerank(egx_graph *const *graphs, int32_t count, const egx_solve_opts *opts,
+                       double *const *PR, int32_t *iterations, egx_solve_stats *stats, int32_t flags);
 
 void egx_graph_free(egx_graph *g);
 
@@ -236,6 +257,63 @@ int egx_probe_copy(int64_t bytes, int32_t reps, double *seconds_per_copy);
 /* n_gathers random 4-byte loads from a working set of working_set_floats
  * floats; mean seconds per pass over reps. */
 int egx_probe_gather(int64_t working_set_floats, int64_t n_gathers, int32_t reps, double *seconds_per_pass);
+
+/* ------------------------------------------------------------------------ */
+/* Serving (no counterpart in the reference, whose pagerank.py:141-179 solves */
+/* one graph per call): a worker thread answers global and personalised      */
+/* PageRank requests against a pool of device-resident graphs by continuous  */
+/* batching.  Each graph with work gets one batch session (64 slots, own     */
+/* stream); queued requests join running sessions at iteration-step          */
+/* boundaries; sessions open while the HBM budget holds them and idle ones   */
+/* close least-recently-used first; busy sessions on different graphs run    */
+/* concurrently.  A result is bit-identical alone and in any mix.            */
+/* ------------------------------------------------------------------------ */
+typedef struct egx_server egx_server; /* opaque */
+
+typedef struct {
+    int64_t hbm_budget_bytes; /* session memory the server may hold; <= 0: 80% of free HBM at open */
+    int32_t max_sessions;     /* concurrently open sessions (streams); <= 0: 16 */
+    int32_t step_iters;       /* iterations enqueued per session between polls; <= 0: 4 */
+} egx_server_opts;
+
+typedef struct {
+    int32_t iterations;
+    int32_t width;            /* X row width (8..64 columns) of the session at completion */
+    double queue_seconds;     /* submit -> admitted into a slot */
+    double service_seconds;   /* admitted -> result in the caller's buffer */
+    double latency_seconds;   /* submit -> result in the caller's buffer */
+} egx_request_info;
+
+typedef struct {
+    int64_t submitted;
+    int64_t completed;
+    int64_t failed;
+    int64_t steps;               /* scheduling rounds that launched work */
+    int64_t sessions_opened;
+    int64_t sessions_closed;
+    int64_t peak_active_columns; /* requests resident on the device at once */
+    int64_t peak_session_bytes;
+    int64_t budget_bytes;
+} egx_server_stats;
+
+/* The pool's graphs must stay alive (and not be re-binned) until close. */
+int egx_server_open(egx_graph *const *pool, int32_t count, const egx_server_opts *opts, egx_server **out);
+
+/* Queue a request on pool[graph]: nseeds = 0 is global PageRank, else the
+ * teleport and dangling mass go uniformly to the seed nodes.  PR (n doubles,
+ * or NULL) receives the sum-normalised ranks; it must stay valid until
+ * egx_server_wait returns for this request. */
+int egx_server_submit(egx_server *s, int32_t graph, const int32_t *seeds, int32_t nseeds, double alpha,
+                      int32_t max_iter, double tol, double *PR, int64_t *request_id);
+
+/* Wait for a request (timeout_seconds < 0: no limit) and collect it; returns
+ * the request's status.  A request is collected once. */
+int egx_server_wait(egx_server *s, int64_t request_id, double timeout_seconds, egx_request_info *info);
+
+int egx_server_stats_get(egx_server *s, egx_server_stats *out);
+
+/* Stops the worker; requests still queued or running fail. */
+void egx_server_close(egx_server *s);
 
 #ifdef __cplusplus
 }

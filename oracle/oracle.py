@@ -357,6 +357,16 @@ def rmat_csr(scale, edge_factor=16, seed=1, a=0.57, b=0.19, c=0.19):
     return simple_csr_from_coo(n, src, dst)
 
 
+def rmat_n_csr(n, nedges, seed=1, a=0.57, b=0.19, c=0.19):
+    """R-MAT on any n >= 2 nodes: nedges draws of ceil(log2 n) quadrant
+    levels; draws with an endpoint >= n are dropped, as are self-loops and
+    duplicates.  For n = 2^scale this is rmat_csr."""
+    scale = max(int(n - 1).bit_length(), 1)
+    src, dst = rmat_coo(scale, nedges, seed, a, b, c)
+    keep = (src < n) & (dst < n)
+    return simple_csr_from_coo(n, src[keep], dst[keep])
+
+
 def er_csr(n, nedges, seed):
     src, dst = er_coo(n, nedges, seed)
     return simple_csr_from_coo(n, src, dst)

@@ -122,9 +122,37 @@ def test_continuous_reassignment(gpu_ready):
         assert np.array_equal(b.result(0), ref["A"][0])
         assert np.array_equal(b.result(5), ref["C"][0])
         b.release(5)
-        b.assign(9, B)
+        b.assign(9, B)  # widens the X rows from 8 to 16 columns while slot 0 holds a result
         b.run(max_launches=300)
         assert np.array_equal(b.result(9), ref["B"][0])
+        assert np.array_equal(b.result(0), ref["A"][0])
+        b.release(9)
+        b.assign(40, C)  # 64 columns
+        b.run(max_launches=300)
+        assert np.array_equal(b.result(40), ref["C"][0])
+        assert np.array_equal(b.result(0), ref["A"][0])
+        b.release(40)
+        b.step(1)  # narrows back to 8 columns; slot 0 keeps its result
+        assert np.array_equal(b.result(0), ref["A"][0])
+
+
+@pytest.mark.parametrize("slot", [0, 12, 31, 63])
+def test_column_bits_do_not_depend_on_width(gpu_ready, slot):
+    """One request alone at slot `slot` (X rows 8, 16, 32 or 64 columns wide)
+    gives the same bits as at slot 0."""
+    V, E = O.rmat_csr(12, 16, seed=21)
+    W = O.edge_weights(V, E, 2)
+    g = native.DeviceGraph.from_csr(V, E, W)
+    with g.batch() as b:
+        b.assign(0, [7, 70, 700], alpha=0.9, max_iter=60, tol=1e-7)
+        b.run(max_launches=100)
+        ref = b.result(0)
+        it_ref = b.poll()["iterations"][0]
+    with g.batch() as b:
+        b.assign(slot, [7, 70, 700], alpha=0.9, max_iter=60, tol=1e-7)
+        b.run(max_launches=100)
+        assert b.poll()["iterations"][slot] == it_ref
+        assert np.array_equal(b.result(slot), ref)
 
 
 def test_global_column_equals_plain_pagerank(gpu_ready):

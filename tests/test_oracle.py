@@ -285,3 +285,33 @@ def test_openmp_baseline_agrees_with_oracle(weighted):
     omp, it = O.pagerank_omp(V, E, W, 0.85, 100, 1e-6, threads=4)
     assert it == it_ref
     assert np.abs(omp / omp.sum() - ref).max() < 1e-12
+
+
+def test_rmat_n_is_rmat_at_powers_of_two_and_rejects_out_of_range():
+    V, E = O.rmat_n_csr(1 << 10, 16 << 10, seed=5)
+    Vs, Es = O.rmat_csr(10, 16, seed=5)
+    assert np.array_equal(V, Vs) and np.array_equal(E, Es)
+    # n = 1000: the draws of the 1024-node graph, kept when both ends are < 1000
+    V2, E2 = O.rmat_n_csr(1000, 16 << 10, seed=5)
+    src, dst = O.csr_rows(Vs, Es)
+    keep = (src < 1000) & (dst < 1000)
+    assert len(V2) == 1001 and V2[-1] == int(keep.sum())
+    assert np.array_equal(E2, dst[keep])
+
+
+def test_hetero_workload_specs():
+    from eggpu_amd import workloads
+
+    a = workloads.hetero_specs(512)
+    assert a == workloads.hetero_specs(512)
+    assert len(a) == 512 and sum(s.kind == "er" for s in a) == 256
+    ns = np.array([s.n for s in a])
+    deg = np.array([s.nedges / s.n for s in a])
+    assert ns.min() >= 1000 and ns.max() <= 1_000_000
+    assert 4.0 - 1e-3 <= deg.min() and deg.max() <= 32.0 + 1e-3
+    # log-uniform: about a third of the graphs per decade
+    dec = np.bincount(np.minimum(np.log10(ns).astype(int) - 3, 2), minlength=3) / 512
+    assert np.all(np.abs(dec - 1 / 3) < 0.08)
+    pool = workloads.serving_pool_specs(32)
+    assert len(pool) == 32 and any(s.weighted for s in pool) and not all(s.weighted for s in pool)
+    assert min(s.nedges for s in pool) >= 1e4 and max(s.nedges for s in pool) <= 5e7
