@@ -73,6 +73,7 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default bins")
     ap.add_argument("--sections", default="batched_ppr,multi_graph,serving",
                     help="secondary workloads reported beside the headline line ('' for none)")
     args = ap.parse_args()
@@ -95,6 +96,18 @@ def main():
     info = g.info()
     n, m = int(info["n"]), int(info["m"])
     log(f"n={n} m={m} dangling={info['n_dangling']} build {build_wall:.3f}s")
+
+    # bins chosen by the autotuner (coordinate descent from the defaults)
+    tuned = None
+    if not args.no_tune:
+        from eggpu_amd import autotune
+
+        t1 = time.perf_counter()
+        cfg, sec, trials = autotune.tune(g, passes=1, iters=4)
+        tuned = {"bins": cfg, "us_per_iter": round(sec * 1e6, 1), "trials": len(trials),
+                 "seconds": round(time.perf_counter() - t1, 2)}
+        log(f"autotuned bins {cfg} ({sec * 1e3:.3f} ms/iter, {len(trials)} trials)")
+        info = g.info()
 
     # warmup
     if args.warmup > 0:
@@ -160,6 +173,7 @@ def main():
             "iterate_dtype": "f32",
             "reductions": "f64 (dangling mass, L1 residual, teleport, row sums)",
             "bins": {k: info[k] for k in ("thread_row_max", "wave_row_max", "hub_segment", "tile_items")},
+            "autotune": tuned,
             "hub_rows": info["rows_hub"],
             "tiles": info["tiles"],
             "build_seconds": round(info["build_seconds"], 4),

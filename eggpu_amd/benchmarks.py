@@ -9,13 +9,15 @@ one column for a personalised one.
 """
 from __future__ import annotations
 
+import os
 import sys
+import tempfile
 import threading
 import time
 
 import numpy as np
 
-from . import native, workloads
+from . import autotune, native, workloads
 
 
 def _log(msg):
@@ -76,6 +78,12 @@ def multi_graph(count=512, K=20, seed=2024, plain_sample=512):
     build = time.perf_counter() - t0
     arcs = sum(int(g.info()["m"]) for g in graphs)
     _log(f"{count} graphs built in {build:.1f}s, {arcs} arcs")
+    t0 = time.perf_counter()
+    with tempfile.TemporaryDirectory() as d:
+        chosen = autotune.apply_to_set(graphs, cache=autotune.TuneCache(os.path.join(d, "bins.json")), passes=1,
+                                       iters=3)
+    tune_s = time.perf_counter() - t0
+    _log(f"autotuned {len(chosen)} graph families in {tune_s:.1f}s")
     native.multi_pagerank(graphs[:4], max_iter=2, tol=0.0, download=False)  # warm
     t0 = time.perf_counter()
     _, iters, st = native.multi_pagerank(graphs, max_iter=K, tol=0.0, download=False)
@@ -106,6 +114,8 @@ def multi_graph(count=512, K=20, seed=2024, plain_sample=512):
         "tol1e-6_wall_s": round(tol_wall, 4),
         "tol1e-6_iterations_range": [int(it_tol.min()), int(it_tol.max())],
         "build_s": round(build, 2),
+        "autotune_s": round(tune_s, 2),
+        "autotuned_families": len(chosen),
     }
     for g in graphs:
         g.close()
@@ -209,3 +219,22 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
     for g in graphs:
         g.close()
     return res
+
+
+def main(argv=None):
+    """python -m eggpu_amd.benchmarks [batched_ppr|multi_graph|serving] ... -- one JSON line each."""
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("sections", nargs="*", default=["batched_ppr", "multi_graph", "serving"])
+    args = ap.parse_args(argv)
+    native.lib()
+    for name in args.sections:
+        if name not in ("batched_ppr", "multi_graph", "serving"):
+            raise SystemExit(f"unknown section {name}")
+        print(json.dumps({name: globals()[name]()}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

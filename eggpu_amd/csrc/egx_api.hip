@@ -178,8 +178,9 @@ Graph::~Graph() {
 size_t Graph::device_bytes() const {
     const DevBuf* bufs[] = {&in_row, &in_col, &in_val, &w_csc,  &out_w,  &out_cnt, &deg,    &csr_V,
                             &csr_E,  &csr_W,  &perm,   &rank,   &p_row,  &p_col,   &p_val,  &p_w,
-                            &p_outw, &p_deg,  &p_nval, &tiles,  &hsegs,  &hrows,  &hub_part, &x[0],  &x[1],
-                            &s[0],   &s[1],   &state,  &partials, &out};
+                            &p_outw, &p_deg,  &p_nval, &tiles,  &t_col,  &t_pos,  &t_val,  &hsegs,
+                            &hrows,  &hub_part, &x[0],  &x[1],
+                            &s[0],   &s[1],   &state,  &partials, &red_part, &out};
     size_t total = 0;
     for (const DevBuf* b : bufs) total += b->bytes;
     return total;
@@ -282,6 +283,8 @@ int egx_graph_info_get(const egx_graph* gh, egx_graph_info* out) {
     out->hub_segment = g.bins.hub_segment;
     out->tile_items = g.bins.tile_items;
     out->hub_rows_first = g.n_hub;
+    out->tile_sort = g.tsorted ? 1 : 0;
+    out->hot_sources = g.bins.hot_sources;
     out->reordered = g.reordered ? 1 : 0;
     out->layout_seconds = g.layout_seconds;
     out->rows_thread = g.rows_thread;
@@ -310,6 +313,8 @@ int egx_graph_set_bins(egx_graph* gh, const egx_bin_opts* opts) {
     }
     if (opts->tile_items > 0) cfg.tile_items = opts->tile_items;
     if (opts->reorder >= 0) cfg.reorder = opts->reorder ? 1 : 0;
+    if (opts->tile_sort >= 0) cfg.tile_sort = opts->tile_sort ? 1 : 0;
+    if (opts->hot_sources != 0) cfg.hot_sources = opts->hot_sources > 0 ? opts->hot_sources : 0;
     return egx::rebuild_bins(g, cfg);
 }
 

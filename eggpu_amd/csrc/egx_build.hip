@@ -191,6 +191,42 @@ __global__ void k_tile_starts(const int32_t* __restrict__ row, int32_t n_hub, in
     tiles[k] = lo;
 }
 
+// Sort keys of the tile arcs: (tile << 32 | source), payload = arc index.
+__global__ void k_tile_sort_keys(const int32_t* __restrict__ row, const int32_t* __restrict__ col,
+                                 const int32_t* __restrict__ tiles, uint64_t* __restrict__ keys,
+                                 uint32_t* __restrict__ vals, int64_t base) {
+    const int32_t k = blockIdx.x;
+    const int32_t a0 = row[tiles[k]], a1 = row[tiles[k + 1]];
+    for (int32_t p = a0 + (int32_t)threadIdx.x; p < a1; p += blockDim.x) {
+        keys[p - base] = ((uint64_t)(uint32_t)k << 32) | (uint64_t)(uint32_t)col[p];
+        vals[p - base] = (uint32_t)(p - base);
+    }
+}
+
+// t_col / t_pos (and t_val) of tile k from its sorted (key, arc) pairs.
+__global__ void k_tile_sorted_arrays(const int32_t* __restrict__ row, const int32_t* __restrict__ tiles,
+                                     const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                     const float* __restrict__ val, int64_t base, int32_t* __restrict__ t_col,
+                                     uint16_t* __restrict__ t_pos, float* __restrict__ t_val) {
+    const int32_t k = blockIdx.x;
+    const int32_t a0 = row[tiles[k]], a1 = row[tiles[k + 1]];
+    for (int32_t p = a0 + (int32_t)threadIdx.x; p < a1; p += blockDim.x) {
+        const int64_t arc = base + (int64_t)vals[p - base];
+        t_col[p] = (int32_t)(uint32_t)keys[p - base];
+        t_pos[p] = (uint16_t)(arc - a0);
+        if (t_val) t_val[p] = val[arc];
+    }
+}
+
+// t_val of every tile from t_pos (after an alpha change).
+__global__ void k_tile_vals(const int32_t* __restrict__ row, const int32_t* __restrict__ tiles,
+                            const uint16_t* __restrict__ t_pos, const float* __restrict__ val,
+                            float* __restrict__ t_val) {
+    const int32_t k = blockIdx.x;
+    const int32_t a0 = row[tiles[k]], a1 = row[tiles[k + 1]];
+    for (int32_t p = a0 + (int32_t)threadIdx.x; p < a1; p += blockDim.x) t_val[p] = val[a0 + t_pos[p]];
+}
+
 __global__ void k_layout_nodes(const uint32_t* __restrict__ perm, const int32_t* __restrict__ in_row,
                                const float* __restrict__ deg, const double* __restrict__ out_w, int32_t n,
                                int32_t* __restrict__ rank, uint32_t* __restrict__ cnt, float* __restrict__ p_deg,
@@ -237,6 +273,46 @@ inline unsigned nblocks(int64_t n, int threads) { return (unsigned)((n + threads
 
 }  // namespace
 
+// Source-ordered copies of the tile arcs (or none when cfg.tile_sort is 0).
+// A tile holds at most kTileLds arcs, so an arc's slot fits 16 bits.
+int build_tile_sort(Graph& g, const BinCfg& cfg) {
+    g.tsorted = false;
+    g.t_col.release();
+    g.t_pos.release();
+    g.t_val.release();
+    if (!cfg.tile_sort || g.n_tiles == 0) return EGX_SUCC;
+    hipStream_t s = g.stream;
+    int32_t base32 = 0;
+    EGX_HIP_TRY(hipMemcpy(&base32, g.it_row + g.n_hub, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    const int64_t base = base32;
+    const int64_t mt = g.m - base;
+    if (mt <= 0) return EGX_SUCC;
+    const size_t mm = (size_t)std::max<int64_t>(g.m, 1);
+    EGX_TRY(g.t_col.alloc(mm * 4));
+    EGX_TRY(g.t_pos.alloc(mm * 2));
+    if (g.weighted) EGX_TRY(g.t_val.alloc(mm * 4));
+    {
+        DevBuf keys, keys_alt, vals, vals_alt;
+        EGX_TRY(keys.alloc((size_t)mt * 8));
+        EGX_TRY(keys_alt.alloc((size_t)mt * 8));
+        EGX_TRY(vals.alloc((size_t)mt * 4));
+        EGX_TRY(vals_alt.alloc((size_t)mt * 4));
+        const unsigned nt = (unsigned)g.n_tiles;
+        k_tile_sort_keys<<<nt, 256, 0, s>>>(g.it_row, g.it_col, g.tiles.as<int32_t>(), keys.as<uint64_t>(),
+                                            vals.as<uint32_t>(), base);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        EGX_TRY(radix_sort_u64(keys.as<uint64_t>(), keys_alt.as<uint64_t>(), vals.as<uint32_t>(),
+                               vals_alt.as<uint32_t>(), mt, 32 + bits_for((uint64_t)g.n_tiles), s));
+        k_tile_sorted_arrays<<<nt, 256, 0, s>>>(g.it_row, g.tiles.as<int32_t>(), keys.as<uint64_t>(),
+                                                vals.as<uint32_t>(), g.it_val, base, g.t_col.as<int32_t>(),
+                                                g.t_pos.as<uint16_t>(), g.weighted ? g.t_val.as<float>() : nullptr);
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    }
+    g.tsorted = true;
+    return EGX_SUCC;
+}
+
 int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
     BinCfg cfg = cfg_in;
     if (cfg.thread_row_max < 0) cfg.thread_row_max = 0;
@@ -246,6 +322,7 @@ int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
     if (cfg.tile_items > kTileLds - cfg.wave_row_max) cfg.tile_items = kTileLds - cfg.wave_row_max;
     if (cfg.hub_segment < 256) cfg.hub_segment = 256;
     if (cfg.reorder != 0) cfg.reorder = 1;
+    if (cfg.tile_sort != 0) cfg.tile_sort = 1;
     if (!g.layout_built || g.layout_hub != cfg.wave_row_max || g.layout_reorder != cfg.reorder)
         EGX_TRY(build_layout(g, cfg));
     g.bins = cfg;
@@ -305,7 +382,7 @@ int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
         if (arcs > (int64_t)cfg.thread_row_max * r) g.rows_wave += r;
         else g.rows_thread += r;
     }
-    return EGX_SUCC;
+    return build_tile_sort(g, cfg);
 }
 
 int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const double* dW, int32_t n, int64_t m,
@@ -512,6 +589,12 @@ int refresh_alpha(Graph& g, double alpha) {
     if (g.m > 0) {
         k_refresh_vals<<<nblocks(g.m, 256), 256, 0, g.stream>>>(g.it_col, g.it_w, g.it_outw, g.m, alpha, g.it_val);
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        if (g.tsorted && g.n_tiles > 0) {
+            k_tile_vals<<<(unsigned)g.n_tiles, 256, 0, g.stream>>>(g.it_row, g.tiles.as<int32_t>(),
+                                                                   g.t_pos.as<uint16_t>(), g.it_val,
+                                                                   g.t_val.as<float>());
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        }
     }
     g.alpha_vals = alpha;
     if (!g.reordered) g.alpha_orig = alpha;  // in_val is the iteration array

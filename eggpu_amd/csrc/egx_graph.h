@@ -27,6 +27,9 @@
 //      tiles   int32[]     merge-path split of the other rows: tile k holds the
 //                          rows whose (row + arc) diagonal starts in
 //                          [k*tile_items, (k+1)*tile_items)
+//      t_col / t_pos / t_val  (tile_sort) each tile's arcs stably sorted by
+//                          source, with the arc's slot in row order: lanes
+//                          of one gather instruction then share cache lines
 // Solve workspace: x[2], s[2] iterate and contribution (x/deg) vectors,
 // double buffered, sized for FP64; partials (3 x grid, SoA), hub_part, out.
 #pragma once
@@ -74,6 +77,8 @@ struct BinCfg {
     int32_t hub_segment = 8192;   // arcs per hub segment (one workgroup)
     int32_t tile_items = 5888;    // rows + arcs per tile (merge-path split)
     int32_t reorder = 1;          // relabel by out-degree (hub rows always first)
+    int32_t tile_sort = 0;        // gather each tile's arcs in source order (LDS scatter back to row order)
+    int32_t hot_sources = 0;      // > 0: sources past this label are gathered non-temporally
 };
 
 struct Graph {
@@ -119,12 +124,17 @@ struct Graph {
     int32_t n_hub = 0;            // rows [0, n_hub) of the iteration layout are hub rows
     DevBuf tiles;
     int64_t n_tiles = 0;
+    // tile arcs in source order (tile_sort): t_col source, t_pos slot of the
+    // arc in its tile's row-order LDS staging, t_val weight (weighted); same
+    // arc index space as it_col, entries [row[n_hub], m) used
+    DevBuf t_col, t_pos, t_val;
+    bool tsorted = false;
     DevBuf hsegs, hrows, hub_part;
     int64_t n_hseg = 0, n_hrows = 0;
     int64_t rows_thread = 0, rows_wave = 0;
 
     // solve workspace (allocated on first use, sized for FP64)
-    DevBuf x[2], s[2], state, partials, out;
+    DevBuf x[2], s[2], state, partials, red_part, out;
     int64_t partial_blocks = 0;
     void* host_state = nullptr;  // pinned PrState
 

@@ -1,6 +1,6 @@
 // egx_pagerank.hip -- power iteration on the device-resident iteration layout.
 //
-// Two launches per iteration replace the reference's cuSPARSE SpMV, cuBLAS
+// Three launches per iteration replace the reference's cuSPARSE SpMV, cuBLAS
 // Sdot, two memsets, fused update/reduce kernel, scale kernel and two host
 // round trips (pagerank.cu:591-664):
 //
@@ -20,11 +20,11 @@
 //     per thread), then reduces rows from LDS -- one thread per row when the
 //     tile's mean row length is at most thread_row_max, one wavefront per
 //     row with a wave64 shuffle reduction otherwise -- and finalises them.
-// Reduce (one workgroup per graph) finalises hub rows from their segment
-// partials in segment order and folds the per-block (err, sum, dang) slots
-// in index order: results are run-to-run deterministic and no launch uses a
-// global atomic (device-scope atomics on one address serialise across the 8
-// XCDs).  Row sums, the teleport term, the dangling mass and the L1 residual
+// Reduce (a few workgroups, then one wavefront) finalises hub rows from
+// their segment partials in segment order and folds the per-block (err, sum,
+// dang) slots in a fixed order: results are run-to-run deterministic and no
+// launch uses a global atomic (device-scope atomics on one address serialise
+// across the 8 XCDs).  Row sums, the teleport term, the dangling mass and the L1 residual
 // are FP64; iterates are stored as T (float or double).  Unweighted graphs
 // keep s = x/outdeg beside x, so each arc costs one 4-byte index and one
 // gather.
@@ -50,7 +50,9 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kWavesPerBlock = kThreads / kWave;
-constexpr int kReduceThreads = 1024;
+constexpr int kReduceThreads = 256;  // per reduce-part workgroup
+constexpr int kRedItems = 512;       // hub rows + partial slots per reduce-part workgroup (target)
+constexpr int kMaxRed = 1024;        // reduce-part workgroups per graph, at most
 constexpr int kBatch = 8;  // gathers in flight per thread
 
 template <typename T>
@@ -60,6 +62,9 @@ struct IterArgs {
     const float* val;
     const float* deg;
     const int32_t* tiles;
+    const int32_t* tcol;    // tile arcs in source order (null: gather in row order)
+    const uint16_t* tpos;   // their slots in the row-order LDS staging
+    const float* tval;      // their weights (weighted)
     const HubSeg* hsegs;
     const HubRow* hrows;
     double* hub_part;
@@ -71,11 +76,14 @@ struct IterArgs {
     T* s_out;
     PrState* state;
     double* partials;  // SoA: err[grid], sum[grid], dang[grid]
+    double* red_part;  // SoA: err[n_red], sum[n_red], dang[n_red]
     int64_t n_hrows;
     int32_t n;
     int32_t thread_row_max;
     uint32_t nblk_h;
     uint32_t grid;
+    uint32_t n_red;    // reduce-part workgroups
+    int32_t cold_from; // first source label gathered non-temporally (n: none)
     double alpha;
     double tol_n;
     int32_t max_iter;
@@ -95,8 +103,11 @@ __device__ __forceinline__ void bind(IterArgs<T>& a) {
 // s[u] = x[u]/outdeg[u].
 template <typename T, bool W>
 __device__ __forceinline__ T term(const IterArgs<T>& a, int32_t u, float w) {
-    if (W) return (T)w * a.x_in[u];
-    return a.s_in[u];
+    // sources past the hot prefix are read non-temporally, so they do not
+    // evict the prefix that most arcs gather from
+    const T* v = W ? a.x_in : a.s_in;
+    const T g = (u >= a.cold_from) ? __builtin_nontemporal_load(&v[u]) : v[u];
+    return W ? (T)w * g : g;
 }
 
 template <typename T, bool W>
@@ -117,6 +128,35 @@ template <typename T>
 __device__ __forceinline__ double jump_of(const IterArgs<T>& a) {
     const double n_d = (double)a.n;
     return (1.0 - a.alpha) / n_d + a.alpha * a.state->dang / n_d;  // pagerank.py:125
+}
+
+// Gather the terms of a tile's arcs [a0, a0+na) into lds[] in row order.
+// S: the arcs are read in source order (neighbouring lanes then hit the same
+// cache lines) and each term is written to its row-order slot.
+template <typename T, bool W, bool S>
+__device__ __forceinline__ void stage_tile(const IterArgs<T>& a, int32_t a0, int32_t na, T* lds) {
+    const int32_t* col = S ? a.tcol : a.col;
+    const float* val = S ? a.tval : a.val;
+    const int t = threadIdx.x;
+    for (int32_t base = 0; base < na; base += kThreads * kBatch) {
+        int32_t u[kBatch];
+        float w[kBatch];
+        int32_t pos[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+            const int32_t j = base + q * kThreads + t;
+            const bool ok = j < na;
+            u[q] = ok ? __builtin_nontemporal_load(&col[a0 + j]) : -1;
+            w[q] = (W && ok) ? __builtin_nontemporal_load(&val[a0 + j]) : 0.0f;
+            pos[q] = S ? (ok ? (int32_t)__builtin_nontemporal_load(&a.tpos[a0 + j]) : 0) : j;
+        }
+        T g[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) g[q] = (u[q] >= 0) ? term<T, W>(a, u[q], w[q]) : (T)0;
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q)
+            if (u[q] >= 0) lds[pos[q]] = g[q];
+    }
 }
 
 // One SpMV work item.  col / val are streamed once per iteration:
@@ -165,25 +205,8 @@ __device__ __forceinline__ void spmv_block(const IterArgs<T>& a, uint32_t b, T* 
     const int32_t a0 = a.row[r0];
     const int32_t na = a.row[r1] - a0;
     const int32_t nr = r1 - r0;
-    for (int32_t base = 0; base < na; base += kThreads * kBatch) {
-        int32_t u[kBatch];
-        float w[kBatch];
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const int32_t j = base + q * kThreads + t;
-            const bool ok = j < na;
-            u[q] = ok ? __builtin_nontemporal_load(&a.col[a0 + j]) : -1;
-            w[q] = (W && ok) ? __builtin_nontemporal_load(&a.val[a0 + j]) : 0.0f;
-        }
-        T g[kBatch];
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) g[q] = (u[q] >= 0) ? term<T, W>(a, u[q], w[q]) : (T)0;
-#pragma unroll
-        for (int q = 0; q < kBatch; ++q) {
-            const int32_t j = base + q * kThreads + t;
-            if (j < na) lds[j] = g[q];
-        }
-    }
+    if (a.tpos) stage_tile<T, W, true>(a, a0, na, lds);
+    else stage_tile<T, W, false>(a, a0, na, lds);
     __syncthreads();
     double err = 0.0, sum = 0.0, dang = 0.0;
     if ((int64_t)na > (int64_t)a.thread_row_max * nr) {
@@ -214,15 +237,20 @@ __device__ __forceinline__ void spmv_block(const IterArgs<T>& a, uint32_t b, T* 
     }
 }
 
-// Finalise hub rows from their segment partials (segment order), fold every
-// partial slot in index order, update the state and take the stop decision
-// err < tol*n (pagerank.py:131) on the device.  One workgroup per graph.
+// Reduce, in two launches.  Part r (of n_red workgroups) finalises its share
+// of the hub rows from their segment partials (segment order) and folds its
+// share of the per-block partial slots; the final launch (one wavefront)
+// folds the n_red part sums in index order, updates the state and takes the
+// stop decision err < tol*n (pagerank.py:131) on the device.  Every sum has
+// a fixed order, so results are run-to-run deterministic.
 template <typename T, bool W>
-__device__ __forceinline__ void reduce_block(const IterArgs<T>& a, double* sh_red) {
+__device__ __forceinline__ void reduce_part_block(const IterArgs<T>& a, uint32_t r, double* sh_red) {
     const double jump = jump_of(a);
     const int t = threadIdx.x;
+    const int64_t stride = (int64_t)a.n_red * kReduceThreads;
+    const int64_t first = (int64_t)r * kReduceThreads + t;
     double e = 0.0, su = 0.0, dg = 0.0;
-    for (int64_t h = t; h < a.n_hrows; h += kReduceThreads) {
+    for (int64_t h = first; h < a.n_hrows; h += stride) {
         const HubRow hr = a.hrows[h];
         double row = 0.0;
         for (int32_t q = 0; q < hr.nseg; ++q) row += a.hub_part[hr.first + q];
@@ -231,14 +259,32 @@ __device__ __forceinline__ void reduce_block(const IterArgs<T>& a, double* sh_re
     const double* pe = a.partials;
     const double* ps = a.partials + a.grid;
     const double* pd = a.partials + 2 * (size_t)a.grid;
-#pragma unroll 8
-    for (uint32_t i = t; i < a.grid; i += kReduceThreads) {
+    for (int64_t i = first; i < (int64_t)a.grid; i += stride) {
         e += pe[i];
         su += ps[i];
         dg += pd[i];
     }
     block_sum3(e, su, dg, sh_red);
     if (t == 0) {
+        a.red_part[r] = e;
+        a.red_part[a.n_red + r] = su;
+        a.red_part[2 * (size_t)a.n_red + r] = dg;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void reduce_final_wave(const IterArgs<T>& a) {
+    const int l = lane_id();
+    double e = 0.0, su = 0.0, dg = 0.0;
+    for (uint32_t r = l; r < a.n_red; r += kWave) {
+        e += a.red_part[r];
+        su += a.red_part[a.n_red + r];
+        dg += a.red_part[2 * (size_t)a.n_red + r];
+    }
+    e = wave_sum(e);
+    su = wave_sum(su);
+    dg = wave_sum(dg);
+    if (l == 0) {
         PrState* st = a.state;
         const int32_t it = st->iter + 1;
         st->err = e;
@@ -263,7 +309,13 @@ __global__ __launch_bounds__(kReduceThreads) void k_pr_reduce(IterArgs<T> a) {
     __shared__ double sh_red[3 * (kReduceThreads / kWave)];
     if (a.state->done) return;
     bind(a);
-    reduce_block<T, W>(a, sh_red);
+    reduce_part_block<T, W>(a, blockIdx.x, sh_red);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_pr_final(IterArgs<T> a) {
+    if (a.state->done) return;
+    reduce_final_wave<T>(a);
 }
 
 // Many graphs, one launch: block b works for graph blk_graph[b], as its
@@ -282,12 +334,22 @@ __global__ __launch_bounds__(kThreads) void k_pr_spmv_multi(const IterArgs<T>* _
 }
 
 template <typename T, bool W>
-__global__ __launch_bounds__(kReduceThreads) void k_pr_reduce_multi(const IterArgs<T>* __restrict__ table) {
+__global__ __launch_bounds__(kReduceThreads) void k_pr_reduce_multi(const IterArgs<T>* __restrict__ table,
+                                                                    const int32_t* __restrict__ red_graph,
+                                                                    const uint32_t* __restrict__ red_base) {
     __shared__ double sh_red[3 * (kReduceThreads / kWave)];
-    IterArgs<T> a = table[blockIdx.x];
+    const int32_t gi = red_graph[blockIdx.x];
+    IterArgs<T> a = table[gi];
     if (a.state->done) return;
     bind(a);
-    reduce_block<T, W>(a, sh_red);
+    reduce_part_block<T, W>(a, blockIdx.x - red_base[gi], sh_red);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_pr_final_multi(const IterArgs<T>* __restrict__ table) {
+    const IterArgs<T> a = table[blockIdx.x];
+    if (a.state->done) return;
+    reduce_final_wave<T>(a);
 }
 
 // x0 = 1/n (as T), s0 = x0/deg.
@@ -328,6 +390,7 @@ int ensure_workspace(Graph& g, uint32_t grid) {
         EGX_TRY(g.partials.alloc((size_t)grid * 3 * 8));
         g.partial_blocks = grid;
     }
+    if (g.red_part.bytes < (size_t)kMaxRed * 3 * 8) EGX_TRY(g.red_part.alloc((size_t)kMaxRed * 3 * 8));
     if (!g.host_state) EGX_HIP_TRY(hipHostMalloc(&g.host_state, sizeof(PrState)), EGX_FAILED_TO_ALLOCATE_HOST_MEM);
     return EGX_SUCC;
 }
@@ -383,6 +446,11 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
     a.val = g.it_val;
     a.deg = g.it_deg;
     a.tiles = g.tiles.as<int32_t>();
+    if (g.tsorted) {
+        a.tcol = g.t_col.as<int32_t>();
+        a.tpos = g.t_pos.as<uint16_t>();
+        a.tval = g.weighted ? g.t_val.as<float>() : nullptr;
+    }
     a.hsegs = g.hsegs.as<HubSeg>();
     a.hrows = g.hrows.as<HubRow>();
     a.hub_part = g.hub_part.as<double>();
@@ -392,11 +460,14 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
     a.sb[1] = g.s[1].as<T>();
     a.state = g.state.as<PrState>();
     a.partials = g.partials.as<double>();
+    a.red_part = g.red_part.as<double>();
     a.n_hrows = g.n_hrows;
     a.n = n;
     a.thread_row_max = g.bins.thread_row_max;
+    a.cold_from = (g.bins.hot_sources > 0 && g.bins.hot_sources < n) ? g.bins.hot_sources : n;
     a.nblk_h = (uint32_t)g.n_hseg;
     a.grid = grid;
+    a.n_red = (uint32_t)std::min<int64_t>(kMaxRed, std::max<int64_t>(1, (g.n_hrows + grid + kRedItems - 1) / kRedItems));
     a.alpha = p.alpha;
     a.tol_n = std::max(0.0, p.tol) * (double)n;
     a.max_iter = p.max_iter;
@@ -438,7 +509,8 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
                 EGX_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
                 for (int32_t q = 0; q < todo; ++q) {
                     k_pr_spmv<T, W><<<grid, kThreads, 0, s>>>(a);
-                    k_pr_reduce<T, W><<<1, kReduceThreads, 0, s>>>(a);
+                    k_pr_reduce<T, W><<<a.n_red, kReduceThreads, 0, s>>>(a);
+                    k_pr_final<T><<<1, kWave, 0, s>>>(a);
                 }
                 EGX_HIP_TRY(hipStreamEndCapture(s, &cap.graph), EGX_DEVICE_ERR);
                 EGX_HIP_TRY(hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0), EGX_DEVICE_ERR);
@@ -454,7 +526,9 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
                     EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 1], s), EGX_DEVICE_ERR);
                     EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 2], s), EGX_DEVICE_ERR);
                 }
-                k_pr_reduce<T, W><<<1, kReduceThreads, 0, s>>>(a);
+                k_pr_reduce<T, W><<<a.n_red, kReduceThreads, 0, s>>>(a);
+                EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+                k_pr_final<T><<<1, kWave, 0, s>>>(a);
                 EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
                 if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 3], s), EGX_DEVICE_ERR);
             }
@@ -513,10 +587,24 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     std::vector<int32_t> blk((size_t)total);
     for (int32_t i = 0; i < count; ++i)
         std::fill(blk.begin() + base[(size_t)i], blk.begin() + base[(size_t)i] + table[(size_t)i].grid, i);
-    DevBuf d_table, d_blk, d_base;
+    // reduce-part workgroups: red_graph / red_base play the part of blk / base
+    std::vector<uint32_t> rbase((size_t)count);
+    uint64_t rtotal = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        rbase[(size_t)i] = (uint32_t)rtotal;
+        rtotal += table[(size_t)i].n_red;
+    }
+    std::vector<int32_t> rblk((size_t)rtotal);
+    for (int32_t i = 0; i < count; ++i)
+        std::fill(rblk.begin() + rbase[(size_t)i], rblk.begin() + rbase[(size_t)i] + table[(size_t)i].n_red, i);
+    DevBuf d_table, d_blk, d_base, d_rblk, d_rbase;
     EGX_TRY(d_table.alloc(sizeof(IterArgs<T>) * (size_t)count));
     EGX_TRY(d_blk.alloc(blk.size() * 4));
     EGX_TRY(d_base.alloc(base.size() * 4));
+    EGX_TRY(d_rblk.alloc(rblk.size() * 4));
+    EGX_TRY(d_rbase.alloc(rbase.size() * 4));
+    EGX_HIP_TRY(hipMemcpyAsync(d_rblk.p, rblk.data(), rblk.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(d_rbase.p, rbase.data(), rbase.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpyAsync(d_table.p, table.data(), sizeof(IterArgs<T>) * (size_t)count, hipMemcpyHostToDevice, s),
                 EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpyAsync(d_blk.p, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
@@ -546,7 +634,9 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
                 for (int32_t q = 0; q < chunk; ++q) {
                     k_pr_spmv_multi<T, W><<<(unsigned)total, kThreads, 0, s>>>(
                         d_table.as<IterArgs<T>>(), d_blk.as<int32_t>(), d_base.as<uint32_t>());
-                    k_pr_reduce_multi<T, W><<<(unsigned)count, kReduceThreads, 0, s>>>(d_table.as<IterArgs<T>>());
+                    k_pr_reduce_multi<T, W><<<(unsigned)rtotal, kReduceThreads, 0, s>>>(
+                        d_table.as<IterArgs<T>>(), d_rblk.as<int32_t>(), d_rbase.as<uint32_t>());
+                    k_pr_final_multi<T><<<(unsigned)count, kWave, 0, s>>>(d_table.as<IterArgs<T>>());
                 }
                 EGX_HIP_TRY(hipStreamEndCapture(s, &cap.graph), EGX_DEVICE_ERR);
                 EGX_HIP_TRY(hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0), EGX_DEVICE_ERR);
@@ -557,7 +647,10 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
                 k_pr_spmv_multi<T, W><<<(unsigned)total, kThreads, 0, s>>>(d_table.as<IterArgs<T>>(),
                                                                           d_blk.as<int32_t>(), d_base.as<uint32_t>());
                 EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-                k_pr_reduce_multi<T, W><<<(unsigned)count, kReduceThreads, 0, s>>>(d_table.as<IterArgs<T>>());
+                k_pr_reduce_multi<T, W><<<(unsigned)rtotal, kReduceThreads, 0, s>>>(
+                    d_table.as<IterArgs<T>>(), d_rblk.as<int32_t>(), d_rbase.as<uint32_t>());
+                EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+                k_pr_final_multi<T><<<(unsigned)count, kWave, 0, s>>>(d_table.as<IterArgs<T>>());
                 EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
             }
         }

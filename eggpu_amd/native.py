@@ -62,6 +62,9 @@ class GraphInfo(ctypes.Structure):
         ("tile_items", ctypes.c_int32),
         ("reordered", ctypes.c_int32),
         ("hub_rows_first", ctypes.c_int32),
+        ("tile_sort", ctypes.c_int32),
+        ("hot_sources", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
         ("rows_thread", ctypes.c_int64),
         ("rows_wave", ctypes.c_int64),
         ("rows_hub", ctypes.c_int64),
@@ -83,6 +86,8 @@ class BinOpts(ctypes.Structure):
         ("hub_segment", ctypes.c_int32),
         ("tile_items", ctypes.c_int32),
         ("reorder", ctypes.c_int32),
+        ("tile_sort", ctypes.c_int32),
+        ("hot_sources", ctypes.c_int32),
     ]
 
 
@@ -329,9 +334,13 @@ class DeviceGraph:
     def n(self):
         return int(self.info()["n"])
 
-    def set_bins(self, thread_row_max=0, wave_row_max=0, hub_segment=0, tile_items=0, reorder=-1):
-        """Re-bin rows (values <= 0 keep the current setting; reorder < 0 keeps it)."""
-        o = BinOpts(int(thread_row_max), int(wave_row_max), int(hub_segment), int(tile_items), int(reorder))
+    def set_bins(self, thread_row_max=0, wave_row_max=0, hub_segment=0, tile_items=0, reorder=-1, tile_sort=-1,
+                 hot_sources=0):
+        """Re-bin rows (values <= 0 keep the current setting; reorder and
+        tile_sort < 0 keep theirs; hot_sources 0 keeps it, < 0 turns
+        non-temporal cold gathers off)."""
+        o = BinOpts(int(thread_row_max), int(wave_row_max), int(hub_segment), int(tile_items), int(reorder),
+                    int(tile_sort), int(hot_sources))
         _check(lib().egx_graph_set_bins(self.handle, ctypes.byref(o)))
 
     def download_csc(self):

@@ -2,7 +2,7 @@
 
     python -m eggpu_amd.probe copy   [--bytes B] [--reps R]
     python -m eggpu_amd.probe gather [--n N] [--reps R]
-    python -m eggpu_amd.probe calib  [--scale S] [--iters K] [--bytes B]
+    python -m eggpu_amd.probe calib  [--scale S] [--iters K] [--bytes B] [--bins JSON]
         one plain copy of B bytes, then K PageRank iterations on R-MAT
         scale S: the workload rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs
         use, so the copy's known bytes calibrate the SpMV's counters
@@ -58,6 +58,8 @@ def cmd_gather(a):
 def cmd_calib(a):
     _copy(a.bytes, 1)
     g = native.DeviceGraph.rmat(a.scale, 16, seed=1)
+    if a.bins:
+        g.set_bins(**json.loads(a.bins))
     inf = g.info()
     _, st = g.pagerank_stats(0.85, a.iters, 0.0, time_each_launch=False)
     print(json.dumps({"probe": "calib", "copy_bytes": a.bytes, "n": inf["n"], "m": inf["m"],
@@ -214,6 +216,7 @@ def main(argv=None):
     p.add_argument("--scale", type=int, default=24)
     p.add_argument("--iters", type=int, default=3)
     p.add_argument("--bytes", type=int, default=1 << 30)
+    p.add_argument("--bins", help='JSON of egx_graph_set_bins fields, e.g. {"tile_sort": 1}')
     p = sub.add_parser("stats")
     p.add_argument("db")
     p.add_argument("--out")

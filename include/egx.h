@@ -97,6 +97,9 @@ typedef struct {
     int32_t tile_items;
     int32_t reordered;       /* 1: iteration layout relabelled by out-degree */
     int32_t hub_rows_first;  /* hub rows, stored first in the iteration layout */
+    int32_t tile_sort;       /* 1: tile arcs gathered in source order */
+    int32_t hot_sources;     /* gathers past this relabelled id are non-temporal (0: none) */
+    int32_t pad0;
     int64_t rows_thread;     /* rows reduced one thread per row */
     int64_t rows_wave;       /* rows reduced one wavefront per row */
     int64_t rows_hub;        /* rows of the hub bin (segmented, one workgroup per segment) */
@@ -119,6 +122,9 @@ typedef struct {
     int32_t hub_segment;
     int32_t tile_items;
     int32_t reorder;        /* 1: relabel vertices by out-degree, 0: keep ids, <0: keep setting */
+    int32_t tile_sort;      /* 1: gather tile arcs in source order, 0: row order, <0: keep setting */
+    int32_t hot_sources;    /* > 0: gather sources past this relabelled id non-temporally;
+                               0: keep setting; < 0: all gathers cached */
 } egx_bin_opts;
 
 typedef struct {

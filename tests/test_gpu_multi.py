@@ -47,6 +47,8 @@ def test_multi_equals_single_bitwise(gpu_ready, weighted, precision):
     specs = _small_set(12, seed=11 + int(weighted), weighted=weighted)
     _say(f"multi vs single: {len(specs)} graphs weighted={weighted} precision={precision}")
     graphs = [s.build(native) for s in specs]
+    for i, g in enumerate(graphs):  # mixed bin settings inside one launch
+        g.set_bins(tile_sort=i % 2, tile_items=(2944, 5888, 4416)[i % 3])
     alone = [g.pagerank(0.85, 100, 1e-6, precision=precision) for g in graphs]
     for hip_graph in (False, True):
         outs, iters, st = native.multi_pagerank(graphs, 0.85, 100, 1e-6, precision=precision,

@@ -203,6 +203,30 @@ def test_bin_settings_do_not_change_results(gpu_ready, bins):
     _assert_close(pr64, ref64, f"bins fp64 {bins}", tol=FP64_TOL, l1=1e-10)
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_source_sorted_tiles_are_bit_identical(gpu_ready, weighted):
+    """Gathering tile arcs in source order writes the same terms into the same
+    LDS slots, and non-temporal loads of cold sources read the same values:
+    every result bit is unchanged, also after an alpha change (which
+    refreshes the sorted weights)."""
+    V, E = O.rmat_csr(15, 16, seed=9)
+    W = O.edge_weights(V, E, 4) if weighted else None
+    g = native.DeviceGraph.from_csr(V, E, W)
+    runs = {}
+    for ts, hot in ((0, -1), (1, -1), (1, 1000), (0, 5000)):
+        g.set_bins(tile_sort=ts, tile_items=2944, hot_sources=hot)
+        assert g.info()["tile_sort"] == ts and g.info()["hot_sources"] == max(hot, 0)
+        runs[ts, hot] = [g.pagerank(0.85, 100, 1e-6), g.pagerank(0.9, 30, 0.0),
+                         g.pagerank(0.85, 20, 0.0, precision=native.PRECISION_FP64)]
+    for key in runs:
+        for (a, ia, _), (b, ib, _) in zip(runs[0, -1], runs[key]):
+            assert ia == ib and np.array_equal(a, b), key
+    runs[1] = runs[1, 1000]
+    ref, it_ref, _ = O.pagerank_csr(V, E, W, 0.85, 100, 1e-6)
+    _assert_close(runs[1][0][0], ref, f"sorted tiles weighted={weighted}")
+    assert runs[1][0][1] == it_ref
+
+
 def test_solves_are_deterministic(gpu_ready):
     g = native.DeviceGraph.rmat(16, 16, seed=4)
     a, ia, _ = g.pagerank(0.85, 40, 0.0)
