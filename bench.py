@@ -62,7 +62,41 @@ def cpu_baseline(g, n, m, iters, threads):
         "kind": "port",
         "sample": f"same R-MAT scale graph (n={n}, m={m}), {it} iterations at fixed K, OpenMP push with "
                   f"atomic scatter (pagerank.cpp:21-122 semantics), {dt:.2f} s",
+        "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()},
     }
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_c1(V, E, threads):
+    """R-MAT 14 on the host: the sequential FP64 restatement of
+    pagerank.py:75-139 (one core) at tol=1e-6 and at K=20, and the OpenMP
+    push of pagerank.cpp:21-122 at K=20."""
+    from oracle import oracle as O
+
+    n, m = len(V) - 1, len(E)
+    src, dst = O.csr_rows(V, E)
+    t0 = time.perf_counter()
+    _, it, _ = O.pagerank_rows(n, src, dst, None, 0.85, 100, 1e-6)
+    seq_tol = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.pagerank_rows(n, src, dst, None, 0.85, 20, 0.0)
+    seq_k = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.pagerank_omp(V, E, None, 0.85, 20, 0.0, threads=threads)
+    omp_k = time.perf_counter() - t0
+    return {"sequential_tol1e-6_ms": round(seq_tol * 1e3, 3), "sequential_tol1e-6_iterations": it,
+            "sequential_K20_gteps": round(m * 20 / seq_k / 1e9, 4), "openmp_K20_gteps": round(m * 20 / omp_k / 1e9, 4),
+            "openmp_threads": threads}
 
 
 def main():
@@ -139,6 +173,13 @@ def main():
         log(f"cpu baseline: {args.cpu_iters} iterations, {threads} threads")
         cpu = cpu_baseline(g, n, m, args.cpu_iters, threads)
 
+    # the CPU-runnable configuration (R-MAT 14) on the GPU, with its CPU legs
+    from eggpu_amd import benchmarks
+
+    c1, V1, E1 = benchmarks.c1_small()
+    if cpu is not None:
+        cpu["c1"] = cpu_c1(V1, E1, min(16, os.cpu_count() or 1))
+
     secondary = {}
     wanted = [x for x in args.sections.split(",") if x]
     if wanted:
@@ -196,7 +237,7 @@ def main():
             "traffic_source": traffic_src,
         },
         "cpu_baseline": cpu,
-        "secondary": secondary,
+        "secondary": dict(secondary, c1_small=c1),
     }
     print(json.dumps(line), flush=True)
 

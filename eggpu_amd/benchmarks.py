@@ -122,6 +122,57 @@ def multi_graph(count=512, K=20, seed=2024, plain_sample=512):
     return res
 
 
+def c1_small(scale=14, K=20):
+    """The CPU-runnable configuration on the GPU: R-MAT scale 14 through the
+    whole entry (Python graph -> out-CSR -> device build -> solve -> Mapping),
+    cold and warm, and the device solve alone.  Returns (result dict, V, E)
+    so a CPU baseline can run on the same arrays."""
+    import os
+
+    from . import pagerank as pr_mod
+    from .classes import DiGraph
+
+    g = native.DeviceGraph.rmat(scale, 16, seed=1, keep_csr=True)
+    V, E, _ = g.download_csr(with_weights=False)
+    n, m = len(V) - 1, len(E)
+    _, it_tol, ks_tol = g.pagerank(0.85, 100, 1e-6, download=False)
+    t0 = time.perf_counter()
+    _, st = g.pagerank_stats(0.85, K, 0.0, time_each_launch=False)
+    fixed_wall = time.perf_counter() - t0
+    G = DiGraph()
+    G.add_nodes_from(range(n))
+    src = np.repeat(np.arange(n), np.diff(V))
+    G.add_edges_from(zip(src.tolist(), E.tolist()))
+    old = {k: os.environ.get(k) for k in ("EASYGRAPH_ENABLE_GPU", "EASYGRAPH_GPU_STRICT_ERRORS")}
+    os.environ["EASYGRAPH_ENABLE_GPU"] = "1"
+    os.environ["EASYGRAPH_GPU_STRICT_ERRORS"] = "1"
+    try:
+        t0 = time.perf_counter()
+        r1 = pr_mod.pagerank(G, alpha=0.85, max_iter=100, tol=1e-6)
+        cold = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        r2 = pr_mod.pagerank(G, alpha=0.85, max_iter=100, tol=1e-6)
+        warm = time.perf_counter() - t0
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    assert len(r1) == n and len(r2) == n
+    g.close()
+    res = {
+        "workload": f"R-MAT scale {scale} (n={n}, m={m}), unweighted, alpha 0.85",
+        "tol1e-6_iterations": it_tol,
+        "tol1e-6_kernel_ms": round(ks_tol * 1e3, 4),
+        f"fixed_K{K}_gteps": round(m * K / fixed_wall / 1e9, 3),
+        f"fixed_K{K}_kernel_ms_per_iter": round(st["kernel_seconds"] / K * 1e3, 4),
+        "eg_pagerank_cold_ms": round(cold * 1e3, 2),
+        "eg_pagerank_warm_ms": round(warm * 1e3, 2),
+    }
+    return res, V, E
+
+
 def _serving_requests(graphs, count, seed):
     rng = np.random.default_rng(seed)
     reqs = []

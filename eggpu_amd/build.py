@@ -26,6 +26,7 @@ SOURCES = [
     "egx_spmm.hip",
     "egx_api.hip",
     "egx_serve.hip",
+    "egx_ingest.hip",
     "egx_probe.hip",
 ]
 

@@ -14,7 +14,7 @@ and ``cpp_easygraph/classes/graph_convert.cpp:447-526``:
   used as stored (undirected artifacts already hold both directions).
 
 The files are memory-mapped, never turned into Python adjacency; the GPU
-route copies them straight into the device build.
+route streams them straight into HBM (``device_graph``) and builds there.
 """
 from __future__ import annotations
 
@@ -111,6 +111,9 @@ class EGGPUBulkGraph:
         self.graph = {"name": meta.get("name", metadata_path.stem), "eggpu_bulk_csr": True}
         self.name = self.graph["name"]
         self._eggpu_bulk_signature = (str(metadata_path), int(meta.get("generation", 0)), n, ne, self._directed)
+        self._validate = bool(validate)
+        self._device = {}
+        self.ingest_report = None
 
     def is_directed(self):
         return self._directed
@@ -136,6 +139,29 @@ class EGGPUBulkGraph:
                     f"weighted PageRank requires an EGGPU CSR artifact containing the {weight!r} edge attribute")
             W = self._W
         return self._V, self._E, W
+
+    def device_graph(self, weight=None, alpha=0.85):
+        """The graph resident in HBM, streamed from the artifact files on first
+        use (egx_graph_from_csr_files: pinned double-buffered reads overlapped
+        with the copies, validation on the device) and kept for later calls."""
+        from . import native
+
+        key = None if weight is None else str(weight)
+        dg = self._device.get(key)
+        if dg is None:
+            wpath = None
+            if weight is not None:
+                if self._W is None or str(weight) != self.weight_key:
+                    raise NotImplementedError(
+                        f"weighted PageRank requires an EGGPU CSR artifact containing the {weight!r} edge attribute")
+                wpath = self.weights_path
+            dg, rep = native.DeviceGraph.from_csr_files(self.offsets_path, self.indices_path, wpath, self.num_nodes,
+                                                        self.num_entries, alpha=alpha, validate=self._validate)
+            self.ingest_report = rep
+            if self.simple_sorted_topology is None:
+                self.simple_sorted_topology = bool(rep["simple_sorted"])
+            self._device[key] = dg
+        return dg
 
     def __repr__(self):
         kind = "DiGraph" if self._directed else "Graph"

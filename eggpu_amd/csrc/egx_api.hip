@@ -2,9 +2,12 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cctype>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "egx_graph.h"
 
@@ -125,11 +128,15 @@ uint32_t threshold32(double x) {
     return (uint32_t)t;
 }
 
-// Per-thread prepared graph for egx_pagerank, after PagerankWorkspace and
-// its signature (pagerank.cu:44-195).
-struct ThreadCache {
+// Per-thread registry of prepared graphs for egx_pagerank, after the
+// PagerankWorkspace signature (pagerank.cu:140-167) and the thread-local
+// device-CSR registry (device_graph_cache.cu:53-168): at most
+// EASYGRAPH_GPU_DEVICE_CSR_CACHE_MAX_ENTRIES entries (default 4, at most 16),
+// within EASYGRAPH_GPU_DEVICE_CSR_CACHE_MAX_BYTES or max((free + owned)/4,
+// 256 MB), least recently used evicted first;
+// EASYGRAPH_GPU_DISABLE_DEVICE_CSR_CACHE turns reuse off.
+struct CacheEntry {
     std::unique_ptr<egx_graph> h;
-    int owner_device = -1;
     uint64_t graph_id = 0;
     const int32_t* V = nullptr;
     const int32_t* E = nullptr;
@@ -137,9 +144,15 @@ struct ThreadCache {
     int32_t n = -1, m = -1;
     int32_t v0 = 0, vn = 0, e0 = 0, em = 0;
     uint64_t w0 = 0, wm = 0;
+    uint64_t last_use = 0;
 };
-thread_local ThreadCache t_cache;
-std::atomic<uint64_t> g_resident{0};
+struct ThreadRegistry {
+    std::vector<CacheEntry> entries;
+    int owner_device = -1;
+    uint64_t clock = 0;
+};
+thread_local ThreadRegistry t_reg;
+std::atomic<uint64_t> g_evictions{0};
 
 uint64_t bits64(double x) {
     uint64_t u;
@@ -147,7 +160,38 @@ uint64_t bits64(double x) {
     return u;
 }
 
-bool cache_matches(const ThreadCache& c, const int32_t* V, const int32_t* E, const double* W, int32_t n, int32_t m,
+bool env_truthy(const char* name) {
+    const char* v = std::getenv(name);
+    if (!v) return false;
+    std::string s(v);
+    for (char& ch : s) ch = (char)std::toupper((unsigned char)ch);
+    return s == "1" || s == "TRUE" || s == "ON" || s == "YES";
+}
+
+size_t max_entries() {
+    const char* raw = std::getenv("EASYGRAPH_GPU_DEVICE_CSR_CACHE_MAX_ENTRIES");
+    if (!raw) return 4;
+    const unsigned long long v = std::strtoull(raw, nullptr, 10);
+    return v == 0 ? 1 : (size_t)std::min<unsigned long long>(v, 16);
+}
+
+size_t resident(const ThreadRegistry& r) {
+    size_t b = 0;
+    for (const CacheEntry& e : r.entries) b += e.h ? e.h->g.device_bytes() : 0;
+    return b;
+}
+
+size_t budget_bytes(const ThreadRegistry& r) {
+    if (const char* raw = std::getenv("EASYGRAPH_GPU_DEVICE_CSR_CACHE_MAX_BYTES")) {
+        const unsigned long long v = std::strtoull(raw, nullptr, 10);
+        if (v > 0) return (size_t)v;
+    }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || total_b == 0) return (size_t)2 << 30;
+    return std::max<size_t>((free_b + resident(r)) / 4, (size_t)256 << 20);
+}
+
+bool entry_matches(const CacheEntry& c, const int32_t* V, const int32_t* E, const double* W, int32_t n, int32_t m,
                    uint64_t graph_id) {
     if (!c.h) return false;
     if (c.graph_id != graph_id || c.V != V || c.E != E || c.W != W || c.n != n || c.m != m) return false;
@@ -162,9 +206,29 @@ bool cache_matches(const ThreadCache& c, const int32_t* V, const int32_t* E, con
     return true;
 }
 
+// Drop least recently used entries (never `keep`) until the limits hold.
+void evict(ThreadRegistry& r, const egx_graph* keep) {
+    const size_t cap = max_entries();
+    for (;;) {
+        const bool over_count = r.entries.size() > cap;
+        const bool over_bytes = r.entries.size() > 1 && resident(r) > budget_bytes(r);
+        if (!over_count && !over_bytes) return;
+        size_t victim = r.entries.size();
+        for (size_t i = 0; i < r.entries.size(); ++i)
+            if (r.entries[i].h.get() != keep && (victim == r.entries.size() ||
+                                                 r.entries[i].last_use < r.entries[victim].last_use))
+                victim = i;
+        if (victim == r.entries.size()) return;
+        r.entries.erase(r.entries.begin() + (std::ptrdiff_t)victim);
+        g_evictions++;
+    }
+}
+
 }  // namespace
 
 void set_last_error(const std::string& msg) { t_last_error = msg; }
+
+int make_graph(std::unique_ptr<egx_graph>& out) { return new_graph(out); }
 
 Graph::~Graph() {
     if (device >= 0) (void)hipSetDevice(device);
@@ -475,39 +539,53 @@ int egx_pagerank(const int32_t* V, const int32_t* E, const double* W, int32_t le
         egx::set_last_error("no HIP device is available");
         return EGX_DEVICE_ERR;
     }
-    egx::ThreadCache& c = egx::t_cache;
-    if (c.owner_device >= 0 && c.owner_device != dev) {
-        // pagerank.cu:373-377: the cached graph is bound to one device per thread
-        egx::set_last_error("this thread's cached PageRank graph lives on another device");
+    egx::ThreadRegistry& r = egx::t_reg;
+    if (!r.entries.empty() && r.owner_device != dev) {
+        // pagerank.cu:373-377: the cached graphs are bound to one device per thread
+        egx::set_last_error("this thread's cached PageRank graphs live on another device");
         return EGX_DEVICE_ERR;
     }
-    if (!egx::cache_matches(c, V, E, W, n, m, graph_id)) {
+    const bool reuse = !egx::env_truthy("EASYGRAPH_GPU_DISABLE_DEVICE_CSR_CACHE");
+    egx::CacheEntry* hit = nullptr;
+    if (reuse)
+        for (egx::CacheEntry& e : r.entries)
+            if (egx::entry_matches(e, V, E, W, n, m, graph_id)) hit = &e;
+    std::unique_ptr<egx_graph> scratch;  // the prepared graph when reuse is off
+    egx_graph* gh = nullptr;
+    if (hit) {
+        egx::g_hits++;
+        if (hit->h->g.weighted && hit->h->g.alpha_vals != alpha) egx::g_refreshes++;
+        hit->last_use = ++r.clock;
+        gh = hit->h.get();
+    } else {
         egx::g_misses++;
-        if (c.h) {
-            egx::g_resident -= c.h->g.device_bytes();
-            c.h.reset();
-        }
         std::unique_ptr<egx_graph> h;
         int rc = egx::graph_from_host_csr(V, E, W, n, m, alpha, false, h);
         if (rc != EGX_SUCC) return rc;
         egx::g_builds++;
-        c.h = std::move(h);
-        c.owner_device = dev;
-        c.graph_id = graph_id;
-        c.V = V;
-        c.E = E;
-        c.W = W;
-        c.n = n;
-        c.m = m;
-        c.v0 = V[0];
-        c.vn = V[n];
-        c.e0 = (m > 0) ? E[0] : 0;
-        c.em = (m > 0) ? E[m - 1] : 0;
-        c.w0 = (W && m > 0) ? egx::bits64(W[0]) : 0;
-        c.wm = (W && m > 0) ? egx::bits64(W[m - 1]) : 0;
-    } else {
-        egx::g_hits++;
-        if (c.h->g.weighted && c.h->g.alpha_vals != alpha) egx::g_refreshes++;
+        gh = h.get();
+        if (reuse) {
+            egx::CacheEntry e;
+            e.h = std::move(h);
+            e.graph_id = graph_id;
+            e.V = V;
+            e.E = E;
+            e.W = W;
+            e.n = n;
+            e.m = m;
+            e.v0 = V[0];
+            e.vn = V[n];
+            e.e0 = (m > 0) ? E[0] : 0;
+            e.em = (m > 0) ? E[m - 1] : 0;
+            e.w0 = (W && m > 0) ? egx::bits64(W[0]) : 0;
+            e.wm = (W && m > 0) ? egx::bits64(W[m - 1]) : 0;
+            e.last_use = ++r.clock;
+            r.entries.push_back(std::move(e));
+            r.owner_device = dev;
+            egx::evict(r, gh);
+        } else {
+            scratch = std::move(h);
+        }
     }
     egx_solve_opts o{};
     o.alpha = alpha;
@@ -515,30 +593,24 @@ int egx_pagerank(const int32_t* V, const int32_t* E, const double* W, int32_t le
     o.tol = threshold;
     o.precision = EGX_PRECISION_FP32;
     o.check_every = 0;
-    int rc = egx_graph_pagerank(c.h.get(), &o, PR, iterations, kernel_seconds);
-    if (c.h) {
-        egx::g_resident = c.h->g.device_bytes();
-    }
-    if (rc != EGX_SUCC) {
-        // pagerank.cu:706-711: drop runtime state after a failure
-        c.h.reset();
-        c.owner_device = -1;
-        egx::g_resident = 0;
+    int rc = egx_graph_pagerank(gh, &o, PR, iterations, kernel_seconds);
+    if (rc != EGX_SUCC && !scratch) {
+        // pagerank.cu:706-711: drop the failed graph's state
+        for (size_t i = 0; i < r.entries.size(); ++i)
+            if (r.entries[i].h.get() == gh) {
+                r.entries.erase(r.entries.begin() + (std::ptrdiff_t)i);
+                break;
+            }
     }
     return rc;
 }
 
 void egx_pagerank_cache_reset(void) {
-    egx::ThreadCache& c = egx::t_cache;
-    c.h.reset();
-    c.owner_device = -1;
-    c.graph_id = 0;
-    c.V = nullptr;
-    c.E = nullptr;
-    c.W = nullptr;
-    c.n = c.m = -1;
-    egx::g_resident = 0;
-    egx::g_hits = egx::g_misses = egx::g_builds = egx::g_refreshes = 0;
+    egx::ThreadRegistry& r = egx::t_reg;
+    r.entries.clear();
+    r.owner_device = -1;
+    r.clock = 0;
+    egx::g_hits = egx::g_misses = egx::g_builds = egx::g_refreshes = egx::g_evictions = 0;
 }
 
 int egx_pagerank_cache_stats(egx_cache_stats* out) {
@@ -547,7 +619,9 @@ int egx_pagerank_cache_stats(egx_cache_stats* out) {
     out->misses = egx::g_misses.load();
     out->builds = egx::g_builds.load();
     out->value_refreshes = egx::g_refreshes.load();
-    out->resident_bytes = egx::g_resident.load();
+    out->resident_bytes = egx::resident(egx::t_reg);
+    out->evictions = egx::g_evictions.load();
+    out->entries = egx::t_reg.entries.size();
     return EGX_SUCC;
 }
 

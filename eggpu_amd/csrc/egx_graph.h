@@ -34,6 +34,7 @@
 // double buffered, sized for FP64; partials (3 x grid, SoA), hub_part, out.
 #pragma once
 
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -186,3 +187,9 @@ int solve_pagerank_multi(std::vector<Graph*>& gs, const SolveParams& p, double* 
 struct egx_graph {
     egx::Graph g;
 };
+
+namespace egx {
+// A new, empty handle bound to the current device, with its stream and events.
+int make_graph(std::unique_ptr<egx_graph>& out);
+}  // namespace egx
+

@@ -586,8 +586,9 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
     Graph& g = *b.g;
     hipStream_t s = b.stream;
     // pull the device column states and buffer parity so host edits do not
-    // lose device progress
+    // lose device progress; widen the layout first (that polls again)
     EGX_TRY(batch_poll(b));
+    if (b.width <= j) EGX_TRY(batch_set_width(b, width_for(j + 1)));
     // drop the old seeds of column j from the table
     for (int32_t v : b.seeds[(size_t)j]) {
         const int32_t t = b.slot_of_row(v);
@@ -663,7 +664,6 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
     b.host_cols[(size_t)j] = cs;
     if (nseeds > 0) b.ptab_dirty = true;
     EGX_TRY(batch_sync_state(b));
-    if (b.width <= j) EGX_TRY(batch_set_width(b, width_for(j + 1)));
     k_spmm_init_col<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), g.n, b.width, j);
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpyAsync(b.cols.p, b.host_cols.data(), sizeof(ColState) * kCols, hipMemcpyHostToDevice, s),

@@ -435,8 +435,11 @@ def _get_cached_cpp_graph(G, prepared=None, weight=None, alpha=0.85):
         entry = cache.get(key)
         if entry is not None and entry[0] == prepared.get("ctx_id"):
             return entry[1]
-    V, E, W = graph_out_csr(G, prepared, weight)
-    dg = _native().DeviceGraph.from_csr(V, E, W, alpha=alpha)
+    if _is_bulk_graph(G) and hasattr(G, "device_graph"):
+        dg = G.device_graph(weight, alpha=alpha)  # artifact files streamed straight into HBM
+    else:
+        V, E, W = graph_out_csr(G, prepared, weight)
+        dg = _native().DeviceGraph.from_csr(V, E, W, alpha=alpha)
     if cache is not None:
         cache[key] = (prepared.get("ctx_id"), dg)
     return dg

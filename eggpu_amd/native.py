@@ -115,6 +115,16 @@ class SolveStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class IngestReport(ctypes.Structure):
+    _fields_ = [("in_range", ctypes.c_int32), ("finite", ctypes.c_int32), ("simple_sorted", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("bytes", ctypes.c_int64), ("read_seconds", ctypes.c_double),
+                ("copy_seconds", ctypes.c_double), ("check_seconds", ctypes.c_double),
+                ("build_seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_sessions", ctypes.c_int32),
                 ("step_iters", ctypes.c_int32)]
@@ -144,6 +154,8 @@ class CacheStats(ctypes.Structure):
         ("builds", ctypes.c_uint64),
         ("value_refreshes", ctypes.c_uint64),
         ("resident_bytes", ctypes.c_uint64),
+        ("evictions", ctypes.c_uint64),
+        ("entries", ctypes.c_uint64),
     ]
 
 
@@ -173,6 +185,9 @@ SIGNATURES = {
                                         ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_uint64, ctypes.c_double, ctypes.c_int32,
                                         ctypes.POINTER(_vp)]),
+    "egx_graph_from_csr_files": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int32,
+                                                ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                                                ctypes.POINTER(IngestReport), ctypes.POINTER(_vp)]),
     "egx_graph_er": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int32,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
                                     ctypes.c_int32, ctypes.POINTER(_vp)]),
@@ -288,6 +303,25 @@ class DeviceGraph:
         _check(lib().egx_graph_from_csr(_ptr(V, _i32p), _ptr(E, _i32p), _ptr(W, _f64p), n, len(E),
                                         float(alpha), int(bool(keep_csr)), ctypes.byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def from_csr_files(cls, offsets_path, indices_path, weights_path=None, n=None, m=None, alpha=0.85,
+                       validate=True, keep_csr=False):
+        """eggpu-csr-v1 files streamed straight into HBM and validated there.
+        Returns (DeviceGraph, ingest report dict)."""
+        import os
+
+        if n is None:
+            n = os.path.getsize(offsets_path) // 4 - 1
+        if m is None:
+            m = os.path.getsize(indices_path) // 4
+        rep = IngestReport()
+        h = _vp()
+        w = os.fsencode(str(weights_path)) if weights_path else None
+        _check(lib().egx_graph_from_csr_files(os.fsencode(str(offsets_path)), os.fsencode(str(indices_path)), w,
+                                              int(n), int(m), float(alpha), int(bool(validate)),
+                                              int(bool(keep_csr)), ctypes.byref(rep), ctypes.byref(h)))
+        return cls(h.value), rep.as_dict()
 
     @classmethod
     def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, weighted=False, wlo=0.1, whi=1.0,

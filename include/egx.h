@@ -67,14 +67,20 @@ int egx_pagerank(const int32_t *V, const int32_t *E, const double *W, int32_t le
                  int32_t len_E, double alpha, int32_t max_iter_num, double threshold,
                  uint64_t graph_id, double *PR, int32_t *iterations, double *kernel_seconds);
 
-/* Device-CSR cache control, after cpp_gpu_device_csr_cache_reset/_stats
+/* Prepared-graph cache of egx_pagerank: a per-thread LRU registry after
+ * device_graph_cache.cu:53-168 (EASYGRAPH_GPU_DEVICE_CSR_CACHE_MAX_ENTRIES,
+ * default 4, at most 16; EASYGRAPH_GPU_DEVICE_CSR_CACHE_MAX_BYTES, default
+ * max((free + owned)/4, 256 MB); EASYGRAPH_GPU_DISABLE_DEVICE_CSR_CACHE).
+ * Control after cpp_gpu_device_csr_cache_reset/_stats
  * (Easy-Graph/cpp_easygraph/cpp_easygraph.cpp:97-113, device_graph_cache.cu:519-527). */
 typedef struct {
     uint64_t hits;
     uint64_t misses;
     uint64_t builds;
     uint64_t value_refreshes; /* alpha changes served without a rebuild */
-    uint64_t resident_bytes;
+    uint64_t resident_bytes;  /* prepared graphs of the calling thread */
+    uint64_t evictions;       /* least-recently-used entries dropped for the entry or byte limit */
+    uint64_t entries;         /* prepared graphs held by the calling thread */
 } egx_cache_stats;
 void egx_pagerank_cache_reset(void);
 int egx_pagerank_cache_stats(egx_cache_stats *out);
@@ -153,6 +159,31 @@ int egx_graph_rmat(int32_t scale, int32_t edge_factor, double a, double b, doubl
 int egx_graph_rmat_n(int32_t n, int64_t nedges, double a, double b, double c, uint64_t seed,
                      int32_t weighted, double wlo, double whi, uint64_t wseed, double alpha,
                      int32_t keep_csr, egx_graph **out);
+
+/* eggpu-csr-v1 artifact files (int32 offsets[n+1], int32 indices[m], optional
+ * float64 weights[m]; eggpu_bulk_graph.py:170-334) streamed straight into
+ * HBM: 32 MB pread chunks through two pinned buffers, each copied while the
+ * next is read (replaces cpp_graph_from_csr_files, graph_convert.cpp:447-526,
+ * plus the first call's upload, pagerank.cu:497-516).  Validation runs on
+ * the device: offsets must start at 0, end at m and not decrease (always);
+ * with validate != 0 endpoints must lie in [0, n) and weights be finite
+ * (otherwise invalid endpoints are skipped, as the reference build does).
+ * weights_path NULL or "" means unit weights.  report may be NULL. */
+typedef struct {
+    int32_t in_range;       /* every endpoint in [0, n) */
+    int32_t finite;         /* every weight finite */
+    int32_t simple_sorted;  /* no self-loop, columns strictly increasing per row */
+    int32_t pad;
+    int64_t bytes;          /* file bytes moved to HBM */
+    double read_seconds;    /* host pread time */
+    double copy_seconds;    /* open to last byte in HBM (reads overlap copies) */
+    double check_seconds;
+    double build_seconds;   /* pull CSC + iteration layout */
+} egx_ingest_report;
+
+int egx_graph_from_csr_files(const char *offsets_path, const char *indices_path, const char *weights_path,
+                             int32_t n, int32_t m, double alpha, int32_t validate, int32_t keep_csr,
+                             egx_ingest_report *report, egx_graph **out);
 
 /* Seeded Erdos-Renyi G(n, M) draw (M pairs, self-loops and duplicates removed). */
 int egx_graph_er(int32_t n, int64_t nedges, uint64_t seed, int32_t weighted, double wlo,

@@ -310,7 +310,34 @@ def test_raw_csr_entry_point_cache(gpu_ready):
     assert it1 == it_ref and np.array_equal(r1, r2)
     _assert_close(r3, ref9, "raw csr alpha 0.9")
     native.pagerank_csr(Vc, Ec, Wc, 0.85, 100, 1e-6, graph_id=43)
-    assert native.cache_stats()["builds"] == 2
+    st = native.cache_stats()
+    assert st["builds"] == 2 and st["entries"] == 2 and st["resident_bytes"] > 0
+    native.cache_reset()
+
+
+def test_raw_csr_cache_is_lru_with_limits(gpu_ready, monkeypatch):
+    """Entry limit and eviction order after device_graph_cache.cu:53-168."""
+    native.cache_reset()
+    V, E = O.rmat_csr(10, 16, seed=2)
+    Vc, Ec = np.ascontiguousarray(V), np.ascontiguousarray(E)
+    monkeypatch.setenv("EASYGRAPH_GPU_DEVICE_CSR_CACHE_MAX_ENTRIES", "2")
+    for gid in (1, 2):
+        native.pagerank_csr(Vc, Ec, None, 0.85, 50, 1e-6, graph_id=gid)
+    native.pagerank_csr(Vc, Ec, None, 0.85, 50, 1e-6, graph_id=1)  # 1 is now the most recent
+    native.pagerank_csr(Vc, Ec, None, 0.85, 50, 1e-6, graph_id=3)  # evicts 2
+    st = native.cache_stats()
+    assert st["entries"] == 2 and st["evictions"] == 1 and st["builds"] == 3 and st["hits"] == 1
+    native.pagerank_csr(Vc, Ec, None, 0.85, 50, 1e-6, graph_id=1)
+    assert native.cache_stats()["hits"] == 2
+    native.pagerank_csr(Vc, Ec, None, 0.85, 50, 1e-6, graph_id=2)  # rebuilt
+    assert native.cache_stats()["builds"] == 4
+    monkeypatch.setenv("EASYGRAPH_GPU_DISABLE_DEVICE_CSR_CACHE", "1")
+    before = native.cache_stats()
+    r, _, _ = native.pagerank_csr(Vc, Ec, None, 0.85, 50, 1e-6, graph_id=1)
+    after = native.cache_stats()
+    assert after["builds"] == before["builds"] + 1 and after["hits"] == before["hits"]
+    ref, _, _ = O.pagerank_csr(V, E, None, 0.85, 50, 1e-6)
+    _assert_close(r, ref, "uncached raw csr")
     native.cache_reset()
 
 
