@@ -335,6 +335,8 @@ int egx_server_open(egx_graph* const* pool, int32_t count, const egx_server_opts
         sv->opts.hbm_budget_bytes = (int64_t)(0.8 * (double)free_b);
     }
     sv->stats.budget_bytes = sv->opts.hbm_budget_bytes;
+    for (const egx_graph_info& inf : sv->info)
+        sv->stats.largest_session_bytes = std::max<int64_t>(sv->stats.largest_session_bytes, session_bytes(inf));
     egx_server* raw = sv.get();
     try {
         sv->worker = std::thread([raw] { raw->run(); });

@@ -41,6 +41,7 @@ constexpr int kRedWaves = kRedThreads / kWave;
 constexpr int kSpmmHubSeg = 2048;    // arcs per hub segment (one workgroup)
 constexpr int kUnroll = 8;
 constexpr int kMinWidth = 8;         // narrowest X row (columns)
+constexpr size_t kArenaBytes = (size_t)1 << 20;  // pinned staging for queued batch updates
 
 struct ColState {
     double dang;     // dangling mass of the current iterate
@@ -285,10 +286,43 @@ __global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
     }
 }
 
-// Column j of x (row stride C) := 1/n (networkx starts personalised runs from uniform).
-__global__ void k_spmm_init_col(float* __restrict__ x, int32_t n, int32_t stride, int32_t j) {
+// Column j of the current X (row stride C) := 1/n (networkx starts
+// personalised runs from uniform).  The current buffer is read from ctl, so
+// the launch can be queued behind iterations that have not run yet.
+__global__ void k_spmm_init_col(float* __restrict__ x0, float* __restrict__ x1, const BatchCtl* __restrict__ ctl,
+                                int32_t n, int32_t stride, int32_t j) {
     const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+    float* x = ctl->cur ? x1 : x0;
     if (v < n) x[(size_t)v * stride + j] = (float)(1.0 / (double)n);
+}
+
+// A queued change of the batch state (teleport entries, slot table entries,
+// one column state, the any-active flag), applied in stream order.
+struct BatchUpdate {
+    int32_t n_ptab;       // ptab[p_idx[i]] = p_val[i]
+    int32_t n_tslot;      // tslot[t_idx[i]] = t_val[i]
+    int32_t col;          // >= 0: cols[col] = cs
+    int32_t set_any;      // 1: ctl->any_active = 1
+    ColState cs;
+};
+constexpr int kMaxUpdateEntries = 256;
+struct BatchUpdateBuf {
+    BatchUpdate h;
+    int32_t p_idx[kMaxUpdateEntries];
+    float p_val[kMaxUpdateEntries];
+    int32_t t_idx[kMaxUpdateEntries];
+    int32_t t_val[kMaxUpdateEntries];
+};
+
+__global__ void k_batch_update(const BatchUpdateBuf* __restrict__ u, float* __restrict__ ptab,
+                               int32_t* __restrict__ tslot, ColState* __restrict__ cols, BatchCtl* __restrict__ ctl) {
+    const int t = threadIdx.x;
+    for (int i = t; i < u->h.n_ptab; i += blockDim.x) ptab[u->p_idx[i]] = u->p_val[i];
+    for (int i = t; i < u->h.n_tslot; i += blockDim.x) tslot[u->t_idx[i]] = u->t_val[i];
+    if (t == 0) {
+        if (u->h.col >= 0) cols[u->h.col] = u->h.cs;
+        if (u->h.set_any) ctl->any_active = 1;
+    }
 }
 
 // dst (row stride cd) := src (row stride cs) for columns < min(cs, cd), 0 beyond.
@@ -301,11 +335,13 @@ __global__ void k_spmm_repack(const float* __restrict__ src, int32_t cs, float* 
     dst[i] = (c < cs) ? src[v * cs + c] : 0.0f;
 }
 
-// out[perm[r]] = x[r][j] / sum_j
-__global__ void k_spmm_output(const float* __restrict__ x, int32_t stride, const ColState* __restrict__ cols,
+// out[perm[r]] = x[r][j] / sum_j, from the current X (read from ctl)
+__global__ void k_spmm_output(const float* __restrict__ x0, const float* __restrict__ x1,
+                              const BatchCtl* __restrict__ ctl, int32_t stride, const ColState* __restrict__ cols,
                               const int32_t* perm, int32_t n, int32_t j, double* __restrict__ out) {
     const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
+    const float* x = ctl->cur ? x1 : x0;
     const double s = cols[j].sum;
     const int32_t v = perm ? perm[r] : r;
     out[v] = (s > 0.0) ? (double)x[(size_t)r * stride + j] * (1.0 / s) : 1.0 / (double)n;
@@ -372,19 +408,24 @@ struct Batch {
     int64_t n_hrows = 0;
     int32_t ptab_cap = 0;
     // host mirror of the teleport table
-    std::vector<std::vector<int32_t>> seeds;  // per column
+    std::vector<std::vector<int32_t>> seeds;  // per column (iteration rows)
     std::vector<int32_t> host_tslot_rows;     // rows that own a table slot
     std::vector<std::vector<float>> host_ptab;
-    std::vector<int32_t> row_to_slot_keys, row_to_slot_vals;
-    bool ptab_dirty = false;
-    std::vector<ColState> host_cols;
+    std::vector<ColState> host_cols;          // active flags are host-owned; the rest as last polled
+    std::vector<int32_t> rank_h;              // caller id -> iteration row (relabelled graphs)
     void* pinned = nullptr;  // ColState[64] + BatchCtl
+    // pinned staging for queued updates: copies from it run in stream order,
+    // the arena is recycled after a stream sync
+    char* arena = nullptr;
+    size_t arena_used = 0;
+    DevBuf upd;               // device copy of the update being applied
     double last_kernel_seconds = 0.0;
 
     ~Batch() {
         if (device >= 0) (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         if (pinned) (void)hipHostFree(pinned);
+        if (arena) (void)hipHostFree(arena);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -444,6 +485,12 @@ int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
     b->seeds.assign(kCols, {});
     EGX_HIP_TRY(hipHostMalloc(&b->pinned, sizeof(ColState) * kCols + sizeof(BatchCtl)),
                 EGX_FAILED_TO_ALLOCATE_HOST_MEM);
+    EGX_HIP_TRY(hipHostMalloc((void**)&b->arena, kArenaBytes), EGX_FAILED_TO_ALLOCATE_HOST_MEM);
+    EGX_TRY(b->upd.alloc(sizeof(BatchUpdateBuf)));
+    if (g.reordered) {
+        b->rank_h.resize((size_t)n);
+        EGX_HIP_TRY(hipMemcpy(b->rank_h.data(), g.rank.p, (size_t)n * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    }
 
     // hub segments of the SpMM (hub rows are the iteration-layout prefix)
     std::vector<int32_t> row((size_t)g.n_hub + 1);
@@ -501,30 +548,56 @@ int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
 
 int batch_poll(Batch& b);
 
-// Upload the host column states and the teleport table (between launches).
-int batch_sync_state(Batch& b) {
+// Queue one update (applied in stream order, no host wait unless the
+// staging arena wraps).
+int batch_queue_update(Batch& b, const BatchUpdateBuf& u) {
     hipStream_t s = b.stream;
-    if (b.ptab_dirty) {
-        std::vector<float> flat((size_t)b.ptab_cap * kCols, 0.0f);
-        for (size_t t = 0; t < b.host_ptab.size(); ++t)
-            std::memcpy(&flat[t * kCols], b.host_ptab[t].data(), kCols * 4);
-        EGX_HIP_TRY(hipMemcpyAsync(b.ptab.p, flat.data(), flat.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
-        if (!b.host_tslot_rows.empty()) {
-            std::vector<int32_t> rows = b.host_tslot_rows, slots(rows.size());
-            for (size_t i = 0; i < rows.size(); ++i) slots[i] = (int32_t)i;
-            DevBuf dr, ds;
-            EGX_TRY(dr.alloc(rows.size() * 4));
-            EGX_TRY(ds.alloc(rows.size() * 4));
-            EGX_HIP_TRY(hipMemcpyAsync(dr.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
-            EGX_HIP_TRY(hipMemcpyAsync(ds.p, slots.data(), slots.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
-            k_set_tslot<<<nblk((int64_t)rows.size(), 256), 256, 0, s>>>(b.tslot.as<int32_t>(), dr.as<int32_t>(),
-                                                                          ds.as<int32_t>(), (int32_t)rows.size());
-            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-        }
-        EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);  // before the staging vectors go away
-        b.ptab_dirty = false;
+    const size_t bytes = sizeof(BatchUpdateBuf);
+    if (b.arena_used + bytes > kArenaBytes) {
+        EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);  // earlier copies out of the arena are done
+        b.arena_used = 0;
     }
+    char* slot = b.arena + b.arena_used;
+    b.arena_used += (bytes + 255) & ~(size_t)255;
+    std::memcpy(slot, &u, bytes);
+    EGX_HIP_TRY(hipMemcpyAsync(b.upd.p, slot, bytes, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    k_batch_update<<<1, 64, 0, s>>>(b.upd.as<BatchUpdateBuf>(), b.ptab.as<float>(), b.tslot.as<int32_t>(),
+                                     b.cols.as<ColState>(), b.ctl.as<BatchCtl>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     return EGX_SUCC;
+}
+
+// Record ptab[t][j] = v in the host mirror and in update u.
+// Apply what u holds so far and start it over (when an entry list is full).
+int flush_update(Batch& b, BatchUpdateBuf& u) {
+    EGX_TRY(batch_queue_update(b, u));
+    u.h.n_ptab = 0;
+    u.h.n_tslot = 0;
+    return EGX_SUCC;
+}
+
+int stage_ptab(Batch& b, BatchUpdateBuf& u, int32_t t, int32_t j, float v) {
+    if (u.h.n_ptab >= kMaxUpdateEntries) EGX_TRY(flush_update(b, u));
+    b.host_ptab[(size_t)t][(size_t)j] = v;
+    u.p_idx[u.h.n_ptab] = t * kCols + j;
+    u.p_val[u.h.n_ptab] = v;
+    u.h.n_ptab += 1;
+    return EGX_SUCC;
+}
+
+int stage_tslot(Batch& b, BatchUpdateBuf& u, int32_t row, int32_t t) {
+    if (u.h.n_tslot >= kMaxUpdateEntries) EGX_TRY(flush_update(b, u));
+    u.t_idx[u.h.n_tslot] = row;
+    u.t_val[u.h.n_tslot] = t;
+    u.h.n_tslot += 1;
+    return EGX_SUCC;
+}
+
+BatchUpdateBuf empty_update() {
+    BatchUpdateBuf u;
+    std::memset(&u, 0, sizeof(u));
+    u.h.col = -1;
+    return u;
 }
 
 // Width covering slots [0, need): the next power of two >= need, at least 8.
@@ -577,6 +650,8 @@ int32_t batch_needed_width(const Batch& b) {
 }
 
 // Assign column j: seeds (original node ids) or none (global PageRank).
+// Queued in stream order behind any iterations already enqueued: the host
+// waits only when the X layout has to widen.
 int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, double alpha, int32_t max_iter,
                  double tol) {
     if (j < 0 || j >= kCols) {
@@ -585,34 +660,28 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
     }
     Graph& g = *b.g;
     hipStream_t s = b.stream;
-    // pull the device column states and buffer parity so host edits do not
-    // lose device progress; widen the layout first (that polls again)
-    EGX_TRY(batch_poll(b));
+    if (nseeds > 0 && !seeds) return EGX_UNSUPPORTED_GRAPH;
+    for (int32_t i = 0; i < nseeds; ++i) {
+        if (seeds[i] < 0 || seeds[i] >= g.n) {
+            set_last_error("seed node out of range");
+            return EGX_UNSUPPORTED_GRAPH;
+        }
+    }
     if (b.width <= j) EGX_TRY(batch_set_width(b, width_for(j + 1)));
+    BatchUpdateBuf u = empty_update();
     // drop the old seeds of column j from the table
     for (int32_t v : b.seeds[(size_t)j]) {
         const int32_t t = b.slot_of_row(v);
-        if (t >= 0) b.host_ptab[(size_t)t][(size_t)j] = 0.0f;
+        if (t >= 0) EGX_TRY(stage_ptab(b, u, t, j, 0.0f));
     }
     b.seeds[(size_t)j].clear();
+    // seeds arrive as caller ids; the table is indexed by iteration rows
     std::vector<int32_t> rseeds;
     if (nseeds > 0) {
-        if (!seeds) return EGX_UNSUPPORTED_GRAPH;
-        // seeds arrive as caller ids; the table is indexed by iteration rows
-        for (int32_t i = 0; i < nseeds; ++i) {
-            if (seeds[i] < 0 || seeds[i] >= g.n) {
-                set_last_error("seed node out of range");
-                return EGX_UNSUPPORTED_GRAPH;
-            }
-        }
         std::vector<int32_t> uniq(seeds, seeds + nseeds);
         std::sort(uniq.begin(), uniq.end());
         uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
-        for (int32_t v : uniq) {
-            int32_t r = v;
-            if (g.reordered) EGX_HIP_TRY(hipMemcpy(&r, g.rank.as<int32_t>() + v, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
-            rseeds.push_back(r);
-        }
+        for (int32_t v : uniq) rseeds.push_back(b.rank_h.empty() ? v : b.rank_h[(size_t)v]);
         const float wgt = (float)(1.0 / (double)rseeds.size());
         for (int32_t r : rseeds) {
             int32_t t = b.slot_of_row(r);
@@ -620,21 +689,11 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
                 // reuse an all-zero slot before growing
                 for (size_t i = 0; i < b.host_ptab.size() && t < 0; ++i) {
                     bool empty = true;
-                    for (int q = 0; q < kCols; ++q) empty = empty && b.host_ptab[i][(size_t)q] == 0.0f;
+                    for (int q = 0; q < kCols && empty; ++q) empty = b.host_ptab[i][(size_t)q] == 0.0f;
                     if (empty) {
                         t = (int32_t)i;
-                        const int32_t old = b.host_tslot_rows[i];
+                        EGX_TRY(stage_tslot(b, u, b.host_tslot_rows[i], -1));  // the old row loses its slot
                         b.host_tslot_rows[i] = r;
-                        // the old row no longer owns a slot
-                        int32_t m1 = -1;
-                        DevBuf dr, ds;
-                        EGX_TRY(dr.alloc(4));
-                        EGX_TRY(ds.alloc(4));
-                        EGX_HIP_TRY(hipMemcpy(dr.p, &old, 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
-                        EGX_HIP_TRY(hipMemcpy(ds.p, &m1, 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
-                        k_set_tslot<<<1, 64, 0, s>>>(b.tslot.as<int32_t>(), dr.as<int32_t>(), ds.as<int32_t>(), 1);
-                        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-                        EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
                     }
                 }
                 if (t < 0) {
@@ -646,10 +705,10 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
                     b.host_ptab.emplace_back(kCols, 0.0f);
                     b.host_tslot_rows.push_back(r);
                 }
+                EGX_TRY(stage_tslot(b, u, r, t));
             }
-            b.host_ptab[(size_t)t][(size_t)j] = wgt;
+            EGX_TRY(stage_ptab(b, u, t, j, wgt));
         }
-        b.ptab_dirty = true;
     }
     b.seeds[(size_t)j] = rseeds;
     ColState cs{};
@@ -662,36 +721,30 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
     const float x0 = (float)(1.0 / (double)g.n);
     cs.dang = (double)g.n_dangling * (double)x0;
     b.host_cols[(size_t)j] = cs;
-    if (nseeds > 0) b.ptab_dirty = true;
-    EGX_TRY(batch_sync_state(b));
-    k_spmm_init_col<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), g.n, b.width, j);
+    u.h.col = j;
+    u.h.cs = cs;
+    u.h.set_any = 1;
+    EGX_TRY(batch_queue_update(b, u));
+    k_spmm_init_col<<<nblk(g.n, 256), 256, 0, s>>>(b.X[0].as<float>(), b.X[1].as<float>(), b.ctl.as<BatchCtl>(),
+                                                    g.n, b.width, j);
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpyAsync(b.cols.p, b.host_cols.data(), sizeof(ColState) * kCols, hipMemcpyHostToDevice, s),
-                EGX_DEVICE_ERR);
-    BatchCtl ctl{};
-    ctl.any_active = 1;
-    ctl.cur = b.cur;
-    EGX_HIP_TRY(hipMemcpyAsync(b.ctl.p, &ctl, sizeof(ctl), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
 
 int batch_release(Batch& b, int32_t j) {
     if (j < 0 || j >= kCols) return EGX_UNSUPPORTED_GRAPH;
-    hipStream_t s = b.stream;
-    EGX_TRY(batch_poll(b));
+    BatchUpdateBuf u = empty_update();
     for (int32_t v : b.seeds[(size_t)j]) {
         const int32_t t = b.slot_of_row(v);
-        if (t >= 0) b.host_ptab[(size_t)t][(size_t)j] = 0.0f;
+        if (t >= 0) EGX_TRY(stage_ptab(b, u, t, j, 0.0f));
     }
     b.seeds[(size_t)j].clear();
+    ColState cs = b.host_cols[(size_t)j];
+    cs.active = 0;
     b.host_cols[(size_t)j].active = 0;
-    b.ptab_dirty = true;
-    EGX_TRY(batch_sync_state(b));
-    EGX_HIP_TRY(hipMemcpyAsync(b.cols.p, b.host_cols.data(), sizeof(ColState) * kCols, hipMemcpyHostToDevice, s),
-                EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
-    return EGX_SUCC;
+    u.h.col = j;
+    u.h.cs = cs;
+    return batch_queue_update(b, u);
 }
 
 // Enqueue `iters` iterations (launches after every column stops return at
@@ -768,12 +821,12 @@ int batch_result(Batch& b, int32_t j, double* host_out) {
     if (j < 0 || j >= kCols) return EGX_UNSUPPORTED_GRAPH;
     Graph& g = *b.g;
     hipStream_t s = b.stream;
-    EGX_TRY(batch_poll(b));  // the device knows which buffer is current
     if (j >= b.width) {
         set_last_error("batch slot holds no request");
         return EGX_UNSUPPORTED_GRAPH;
     }
-    k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), b.width, b.cols.as<ColState>(),
+    k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[0].as<float>(), b.X[1].as<float>(), b.ctl.as<BatchCtl>(),
+                                                  b.width, b.cols.as<ColState>(),
                                                   g.reordered ? g.perm.as<int32_t>() : nullptr, g.n, j,
                                                   b.out.as<double>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);

@@ -141,7 +141,7 @@ class RequestInfo(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "failed", "steps", "sessions_opened",
                                               "sessions_closed", "peak_active_columns", "peak_session_bytes",
-                                              "budget_bytes")]
+                                              "budget_bytes", "largest_session_bytes")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -331,6 +331,7 @@ typedef struct {
     int64_t peak_active_columns; /* requests resident on the device at once */
     int64_t peak_session_bytes;
     int64_t budget_bytes;
+    int64_t largest_session_bytes; /* footprint charged for the pool's largest graph */
 } egx_server_stats;
 
 /* The pool's graphs must stay alive (and not be re-binned) until close. */

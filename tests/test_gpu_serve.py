@@ -121,7 +121,8 @@ def test_budget_admission(pool):
         with pytest.raises(native.EgxError):
             srv.wait(rid)
     # room for one session at a time: graphs take turns, idle sessions close
-    one = max(2 * g.n * 64 * 4 for g in graphs) + (64 << 20)
+    with native.Server(graphs) as srv:
+        one = int(srv.stats()["largest_session_bytes"] * 1.5)
     with native.Server(graphs, hbm_budget_bytes=one) as srv:
         ids = [srv.submit(k % 3, (), max_iter=30, tol=0.0) for k in range(9)]
         for rid in ids:
