@@ -22,6 +22,7 @@
 // per-column seed weights (-1 when v seeds no column).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -149,28 +150,40 @@ __device__ __forceinline__ double row_dot(const SpmmArgs& a, int32_t beg, int32_
     return acc[0];
 }
 
+// Row inputs of the update that do not depend on the gathers; loaded before
+// them so the latencies overlap.
+struct RowIn {
+    float xo;    // x_in[v][c]
+    float deg;   // 0: dangling
+    int32_t ts;  // teleport slot or -1
+};
+
+template <int C>
+__device__ __forceinline__ RowIn row_inputs(const SpmmArgs& a, int32_t v) {
+    RowIn r;
+    r.xo = a.x_in[(size_t)v * C + (lane_id() % C)];
+    r.deg = a.deg[v];
+    r.ts = a.tslot[v];
+    return r;
+}
+
 // Lanes c < C finalise column c of row v.
 template <int C>
 __device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, double cj, int32_t v, double acc,
-                                         double inv_n, double& err, double& sum, double& dang) {
+                                         double inv_n, const RowIn& in, double& err, double& sum, double& dang) {
     const int c = lane_id();
     const size_t at = (size_t)v * C + c;
-    const float xo = a.x_in[at];
     if (!cs.active || cs.done) {
-        a.x_out[at] = xo;
+        a.x_out[at] = in.xo;
         return;
     }
     double tele;
-    if (cs.uniform) {
-        tele = inv_n;
-    } else {
-        const int32_t ts = a.tslot[v];
-        tele = (ts >= 0) ? (double)a.ptab[(size_t)ts * kCols + c] : 0.0;
-    }
+    if (cs.uniform) tele = inv_n;
+    else tele = (in.ts >= 0) ? (double)a.ptab[(size_t)in.ts * kCols + c] : 0.0;
     const float xn = (float)(cs.alpha * acc + cj * tele);
-    err += fabs((double)xn - (double)xo);
+    err += fabs((double)xn - (double)in.xo);
     sum += (double)xn;
-    if (a.deg[v] == 0.0f) dang += (double)xn;
+    if (in.deg == 0.0f) dang += (double)xn;
     a.x_out[at] = xn;
 }
 
@@ -211,8 +224,9 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     if (gw < a.n_waves) {
         const int32_t r0 = a.wave_rows[gw], r1 = a.wave_rows[gw + 1];
         for (int32_t v = r0; v < r1; ++v) {
+            const RowIn in = row_inputs<C>(a, v);
             const double acc = row_dot<C>(a, a.row[v], a.row[v + 1]);
-            if (owner) finalize<C>(a, cs, cj, v, acc, inv_n, err, sum, dang);
+            if (owner) finalize<C>(a, cs, cj, v, acc, inv_n, in, err, sum, dang);
         }
     }
     sh[0][w][lane] = err;
@@ -248,7 +262,7 @@ __global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
             const HubRow hr = a.hrows[h];
             double acc = 0.0;
             for (int32_t k = 0; k < hr.nseg; ++k) acc += a.hub_part[(size_t)(hr.first + k) * kCols + lane];
-            finalize<C>(a, cs, cj, hr.row, acc, inv_n, err, sum, dang);
+            finalize<C>(a, cs, cj, hr.row, acc, inv_n, row_inputs<C>(a, hr.row), err, sum, dang);
         }
     }
     for (uint32_t b = w; b < a.grid; b += kRedWaves) {
@@ -314,15 +328,16 @@ struct BatchUpdateBuf {
     int32_t t_val[kMaxUpdateEntries];
 };
 
+// Entries are applied in list order by one thread: a list may write one
+// address twice (a slot freed and re-used within one update), and the later
+// entry must win.
 __global__ void k_batch_update(const BatchUpdateBuf* __restrict__ u, float* __restrict__ ptab,
                                int32_t* __restrict__ tslot, ColState* __restrict__ cols, BatchCtl* __restrict__ ctl) {
-    const int t = threadIdx.x;
-    for (int i = t; i < u->h.n_ptab; i += blockDim.x) ptab[u->p_idx[i]] = u->p_val[i];
-    for (int i = t; i < u->h.n_tslot; i += blockDim.x) tslot[u->t_idx[i]] = u->t_val[i];
-    if (t == 0) {
-        if (u->h.col >= 0) cols[u->h.col] = u->h.cs;
-        if (u->h.set_any) ctl->any_active = 1;
-    }
+    if (threadIdx.x != 0) return;
+    for (int i = 0; i < u->h.n_ptab; ++i) ptab[u->p_idx[i]] = u->p_val[i];
+    for (int i = 0; i < u->h.n_tslot; ++i) tslot[u->t_idx[i]] = u->t_val[i];
+    if (u->h.col >= 0) cols[u->h.col] = u->h.cs;
+    if (u->h.set_any) ctl->any_active = 1;
 }
 
 // dst (row stride cd) := src (row stride cs) for columns < min(cs, cd), 0 beyond.
@@ -420,10 +435,28 @@ struct Batch {
     size_t arena_used = 0;
     DevBuf upd;               // device copy of the update being applied
     double last_kernel_seconds = 0.0;
+    // a captured run of `step_iters` iterations at `step_width`, replayed by
+    // later steps of the same shape (kernel arguments never change between
+    // iterations; the buffer parity lives on the device)
+    bool use_graph = true;
+    hipGraph_t step_graph = nullptr;
+    hipGraphExec_t step_exec = nullptr;
+    int32_t step_width = 0, step_iters = 0;
+    const void* step_x0 = nullptr;
+    const void* step_x1 = nullptr;
+
+    void drop_step_graph() {
+        if (step_exec) (void)hipGraphExecDestroy(step_exec);
+        if (step_graph) (void)hipGraphDestroy(step_graph);
+        step_exec = nullptr;
+        step_graph = nullptr;
+        step_width = step_iters = 0;
+    }
 
     ~Batch() {
         if (device >= 0) (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_step_graph();
         if (pinned) (void)hipHostFree(pinned);
         if (arena) (void)hipHostFree(arena);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -487,6 +520,7 @@ int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
                 EGX_FAILED_TO_ALLOCATE_HOST_MEM);
     EGX_HIP_TRY(hipHostMalloc((void**)&b->arena, kArenaBytes), EGX_FAILED_TO_ALLOCATE_HOST_MEM);
     EGX_TRY(b->upd.alloc(sizeof(BatchUpdateBuf)));
+    if (const char* e = std::getenv("EGGPU_AMD_BATCH_GRAPHS")) b->use_graph = std::strcmp(e, "0") != 0;
     if (g.reordered) {
         b->rank_h.resize((size_t)n);
         EGX_HIP_TRY(hipMemcpy(b->rank_h.data(), g.rank.p, (size_t)n * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
@@ -561,7 +595,7 @@ int batch_queue_update(Batch& b, const BatchUpdateBuf& u) {
     b.arena_used += (bytes + 255) & ~(size_t)255;
     std::memcpy(slot, &u, bytes);
     EGX_HIP_TRY(hipMemcpyAsync(b.upd.p, slot, bytes, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
-    k_batch_update<<<1, 64, 0, s>>>(b.upd.as<BatchUpdateBuf>(), b.ptab.as<float>(), b.tslot.as<int32_t>(),
+    k_batch_update<<<1, kWave, 0, s>>>(b.upd.as<BatchUpdateBuf>(), b.ptab.as<float>(), b.tslot.as<int32_t>(),
                                      b.cols.as<ColState>(), b.ctl.as<BatchCtl>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     return EGX_SUCC;
@@ -747,6 +781,8 @@ int batch_release(Batch& b, int32_t j) {
     return batch_queue_update(b, u);
 }
 
+int enqueue_iterations(Batch& b, const SpmmArgs& a, int32_t iters, hipStream_t s);
+
 // Enqueue `iters` iterations (launches after every column stops return at
 // once).  Asynchronous: returns before the work completes.
 int batch_step(Batch& b, int32_t iters, bool record_time) {
@@ -774,9 +810,41 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
     a.nblk_h = b.nblk_h;
     a.grid = b.grid;
     a.n_waves = b.n_waves;
-    if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev0, s), EGX_DEVICE_ERR);
     a.xbuf[0] = b.X[0].as<float>();
     a.xbuf[1] = b.X[1].as<float>();
+    if (b.use_graph && iters >= 2) {
+        if (!(b.step_exec && b.step_width == b.width && b.step_iters == iters && b.step_x0 == b.X[0].p &&
+              b.step_x1 == b.X[1].p)) {
+            b.drop_step_graph();
+            EGX_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
+            const int rc = enqueue_iterations(b, a, iters, s);
+            hipGraph_t graph = nullptr;
+            const hipError_t e = hipStreamEndCapture(s, &graph);
+            if (rc != EGX_SUCC) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            EGX_HIP_TRY(e, EGX_DEVICE_ERR);
+            b.step_graph = graph;
+            EGX_HIP_TRY(hipGraphInstantiate(&b.step_exec, b.step_graph, nullptr, nullptr, 0), EGX_DEVICE_ERR);
+            b.step_width = b.width;
+            b.step_iters = iters;
+            b.step_x0 = b.X[0].p;
+            b.step_x1 = b.X[1].p;
+        }
+        if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev0, s), EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipGraphLaunch(b.step_exec, s), EGX_DEVICE_ERR);
+        if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev1, s), EGX_DEVICE_ERR);
+        return EGX_SUCC;
+    }
+    if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev0, s), EGX_DEVICE_ERR);
+    EGX_TRY(enqueue_iterations(b, a, iters, s));
+    if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev1, s), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
+// `iters` iterations (rows + reduce launches) at the batch's current width.
+int enqueue_iterations(Batch& b, const SpmmArgs& a, int32_t iters, hipStream_t s) {
     for (int32_t i = 0; i < iters; ++i) {
         switch (b.width) {
             case 8: k_spmm_rows<8><<<b.grid, kRowThreads, 0, s>>>(a); break;
@@ -793,7 +861,6 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
         }
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     }
-    if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev1, s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
 

@@ -385,13 +385,18 @@ def graph_out_csr(G, prepared, weight=None):
     src, dst, wts = [], [], []
     weighted = weight is not None
     if prepared["directed"]:
+        # whole rows at a time: one extend per node instead of one append per arc
+        rows, counts = [], []
+        get = idx.__getitem__
         for u, nbrs in adj.items():
-            iu = idx[u]
-            for v, data in nbrs.items():
-                src.append(iu)
-                dst.append(idx[v])
-                if weighted:
-                    wts.append(float(data.get(weight, 1.0)))
+            if not nbrs:
+                continue
+            rows.append(idx[u])
+            counts.append(len(nbrs))
+            dst.extend(map(get, nbrs))
+            if weighted:
+                wts.extend(float(d.get(weight, 1.0)) for d in nbrs.values())
+        src = np.repeat(np.asarray(rows, dtype=np.int64), np.asarray(counts, dtype=np.int64))
     else:
         seen = set()
         for u, nbrs in adj.items():

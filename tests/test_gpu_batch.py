@@ -155,6 +155,33 @@ def test_column_bits_do_not_depend_on_width(gpu_ready, slot):
         assert np.array_equal(b.result(slot), ref)
 
 
+def test_graph_replay_equals_eager_launches(gpu_ready, monkeypatch):
+    """Steps replayed from a captured hipGraph give the bits of eager launches,
+    replay after replay, with host work, re-assignment and a width change
+    between the replays."""
+    V, E = O.rmat_csr(12, 16, seed=17)
+    W = O.edge_weights(V, E, 9)
+    g = native.DeviceGraph.from_csr(V, E, W)
+    runs = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("EGGPU_AMD_BATCH_GRAPHS", mode)
+        out = []
+        with g.batch() as b:
+            b.assign(0, [1, 2, 3], max_iter=40, tol=0.0)
+            b.assign(3, [], alpha=0.9, max_iter=40, tol=0.0)
+            for _ in range(3):
+                b.step(4)
+                b.poll()
+            b.assign(12, [77], max_iter=40, tol=0.0)  # widens to 16 columns: a new capture
+            b.run(max_launches=60, check_every=4)
+            out = [b.result(0), b.result(3), b.result(12)]
+            st = b.poll()
+            assert st["iterations"][0] == st["iterations"][3] == st["iterations"][12] == 40
+        runs[mode] = out
+    for a, c in zip(runs["0"], runs["1"]):
+        assert np.array_equal(a, c)
+
+
 def test_global_column_equals_plain_pagerank(gpu_ready):
     V, E = O.rmat_csr(14, 16, seed=1)
     g = native.DeviceGraph.from_csr(V, E)
