@@ -127,10 +127,12 @@ def c1_small(scale=14, K=20):
     whole entry (Python graph -> out-CSR -> device build -> solve -> Mapping),
     cold and warm, and the device solve alone.  Returns (result dict, V, E)
     so a CPU baseline can run on the same arrays."""
+    import importlib
     import os
 
-    from . import pagerank as pr_mod
     from .classes import DiGraph
+
+    pr_mod = importlib.import_module("eggpu_amd.pagerank")  # the module; the package exports the function
 
     g = native.DeviceGraph.rmat(scale, 16, seed=1, keep_csr=True)
     V, E, _ = g.download_csr(with_weights=False)
