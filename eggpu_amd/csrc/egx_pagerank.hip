@@ -53,6 +53,8 @@ constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int kReduceThreads = 256;  // per reduce-part workgroup
 constexpr int kRedItems = 512;       // hub rows + partial slots per reduce-part workgroup (target)
 constexpr int kMaxRed = 1024;        // reduce-part workgroups per graph, at most
+constexpr int kXcds = 8;             // MI355X: accelerator complex dies, one L2 each
+constexpr int64_t kAffineBytes = (int64_t)2 << 20;  // gathered vector kept on one XCD up to this size
 constexpr int kBatch = 8;  // gathers in flight per thread
 
 template <typename T>
@@ -319,18 +321,19 @@ __global__ __launch_bounds__(kWave) void k_pr_final(IterArgs<T> a) {
 }
 
 // Many graphs, one launch: block b works for graph blk_graph[b], as its
-// local block b - blk_base[graph].
+// local block blk_local[b]; blk_graph[b] < 0 is padding.
 template <typename T, bool W>
 __global__ __launch_bounds__(kThreads) void k_pr_spmv_multi(const IterArgs<T>* __restrict__ table,
                                                             const int32_t* __restrict__ blk_graph,
-                                                            const uint32_t* __restrict__ blk_base) {
+                                                            const uint32_t* __restrict__ blk_local) {
     __shared__ T lds[kTileLds];
     __shared__ double sh_red[3 * kWavesPerBlock];
     const int32_t gi = blk_graph[blockIdx.x];
+    if (gi < 0) return;
     IterArgs<T> a = table[gi];
     if (a.state->done) return;
     bind(a);
-    spmv_block<T, W>(a, blockIdx.x - blk_base[gi], lds, sh_red);
+    spmv_block<T, W>(a, blk_local[blockIdx.x], lds, sh_red);
 }
 
 template <typename T, bool W>
@@ -412,10 +415,13 @@ struct LaunchTimer {
 };
 
 // A captured run of iterations, replayed until every graph has stopped.
+// Destroyed only after the stream it was launched on has drained.
 struct CapturedRun {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
     ~CapturedRun() {
+        if (exec && stream) (void)hipStreamSynchronize(stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
@@ -496,6 +502,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
     LaunchTimer timer;
     if (p.time_each_launch) EGX_TRY(timer.reserve((size_t)4 * p.max_iter));
     CapturedRun cap;
+    cap.stream = s;
     EGX_HIP_TRY(hipEventRecord(g.ev0, s), EGX_DEVICE_ERR);
     int32_t launched = 0;
     int32_t chunk = p.check_every > 0 ? p.check_every : 4;
@@ -584,9 +591,49 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
         set_last_error("too many work items for one launch");
         return EGX_UNSUPPORTED_GRAPH;
     }
-    std::vector<int32_t> blk((size_t)total);
-    for (int32_t i = 0; i < count; ++i)
-        std::fill(blk.begin() + base[(size_t)i], blk.begin() + base[(size_t)i] + table[(size_t)i].grid, i);
+    // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs
+    // on XCD b % 8), and each XCD has its own L2.  A graph whose gathered
+    // vector fits in one L2 gets all its work items on one XCD (the least
+    // loaded, largest graphs first), so that vector is cached once instead
+    // of eight times; larger graphs are spread over every XCD.
+    std::vector<std::vector<std::pair<int32_t, uint32_t>>> lanes(kXcds);
+    {
+        std::vector<int32_t> order((size_t)count);
+        for (int32_t i = 0; i < count; ++i) order[(size_t)i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            return table[(size_t)x].grid > table[(size_t)y].grid;
+        });
+        std::vector<uint64_t> load(kXcds, 0);
+        for (int32_t gi : order) {
+            const uint32_t grid = table[(size_t)gi].grid;
+            const bool affine = (int64_t)gs[(size_t)gi]->n * (int64_t)sizeof(T) <= kAffineBytes;
+            if (affine) {
+                const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                for (uint32_t k = 0; k < grid; ++k) lanes[(size_t)x].emplace_back(gi, k);
+                load[(size_t)x] += grid;
+            } else {
+                for (uint32_t k = 0; k < grid; ++k) {
+                    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                    lanes[(size_t)x].emplace_back(gi, k);
+                    load[(size_t)x] += 1;
+                }
+            }
+        }
+    }
+    size_t depth = 0;
+    for (const auto& l : lanes) depth = std::max(depth, l.size());
+    total = (uint64_t)depth * kXcds;
+    if (total > 0x7FFFFFFFull) {
+        set_last_error("too many work items for one launch");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    std::vector<int32_t> blk((size_t)total, -1);
+    std::vector<uint32_t> blk_local((size_t)total, 0);
+    for (int x = 0; x < kXcds; ++x)
+        for (size_t q = 0; q < lanes[(size_t)x].size(); ++q) {
+            blk[q * kXcds + (size_t)x] = lanes[(size_t)x][q].first;
+            blk_local[q * kXcds + (size_t)x] = lanes[(size_t)x][q].second;
+        }
     // reduce-part workgroups: red_graph / red_base play the part of blk / base
     std::vector<uint32_t> rbase((size_t)count);
     uint64_t rtotal = 0;
@@ -600,7 +647,7 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     DevBuf d_table, d_blk, d_base, d_rblk, d_rbase;
     EGX_TRY(d_table.alloc(sizeof(IterArgs<T>) * (size_t)count));
     EGX_TRY(d_blk.alloc(blk.size() * 4));
-    EGX_TRY(d_base.alloc(base.size() * 4));
+    EGX_TRY(d_base.alloc(blk_local.size() * 4));
     EGX_TRY(d_rblk.alloc(rblk.size() * 4));
     EGX_TRY(d_rbase.alloc(rbase.size() * 4));
     EGX_HIP_TRY(hipMemcpyAsync(d_rblk.p, rblk.data(), rblk.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
@@ -608,7 +655,8 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     EGX_HIP_TRY(hipMemcpyAsync(d_table.p, table.data(), sizeof(IterArgs<T>) * (size_t)count, hipMemcpyHostToDevice, s),
                 EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpyAsync(d_blk.p, blk.data(), blk.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpyAsync(d_base.p, base.data(), base.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(d_base.p, blk_local.data(), blk_local.size() * 4, hipMemcpyHostToDevice, s),
+                EGX_DEVICE_ERR);
     // per-graph states are gathered into one pinned copy for polling
     std::vector<PrState> hstates((size_t)count);
     void* pinned = nullptr;
@@ -621,6 +669,7 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
 
     const int32_t chunk = p.check_every > 0 ? p.check_every : 8;
     CapturedRun cap;
+    cap.stream = s;
     // alpha refreshes run on each graph's own stream
     for (Graph* g : gs) EGX_HIP_TRY(hipStreamSynchronize(g->stream), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);

@@ -64,6 +64,7 @@ struct Session {
     int64_t bytes = 0;
     std::vector<std::shared_ptr<Request>> slot;  // kSlots entries
     int32_t busy = 0;
+    bool in_flight = false;  // a step is queued and not yet collected
     Clock::time_point last_used;
 };
 
@@ -135,7 +136,9 @@ struct egx_server {
                (int32_t)sessions.size() >= opts.max_sessions) {
             auto victim = sessions.end();
             for (auto it = sessions.begin(); it != sessions.end(); ++it)
-                if (it->busy == 0 && (victim == sessions.end() || it->last_used < victim->last_used)) victim = it;
+                if (it->busy == 0 && !it->in_flight &&
+                    (victim == sessions.end() || it->last_used < victim->last_used))
+                    victim = it;
             if (victim == sessions.end()) return EGX_SUCC;  // wait for a busy session to drain
             egx_batch_close(victim->b);
             used_bytes -= victim->bytes;
@@ -248,6 +251,9 @@ struct egx_server {
         }
     }
 
+    // Sessions progress independently: a session whose step has finished is
+    // collected and stepped again at once, without waiting for sessions on
+    // larger graphs; queued requests are admitted between these checks.
     void run() {
         (void)hipSetDevice(device);
         for (;;) {
@@ -257,23 +263,36 @@ struct egx_server {
                 if (stop) break;
             }
             admit();
-            bool any = false;
+            bool progressed = false;
+            int64_t launched = 0;
             for (Session& s : sessions) {
-                if (s.busy == 0) continue;
-                if (egx_batch_step(s.b, opts.step_iters) != EGX_SUCC) continue;  // collect reports it
-                any = true;
+                if (s.in_flight) {
+                    const int r = egx_batch_ready(s.b);
+                    if (r == 0) continue;
+                    s.in_flight = false;
+                    collect(s);  // also reports a failed step
+                    progressed = true;
+                }
+                if (s.busy > 0) {
+                    if (egx_batch_step(s.b, opts.step_iters) == EGX_SUCC) {
+                        s.in_flight = true;
+                        ++launched;
+                    } else {
+                        collect(s);
+                    }
+                    progressed = true;
+                }
             }
-            if (any) {
+            if (launched) {
                 std::lock_guard<std::mutex> lk(mu);
-                stats.steps += 1;
+                stats.steps += launched;
             }
-            for (Session& s : sessions)
-                if (s.busy > 0) collect(s);
-            if (!any) {
-                // nothing could run: wait for a submission (or a drained session)
+            if (!progressed) {
+                // work in flight and nothing finished yet: wait briefly for a
+                // submission or for the device
                 std::unique_lock<std::mutex> lk(mu);
-                if (!stop && active_columns() == 0)
-                    cv_work.wait_for(lk, std::chrono::milliseconds(1), [&] { return stop; });
+                if (!stop && pending.empty())
+                    cv_work.wait_for(lk, std::chrono::microseconds(50), [&] { return stop || !pending.empty(); });
             }
         }
         // fail whatever is left
@@ -319,7 +338,7 @@ int egx_server_open(egx_graph* const* pool, int32_t count, const egx_server_opts
     }
     sv->device = pool[0]->g.device;
     if (opts) sv->opts = *opts;
-    if (sv->opts.max_sessions <= 0) sv->opts.max_sessions = 16;
+    if (sv->opts.max_sessions <= 0) sv->opts.max_sessions = 64;
     if (sv->opts.step_iters <= 0) sv->opts.step_iters = 4;
     if (sv->opts.hbm_budget_bytes <= 0) {
         size_t free_b = 0, total_b = 0;

@@ -815,6 +815,8 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
     if (b.use_graph && iters >= 2) {
         if (!(b.step_exec && b.step_width == b.width && b.step_iters == iters && b.step_x0 == b.X[0].p &&
               b.step_x1 == b.X[1].p)) {
+            // the previous replay may still be running: let it finish before its exec goes away
+            if (b.step_exec) EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
             b.drop_step_graph();
             EGX_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
             const int rc = enqueue_iterations(b, a, iters, s);
@@ -1022,6 +1024,20 @@ int egx_batch_result(egx_batch* bh, int32_t slot, double* PR) {
     BatchDevice dg(bh->b->device);
     if (!dg.ok) return EGX_DEVICE_ERR;
     return egx::batch_result(*bh->b, slot, PR);
+}
+
+int egx_batch_ready(egx_batch* bh) {
+    if (!bh || !bh->b) return -EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return -EGX_DEVICE_ERR;
+    const hipError_t e = hipStreamQuery(bh->b->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: clear it
+        return 0;
+    }
+    egx::set_last_error(hipGetErrorString(e));
+    return -EGX_DEVICE_ERR;
 }
 
 void egx_batch_close(egx_batch* bh) {

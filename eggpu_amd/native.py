@@ -214,6 +214,7 @@ SIGNATURES = {
     "egx_batch_release": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "egx_batch_step": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "egx_batch_poll": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _f64p]),
+    "egx_batch_ready": (ctypes.c_int, [_vp]),
     "egx_batch_run": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, _f64p]),
     "egx_batch_result": (ctypes.c_int, [_vp, ctypes.c_int32, _f64p]),
     "egx_batch_close": (None, [_vp]),
@@ -505,6 +506,13 @@ class Batch:
         e = np.zeros(64, np.float64)
         _check(lib().egx_batch_poll(self._h, _ptr(a, _i32p), _ptr(d, _i32p), _ptr(it, _i32p), _ptr(e, _f64p)))
         return {"active": a, "done": d, "iterations": it, "err": e}
+
+    def ready(self):
+        """True when all queued work of the session has finished (never blocks)."""
+        rc = lib().egx_batch_ready(self._h)
+        if rc < 0:
+            _check(-rc)
+        return rc == 1
 
     def run(self, max_launches=200, check_every=0):
         ks = ctypes.c_double(0.0)

@@ -275,6 +275,10 @@ int egx_batch_run(egx_batch *b, int32_t max_launches, int32_t check_every, doubl
 /* Sum-normalised FP64 ranks of column slot, in node id order (PR[n]). */
 int egx_batch_result(egx_batch *b, int32_t slot, double *PR);
 
+/* 1 when everything queued on the session has finished, 0 while work is in
+ * flight, a negative status on error.  Never blocks. */
+int egx_batch_ready(egx_batch *b);
+
 void egx_batch_close(egx_batch *b);
 
 /* One-shot: k columns, column j seeded by seeds[seed_offsets[j] .. seed_offsets[j+1])
@@ -309,7 +313,7 @@ typedef struct egx_server egx_server; /* opaque */
 
 typedef struct {
     int64_t hbm_budget_bytes; /* session memory the server may hold; <= 0: 80% of free HBM at open */
-    int32_t max_sessions;     /* concurrently open sessions (streams); <= 0: 16 */
+    int32_t max_sessions;     /* concurrently open sessions (streams); <= 0: 64 */
     int32_t step_iters;       /* iterations enqueued per session between polls; <= 0: 4 */
 } egx_server_opts;
 
