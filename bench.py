@@ -108,6 +108,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default bins")
+    ap.add_argument("--bins", help='fixed bins instead of the tuner, JSON of egx_graph_set_bins fields')
     ap.add_argument("--sections", default="batched_ppr,multi_graph,serving",
                     help="secondary workloads reported beside the headline line ('' for none)")
     args = ap.parse_args()
@@ -133,7 +134,11 @@ def main():
 
     # bins chosen by the autotuner (coordinate descent from the defaults)
     tuned = None
-    if not args.no_tune:
+    if args.bins:
+        g.set_bins(**json.loads(args.bins))
+        info = g.info()
+        tuned = {"bins": json.loads(args.bins), "fixed": True}
+    elif not args.no_tune:
         from eggpu_amd import autotune
 
         t1 = time.perf_counter()

@@ -91,63 +91,106 @@ struct SpmmArgs {
 };
 
 // Width-C launches: a wavefront holds G = 64/C lane groups of C columns;
-// lane = group*C + column.  Arc k of a row range (k counted from its first
-// arc) belongs to residue class q = k mod 8; every lane keeps R = 8/G FP64
-// accumulators (classes q = group + G*r), each summing its class in arc
-// order, and the 8 classes are combined in one fixed tree,
-// ((q0+q1)+(q2+q3))+((q4+q5)+(q6+q7)): a column's sums -- and so its
+// lane = group*C + column.
+//
+// Canonical row sum: the terms of a row are split by the residue class
+// q = p mod 8 of their arc index p in the iteration layout; each class is
+// summed in arc order in FP64 and the 8 class sums are combined in one fixed
+// tree, ((q0+q1)+(q2+q3))+((q4+q5)+(q6+q7)).  Arcs are gathered in batches
+// aligned to multiples of 8, so arc p always lands in accumulator p mod 8
+// whatever lane schedule a width uses, and a column's sums -- and so its
 // result -- are bit-identical at every width.
+__device__ __forceinline__ double class_tree(const double (&acc)[8]) {
+    return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
+// Gather the aligned batch [p0, p0+8) restricted to [beg, end) for the
+// lane's group: lanes c < 8 of the group load the arcs' (source, value) and
+// every lane of the group reads them from there; eight row loads in flight.
 template <int C>
-__device__ __forceinline__ double row_dot(const SpmmArgs& a, int32_t beg, int32_t end) {
+__device__ __forceinline__ void gather_batch(const SpmmArgs& a, int32_t p0, int32_t beg, int32_t end, float (&g)[8],
+                                             float (&w)[8]) {
+    const int lane = lane_id();
+    const int c = lane % C;
+    const int32_t pl = p0 + (c & 7);
+    const bool okl = c < 8 && pl >= beg && pl < end;
+    const int32_t u_l = okl ? __builtin_nontemporal_load(&a.col[pl]) : 0;
+    const float w_l = okl ? __builtin_nontemporal_load(&a.nval[pl]) : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        int32_t u;
+        float wj;
+        if constexpr (C == kWave) {
+            u = __builtin_amdgcn_readlane(u_l, j);
+            wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), j));
+        } else {
+            u = __shfl(u_l, lane - c + j, kWave);
+            wj = __shfl(w_l, lane - c + j, kWave);
+        }
+        const int32_t p = p0 + j;
+        const bool ok = p >= beg && p < end;
+        g[j] = ok ? a.x_in[(size_t)u * C + c] : 0.0f;
+        w[j] = ok ? wj : 0.0f;
+    }
+}
+
+// Hub segment quarter [q0, q1) of one row: its aligned batches are dealt to
+// 8 fixed work lines (batch i to line i mod 8), each line summed as a row
+// range, and the 8 line totals combined in the class tree.  Groups take the
+// lines g, g+G, ...; the result does not depend on G.
+template <int C>
+__device__ __forceinline__ double hub_quarter(const SpmmArgs& a, int32_t q0, int32_t q1) {
     constexpr int G = kWave / C;
-    constexpr int R = 8 / G;
     const int lane = lane_id();
     const int grp = lane / C;
     const int c = lane % C;
-    double acc[R];
+    const int32_t base = q0 & ~7;
+    if constexpr (G == 1) {
+        // one group walks the 8 lines in order; the tree is built as it goes
+        double pair = 0.0, quad = 0.0, half = 0.0;
+#pragma unroll 1
+        for (int L = 0; L < 8; ++L) {
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int32_t p0 = base + 8 * L; p0 < q1; p0 += 64) {
+                float g[8], w[8];
+                gather_batch<C>(a, p0, q0, q1, g, w);
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = 0.0;
-    for (int32_t p0 = beg; p0 < end; p0 += kWave) {
-        const int32_t p = p0 + lane;
-        const bool ok = p < end;
-        const int32_t u_l = ok ? __builtin_nontemporal_load(&a.col[p]) : 0;
-        const float w_l = ok ? __builtin_nontemporal_load(&a.nval[p]) : 0.0f;
-        const int32_t cnt = min(kWave, end - p0);
-        // lane takes chunk arcs k = grp + G*t, t = 0 .. C-1, eight loads in flight
-        for (int t0 = 0; t0 < C && G * t0 < cnt; t0 += kUnroll) {
-            float g[kUnroll];
-            float w[kUnroll];
-#pragma unroll
-            for (int q = 0; q < kUnroll; ++q) {
-                const int k = grp + G * (t0 + q);
-                int32_t u;
-                if constexpr (G == 1) {
-                    u = __builtin_amdgcn_readlane(u_l, t0 + q);
-                    w[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), t0 + q));
-                } else {
-                    u = __shfl(u_l, k & (kWave - 1), kWave);
-                    w[q] = __shfl(w_l, k & (kWave - 1), kWave);
-                }
-                g[q] = (k < cnt) ? a.x_in[(size_t)u * C + c] : 0.0f;
-                if (k >= cnt) w[q] = 0.0f;
+                for (int j = 0; j < 8; ++j) acc[j] += (double)w[j] * (double)g[j];
             }
-#pragma unroll
-            for (int q = 0; q < kUnroll; ++q) acc[q % R] += (double)w[q] * (double)g[q];  // t0 % R == 0
+            const double lt = class_tree(acc);
+            if ((L & 1) == 0) {
+                pair = lt;
+            } else {
+                const double p2 = pair + lt;          // (t[L-1] + t[L])
+                if ((L & 2) == 0) {
+                    quad = p2;
+                } else {
+                    const double q4 = quad + p2;      // ((..)+(..)) over 4 lines
+                    if (L == 3) half = q4;
+                    else return half + q4;            // L == 7
+                }
+            }
         }
+        return 0.0;  // not reached
     }
-    // classes differing in q bit 0, 1, 2 in that order: group bits are the
-    // low bits of q (lane xor C, 2C, ...), accumulator bits the high ones
+    double line_tot[8 / G];
 #pragma unroll
-    for (int off = C; off < kWave; off <<= 1) {
+    for (int k = 0; k < 8 / G; ++k) {
+        const int L = grp + G * k;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int32_t p0 = base + 8 * L; p0 < q1; p0 += 64) {
+            float g[8], w[8];
+            gather_batch<C>(a, p0, q0, q1, g, w);
 #pragma unroll
-        for (int r = 0; r < R; ++r) acc[r] += __shfl_xor(acc[r], off, kWave);
+            for (int j = 0; j < 8; ++j) acc[j] += (double)w[j] * (double)g[j];
+        }
+        line_tot[k] = class_tree(acc);
     }
+    // line L lives in group L % G, slot L / G
+    double t[8];
 #pragma unroll
-    for (int span = 1; span < R; span <<= 1) {
-#pragma unroll
-        for (int r = 0; r < R; r += 2 * span) acc[r] += acc[r + span];
-    }
-    return acc[0];
+    for (int L = 0; L < 8; ++L) t[L] = __shfl(line_tot[L / G], (L % G) * C + c, kWave);
+    return class_tree(t);
 }
 
 // Row inputs of the update that do not depend on the gathers; loaded before
@@ -167,12 +210,14 @@ __device__ __forceinline__ RowIn row_inputs(const SpmmArgs& a, int32_t v) {
     return r;
 }
 
-// Lanes c < C finalise column c of row v.
+// Column c (= lane % C) of row v: new value, and this row's contributions to
+// the column's L1 change, sum and dangling mass.
 template <int C>
 __device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, double cj, int32_t v, double acc,
-                                         double inv_n, const RowIn& in, double& err, double& sum, double& dang) {
-    const int c = lane_id();
+                                         double inv_n, const RowIn& in, double& e, double& su, double& dg) {
+    const int c = lane_id() % C;
     const size_t at = (size_t)v * C + c;
+    e = su = dg = 0.0;
     if (!cs.active || cs.done) {
         a.x_out[at] = in.xo;
         return;
@@ -181,12 +226,134 @@ __device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, 
     if (cs.uniform) tele = inv_n;
     else tele = (in.ts >= 0) ? (double)a.ptab[(size_t)in.ts * kCols + c] : 0.0;
     const float xn = (float)(cs.alpha * acc + cj * tele);
-    err += fabs((double)xn - (double)in.xo);
-    sum += (double)xn;
-    if (in.deg == 0.0f) dang += (double)xn;
+    e = fabs((double)xn - (double)in.xo);
+    su = (double)xn;
+    if (in.deg == 0.0f) dg = (double)xn;
     a.x_out[at] = xn;
 }
 
+// Rows [r0, r1), narrow widths: group g takes rows r0+g, r0+g+G, ... in
+// rounds of G rows, so G rows are in flight at once.  Each round's row
+// contributions are folded into the column sums in row order.
+template <int C>
+__device__ __forceinline__ void rows_grouped(const SpmmArgs& a, const ColState& cs, double cj, double inv_n,
+                                             int32_t r0, int32_t r1, double& err, double& sum, double& dang) {
+    constexpr int G = kWave / C;
+    const int lane = lane_id();
+    const int grp = lane / C;
+    const int c = lane % C;
+    for (int32_t rb = r0; rb < r1; rb += G) {
+        const int32_t v = rb + grp;
+        const bool has = v < r1;
+        RowIn in{};
+        int32_t beg = 0, end = 0;
+        if (has) {
+            in = row_inputs<C>(a, v);
+            beg = a.row[v];
+            end = a.row[v + 1];
+        }
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int32_t p0 = beg & ~7; p0 < end; p0 += 8) {
+            float g[8], w[8];
+            gather_batch<C>(a, p0, beg, end, g, w);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] += (double)w[j] * (double)g[j];
+        }
+        double e = 0.0, su = 0.0, dg = 0.0;
+        if (has) finalize<C>(a, cs, cj, v, class_tree(acc), inv_n, in, e, su, dg);
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            err += __shfl(e, k * C + c, kWave);
+            sum += __shfl(su, k * C + c, kWave);
+            dang += __shfl(dg, k * C + c, kWave);
+        }
+    }
+}
+
+// Rows [r0, r1), width 64: one stream of aligned batches over all their arcs,
+// crossing row boundaries, so short rows do not limit the gathers in flight.
+// Within a batch, arcs [lo, hi) belong to the current row; a row that ends
+// inside the batch is finished and the rest of the batch goes to the next.
+// Row end offsets come 64 rows at a time, one per lane.
+__device__ __forceinline__ void rows_streamed(const SpmmArgs& a, const ColState& cs, double cj, double inv_n,
+                                              int32_t r0, int32_t r1, double& err, double& sum, double& dang) {
+    if (r0 >= r1) return;
+    const int lane = lane_id();
+    const int32_t A0 = a.row[r0], A1 = a.row[r1];
+    int32_t wbase = r0;  // rows [wbase, wbase+64) are in the lane window
+    int32_t off_l = (wbase + 1 + lane <= r1) ? a.row[wbase + 1 + lane] : A1;  // end offset of row wbase+lane
+    int32_t v = r0;
+    int32_t vend = __builtin_amdgcn_readlane(off_l, 0);
+    RowIn in = row_inputs<kWave>(a, v);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int32_t p0 = A0 & ~7; v < r1; p0 += 8) {
+        double t[8];
+        if (p0 < A1) {
+            float g[8], w[8];
+            gather_batch<kWave>(a, p0, A0, A1, g, w);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = (double)w[j] * (double)g[j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = 0.0;  // trailing rows without arcs
+        }
+        int32_t lo = max(A0, p0) - p0;
+        for (;;) {
+            const int32_t hi = min(vend, p0 + 8) - p0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j >= lo && j < hi) acc[j] += t[j];
+            if (vend >= p0 + 8) break;  // the row goes on past this batch
+            // row v ends inside this batch: update it and move to row v+1
+            double e, su, dg;
+            finalize<kWave>(a, cs, cj, v, class_tree(acc), inv_n, in, e, su, dg);
+            err += e;
+            sum += su;
+            dang += dg;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[j] = 0.0;
+            lo = max(lo, hi);
+            if (++v >= r1) break;
+            if (v - wbase >= kWave) {
+                wbase = v;
+                off_l = (wbase + 1 + lane <= r1) ? a.row[wbase + 1 + lane] : A1;
+            }
+            vend = __builtin_amdgcn_readlane(off_l, v - wbase);
+            in = row_inputs<kWave>(a, v);
+        }
+    }
+}
+
+// Hub segments (a launch of their own: their register needs differ from
+// the row kernel's).  Block b is segment b; its partial slots are zeroed.
+template <int C>
+__global__ __launch_bounds__(kRowThreads) void k_spmm_hub(SpmmArgs a) {
+    __shared__ double sh[kRowWaves][kCols];
+    if (!a.ctl->any_active) return;
+    const int cur = a.ctl->cur;
+    // selects, not an index: a runtime index would move the argument block to scratch
+    a.x_in = cur ? a.xbuf[1] : a.xbuf[0];
+    a.x_out = cur ? a.xbuf[0] : a.xbuf[1];
+    const int lane = lane_id();
+    const int w = wave_id();
+    const uint32_t b = blockIdx.x;
+    // four wavefronts take contiguous quarters, combined in order
+    const HubSeg sg = a.hsegs[b];
+    const int32_t len = sg.end - sg.beg;
+    const int32_t q0 = sg.beg + (int32_t)(((int64_t)len * w) / kRowWaves);
+    const int32_t q1 = sg.beg + (int32_t)(((int64_t)len * (w + 1)) / kRowWaves);
+    sh[w][lane] = hub_quarter<C>(a, q0, q1);
+    __syncthreads();
+    if (w == 0) {
+        double tot = 0.0;
+        for (int k = 0; k < kRowWaves; ++k) tot += sh[k][lane];
+        if (lane < C) a.hub_part[(size_t)b * kCols + lane] = tot;
+        for (int q = 0; q < 3; ++q) a.partials[((size_t)b * 3 + q) * kCols + lane] = 0.0;
+    }
+}
+
+// Non-hub rows: block b (partial slot nblk_h + b) holds 4 wavefronts of the
+// merge-path split.
 template <int C>
 __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     __shared__ double sh[3][kRowWaves][kCols];
@@ -198,46 +365,27 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     const int lane = lane_id();
     const int w = wave_id();
     const uint32_t b = blockIdx.x;
-    const bool owner = lane < C;  // lanes that finalise a column
+    const bool owner = lane < C;  // lanes that write a column's partials
     const ColState cs = a.cols[lane % C];
     const double cj = (1.0 - cs.alpha) + cs.alpha * cs.dang;
     const double inv_n = 1.0 / (double)a.n;
-
-    if (b < a.nblk_h) {
-        // hub segment: four wavefronts take contiguous quarters, combined in order
-        const HubSeg sg = a.hsegs[b];
-        const int32_t len = sg.end - sg.beg;
-        const int32_t q0 = sg.beg + (int32_t)(((int64_t)len * w) / kRowWaves);
-        const int32_t q1 = sg.beg + (int32_t)(((int64_t)len * (w + 1)) / kRowWaves);
-        sh[0][w][lane] = row_dot<C>(a, q0, q1);
-        __syncthreads();
-        if (w == 0) {
-            double tot = 0.0;
-            for (int k = 0; k < kRowWaves; ++k) tot += sh[0][k][lane];
-            if (owner) a.hub_part[(size_t)b * kCols + lane] = tot;
-            for (int q = 0; q < 3; ++q) a.partials[((size_t)b * 3 + q) * kCols + lane] = 0.0;
-        }
-        return;
-    }
     double err = 0.0, sum = 0.0, dang = 0.0;
-    const uint32_t gw = (b - a.nblk_h) * kRowWaves + w;
+    const uint32_t gw = b * kRowWaves + w;
     if (gw < a.n_waves) {
         const int32_t r0 = a.wave_rows[gw], r1 = a.wave_rows[gw + 1];
-        for (int32_t v = r0; v < r1; ++v) {
-            const RowIn in = row_inputs<C>(a, v);
-            const double acc = row_dot<C>(a, a.row[v], a.row[v + 1]);
-            if (owner) finalize<C>(a, cs, cj, v, acc, inv_n, in, err, sum, dang);
-        }
+        if constexpr (C == kWave) rows_streamed(a, cs, cj, inv_n, r0, r1, err, sum, dang);
+        else rows_grouped<C>(a, cs, cj, inv_n, r0, r1, err, sum, dang);
     }
-    sh[0][w][lane] = err;
-    sh[1][w][lane] = sum;
-    sh[2][w][lane] = dang;
+    sh[0][w][lane] = owner ? err : 0.0;
+    sh[1][w][lane] = owner ? sum : 0.0;
+    sh[2][w][lane] = owner ? dang : 0.0;
     __syncthreads();
     if (w == 0) {
+        const size_t slot = (size_t)a.nblk_h + b;
         for (int q = 0; q < 3; ++q) {
             double t = 0.0;
             for (int k = 0; k < kRowWaves; ++k) t += sh[q][k][lane];
-            a.partials[((size_t)b * 3 + q) * kCols + lane] = t;
+            a.partials[(slot * 3 + q) * kCols + lane] = t;
         }
     }
 }
@@ -262,7 +410,11 @@ __global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
             const HubRow hr = a.hrows[h];
             double acc = 0.0;
             for (int32_t k = 0; k < hr.nseg; ++k) acc += a.hub_part[(size_t)(hr.first + k) * kCols + lane];
-            finalize<C>(a, cs, cj, hr.row, acc, inv_n, row_inputs<C>(a, hr.row), err, sum, dang);
+            double e, su, dg;
+            finalize<C>(a, cs, cj, hr.row, acc, inv_n, row_inputs<C>(a, hr.row), e, su, dg);
+            err += e;
+            sum += su;
+            dang += dg;
         }
     }
     for (uint32_t b = w; b < a.grid; b += kRedWaves) {
@@ -847,14 +999,26 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
 
 // `iters` iterations (rows + reduce launches) at the batch's current width.
 int enqueue_iterations(Batch& b, const SpmmArgs& a, int32_t iters, hipStream_t s) {
+    const uint32_t rgrid = b.grid - b.nblk_h;
     for (int32_t i = 0; i < iters; ++i) {
-        switch (b.width) {
-            case 8: k_spmm_rows<8><<<b.grid, kRowThreads, 0, s>>>(a); break;
-            case 16: k_spmm_rows<16><<<b.grid, kRowThreads, 0, s>>>(a); break;
-            case 32: k_spmm_rows<32><<<b.grid, kRowThreads, 0, s>>>(a); break;
-            default: k_spmm_rows<64><<<b.grid, kRowThreads, 0, s>>>(a); break;
+        if (b.nblk_h > 0) {
+            switch (b.width) {
+                case 8: k_spmm_hub<8><<<b.nblk_h, kRowThreads, 0, s>>>(a); break;
+                case 16: k_spmm_hub<16><<<b.nblk_h, kRowThreads, 0, s>>>(a); break;
+                case 32: k_spmm_hub<32><<<b.nblk_h, kRowThreads, 0, s>>>(a); break;
+                default: k_spmm_hub<64><<<b.nblk_h, kRowThreads, 0, s>>>(a); break;
+            }
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
         }
-        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        if (rgrid > 0) {
+            switch (b.width) {
+                case 8: k_spmm_rows<8><<<rgrid, kRowThreads, 0, s>>>(a); break;
+                case 16: k_spmm_rows<16><<<rgrid, kRowThreads, 0, s>>>(a); break;
+                case 32: k_spmm_rows<32><<<rgrid, kRowThreads, 0, s>>>(a); break;
+                default: k_spmm_rows<64><<<rgrid, kRowThreads, 0, s>>>(a); break;
+            }
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        }
         switch (b.width) {
             case 8: k_spmm_reduce<8><<<1, kRedThreads, 0, s>>>(a); break;
             case 16: k_spmm_reduce<16><<<1, kRedThreads, 0, s>>>(a); break;
