@@ -89,6 +89,10 @@ def multi_graph(count=512, K=20, seed=2024, plain_sample=512):
     _, iters, st = native.multi_pagerank(graphs, max_iter=K, tol=0.0, download=False)
     wall = time.perf_counter() - t0
     assert (iters == K).all()
+    # the same launch with XCD placement of the work items
+    t0 = time.perf_counter()
+    native.multi_pagerank(graphs, max_iter=K, tol=0.0, download=False, xcd_placement=True)
+    wall_xcd = time.perf_counter() - t0
     sample = graphs[: max(1, min(plain_sample, count))]
     t0 = time.perf_counter()
     for g in sample:
@@ -107,6 +111,7 @@ def multi_graph(count=512, K=20, seed=2024, plain_sample=512):
         "batch_solves_per_s": round(batch_rate, 2),
         "batch_wall_s": round(wall, 4),
         "batch_gteps": round(arcs * K / wall / 1e9, 2),
+        "batch_solves_per_s_xcd_placement": round(count / wall_xcd, 2),
         "launch_workgroups": int(st["spmv_grid"]),
         "plain_solves_per_s": round(plain_rate, 2),
         "plain_calls_timed": len(sample),

@@ -421,6 +421,7 @@ egx::SolveParams solve_params(const egx_solve_opts* opts, int32_t flags) {
     p.check_every = opts->check_every;
     p.time_each_launch = (flags & EGX_SOLVE_TIME_LAUNCHES) != 0;
     p.use_graph = (flags & EGX_SOLVE_HIP_GRAPH) != 0 && !p.time_each_launch;
+    p.xcd_placement = (flags & EGX_SOLVE_XCD_PLACEMENT) != 0;
     return p;
 }
 

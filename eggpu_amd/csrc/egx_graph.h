@@ -167,6 +167,7 @@ struct SolveParams {
     int32_t check_every = 0;
     bool time_each_launch = false;  // hipEvent pair around every launch
     bool use_graph = false;         // replay captured runs of check_every iterations
+    bool xcd_placement = false;     // multi-graph: keep a small graph's work items on one XCD
 };
 struct SolveStats {
     double kernel_seconds = 0.0;  // first to last iteration launch

@@ -38,6 +38,7 @@
 // graph's result is bit-identical alone and in a batch).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -592,30 +593,40 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
         return EGX_UNSUPPORTED_GRAPH;
     }
     // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs
-    // on XCD b % 8), and each XCD has its own L2.  A graph whose gathered
-    // vector fits in one L2 gets all its work items on one XCD (the least
-    // loaded, largest graphs first), so that vector is cached once instead
-    // of eight times; larger graphs are spread over every XCD.
+    // on XCD b % 8), and each XCD has its own L2.  With XCD placement
+    // (EGX_SOLVE_XCD_PLACEMENT) a graph whose gathered vector fits in one L2
+    // gets all its work items on one XCD (the least loaded, largest graphs
+    // first), so that vector is cached once instead of eight times; larger
+    // graphs are spread over every XCD.  Otherwise work items go out graph
+    // after graph.
     std::vector<std::vector<std::pair<int32_t, uint32_t>>> lanes(kXcds);
     {
+        uint64_t next_lane = 0;
         std::vector<int32_t> order((size_t)count);
         for (int32_t i = 0; i < count; ++i) order[(size_t)i] = i;
         std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
             return table[(size_t)x].grid > table[(size_t)y].grid;
         });
         std::vector<uint64_t> load(kXcds, 0);
+        const bool placement = p.xcd_placement;
         for (int32_t gi : order) {
             const uint32_t grid = table[(size_t)gi].grid;
-            const bool affine = (int64_t)gs[(size_t)gi]->n * (int64_t)sizeof(T) <= kAffineBytes;
+            const bool affine = placement && (int64_t)gs[(size_t)gi]->n * (int64_t)sizeof(T) <= kAffineBytes;
             if (affine) {
                 const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
                 for (uint32_t k = 0; k < grid; ++k) lanes[(size_t)x].emplace_back(gi, k);
                 load[(size_t)x] += grid;
-            } else {
+            } else if (placement) {
                 for (uint32_t k = 0; k < grid; ++k) {
                     const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
                     lanes[(size_t)x].emplace_back(gi, k);
                     load[(size_t)x] += 1;
+                }
+            } else {
+                // plain order: graph after graph, work items dealt round-robin
+                for (uint32_t k = 0; k < grid; ++k) {
+                    const int x = (int)(next_lane++ % kXcds);
+                    lanes[(size_t)x].emplace_back(gi, k);
                 }
             }
         }

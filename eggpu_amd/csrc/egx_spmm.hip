@@ -37,8 +37,6 @@ namespace {
 constexpr int kCols = 64;            // columns per batch (one per lane)
 constexpr int kRowThreads = 256;     // 4 wavefronts
 constexpr int kRowWaves = kRowThreads / kWave;
-constexpr int kRedThreads = 1024;    // reduce launch
-constexpr int kRedWaves = kRedThreads / kWave;
 constexpr int kSpmmHubSeg = 2048;    // arcs per hub segment (one workgroup)
 constexpr int kUnroll = 8;
 constexpr int kMinWidth = 8;         // narrowest X row (columns)
@@ -83,11 +81,13 @@ struct SpmmArgs {
     ColState* cols;
     BatchCtl* ctl;
     double* partials;          // [grid][3][64]
+    double* red_part;          // [n_red][3][64]
     int64_t n_hrows;
     int32_t n;
     uint32_t nblk_h;
     uint32_t grid;
     uint32_t n_waves;
+    uint32_t n_red;            // reduce-part workgroups
 };
 
 // Width-C launches: a wavefront holds G = 64/C lane groups of C columns;
@@ -272,31 +272,64 @@ __device__ __forceinline__ void rows_grouped(const SpmmArgs& a, const ColState& 
 
 // Rows [r0, r1), width 64: one stream of aligned batches over all their arcs,
 // crossing row boundaries, so short rows do not limit the gathers in flight.
-// Within a batch, arcs [lo, hi) belong to the current row; a row that ends
-// inside the batch is finished and the rest of the batch goes to the next.
-// Row end offsets come 64 rows at a time, one per lane.
+// The arc indices of 64 arcs at a time come in one load (the next 64 are
+// prefetched), and the gathers of batch i+1 are issued before batch i is
+// consumed: 16 row loads in flight per wavefront.  Within a batch, arcs
+// [lo, hi) belong to the current row; a row that ends inside the batch is
+// finished and the rest of the batch goes to the next.  Row end offsets come
+// 64 rows at a time, one per lane.
 __device__ __forceinline__ void rows_streamed(const SpmmArgs& a, const ColState& cs, double cj, double inv_n,
                                               int32_t r0, int32_t r1, double& err, double& sum, double& dang) {
     if (r0 >= r1) return;
     const int lane = lane_id();
     const int32_t A0 = a.row[r0], A1 = a.row[r1];
+    const int32_t P0 = A0 & ~7;
+    // 64 arcs [cb, cb+64): lane l holds arc cb+l
+    auto load_chunk = [&](int32_t cb, int32_t& u_l, float& w_l) {
+        const int32_t p = cb + lane;
+        const bool ok = p >= A0 && p < A1;
+        u_l = ok ? __builtin_nontemporal_load(&a.col[p]) : 0;
+        w_l = ok ? __builtin_nontemporal_load(&a.nval[p]) : 0.0f;
+    };
+    // the 8 gathers of batch [p0, p0+8), whose arcs sit at lanes k..k+7 of the chunk
+    auto issue = [&](int32_t p0, int32_t k, int32_t u_l, float w_l, float (&g)[8], float (&w)[8]) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int32_t u = __builtin_amdgcn_readlane(u_l, k + j);
+            const float wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), k + j));
+            const int32_t p = p0 + j;
+            const bool ok = p >= A0 && p < A1;
+            g[j] = ok ? a.x_in[(size_t)u * kWave + lane] : 0.0f;
+            w[j] = ok ? wj : 0.0f;
+        }
+    };
+    int32_t cu, cn;
+    float cw, cwn;
+    load_chunk(P0, cu, cw);
+    load_chunk(P0 + 64, cn, cwn);
+    float g[8], w[8];
+    issue(P0, 0, cu, cw, g, w);
+
     int32_t wbase = r0;  // rows [wbase, wbase+64) are in the lane window
     int32_t off_l = (wbase + 1 + lane <= r1) ? a.row[wbase + 1 + lane] : A1;  // end offset of row wbase+lane
     int32_t v = r0;
     int32_t vend = __builtin_amdgcn_readlane(off_l, 0);
     RowIn in = row_inputs<kWave>(a, v);
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int32_t p0 = A0 & ~7; v < r1; p0 += 8) {
-        double t[8];
-        if (p0 < A1) {
-            float g[8], w[8];
-            gather_batch<kWave>(a, p0, A0, A1, g, w);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) t[j] = (double)w[j] * (double)g[j];
-        } else {
-#pragma unroll
-            for (int j = 0; j < 8; ++j) t[j] = 0.0;  // trailing rows without arcs
+    for (int32_t p0 = P0; v < r1; p0 += 8) {
+        // next batch first, so its loads overlap this batch's use
+        const int32_t p1 = p0 + 8;
+        const int32_t k1 = (p1 - P0) & 63;
+        if (k1 == 0) {
+            cu = cn;
+            cw = cwn;
+            load_chunk(p1 + 64, cn, cwn);
         }
+        float g2[8], w2[8];
+        issue(p1, k1, cu, cw, g2, w2);
+        double t[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = (double)w[j] * (double)g[j];
         int32_t lo = max(A0, p0) - p0;
         for (;;) {
             const int32_t hi = min(vend, p0 + 8) - p0;
@@ -320,6 +353,11 @@ __device__ __forceinline__ void rows_streamed(const SpmmArgs& a, const ColState&
             }
             vend = __builtin_amdgcn_readlane(off_l, v - wbase);
             in = row_inputs<kWave>(a, v);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            g[j] = g2[j];
+            w[j] = w2[j];
         }
     }
 }
@@ -390,10 +428,16 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     }
 }
 
+// Reduce, in two launches.  Part r (of n_red workgroups, 4 wavefronts,
+// lane = column) finalises its share of the hub rows from their segment
+// partials (segment order) and folds its share of the per-block partial
+// slots; the final launch (one wavefront) folds the part sums in index
+// order, updates the column states, takes each column's stop decision and
+// flips the buffer parity.  Every order is fixed by the layout alone, so
+// column sums do not depend on the launch width.
 template <int C>
-__global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
-    __shared__ double sh[3][kRedWaves][kCols];
-    __shared__ int sh_any;
+__global__ __launch_bounds__(kRowThreads) void k_spmm_reduce_part(SpmmArgs a) {
+    __shared__ double sh[3][kRowWaves][kCols];
     if (!a.ctl->any_active) return;
     const int cur = a.ctl->cur;
     // selects, not an index: a runtime index would move the argument block to scratch
@@ -401,12 +445,14 @@ __global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
     a.x_out = cur ? a.xbuf[0] : a.xbuf[1];
     const int lane = lane_id();
     const int w = wave_id();
+    const uint32_t r = blockIdx.x;
     const ColState cs = a.cols[lane];
     const double cj = (1.0 - cs.alpha) + cs.alpha * cs.dang;
     const double inv_n = 1.0 / (double)a.n;
     double err = 0.0, sum = 0.0, dang = 0.0;
+    const int64_t h0 = (a.n_hrows * r) / a.n_red, h1 = (a.n_hrows * (r + 1)) / a.n_red;
     if (lane < C) {
-        for (int64_t h = w; h < a.n_hrows; h += kRedWaves) {
+        for (int64_t h = h0 + w; h < h1; h += kRowWaves) {
             const HubRow hr = a.hrows[h];
             double acc = 0.0;
             for (int32_t k = 0; k < hr.nseg; ++k) acc += a.hub_part[(size_t)(hr.first + k) * kCols + lane];
@@ -417,37 +463,50 @@ __global__ __launch_bounds__(kRedThreads) void k_spmm_reduce(SpmmArgs a) {
             dang += dg;
         }
     }
-    for (uint32_t b = w; b < a.grid; b += kRedWaves) {
-        err += a.partials[((size_t)b * 3 + 0) * kCols + lane];
-        sum += a.partials[((size_t)b * 3 + 1) * kCols + lane];
-        dang += a.partials[((size_t)b * 3 + 2) * kCols + lane];
+    const uint64_t b0 = ((uint64_t)a.grid * r) / a.n_red, b1 = ((uint64_t)a.grid * (r + 1)) / a.n_red;
+    for (uint64_t b = b0 + w; b < b1; b += kRowWaves) {
+        err += a.partials[(b * 3 + 0) * kCols + lane];
+        sum += a.partials[(b * 3 + 1) * kCols + lane];
+        dang += a.partials[(b * 3 + 2) * kCols + lane];
     }
     sh[0][w][lane] = err;
     sh[1][w][lane] = sum;
     sh[2][w][lane] = dang;
-    if (threadIdx.x == 0) sh_any = 0;
     __syncthreads();
     if (w == 0) {
-        double e = 0.0, s = 0.0, d = 0.0;
-        for (int k = 0; k < kRedWaves; ++k) {
-            e += sh[0][k][lane];
-            s += sh[1][k][lane];
-            d += sh[2][k][lane];
-        }
-        ColState* st = &a.cols[lane];
-        if (lane < C && st->active && !st->done) {
-            const int32_t it = st->iter + 1;
-            st->err = e;
-            st->sum = s;
-            st->dang = d;
-            st->iter = it;
-            st->done = (e < st->tol_n || it >= st->max_iter) ? 1 : 0;
-            if (!st->done) atomicOr(&sh_any, 1);
+        for (int q = 0; q < 3; ++q) {
+            double t = 0.0;
+            for (int k = 0; k < kRowWaves; ++k) t += sh[q][k][lane];
+            a.red_part[((size_t)r * 3 + q) * kCols + lane] = t;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        a.ctl->any_active = sh_any;
+}
+
+template <int C>
+__global__ __launch_bounds__(kWave) void k_spmm_reduce_final(SpmmArgs a) {
+    if (!a.ctl->any_active) return;
+    const int cur = a.ctl->cur;
+    const int lane = lane_id();
+    double e = 0.0, s = 0.0, d = 0.0;
+    for (uint32_t r = 0; r < a.n_red; ++r) {
+        e += a.red_part[((size_t)r * 3 + 0) * kCols + lane];
+        s += a.red_part[((size_t)r * 3 + 1) * kCols + lane];
+        d += a.red_part[((size_t)r * 3 + 2) * kCols + lane];
+    }
+    ColState* st = &a.cols[lane];
+    int running = 0;
+    if (lane < C && st->active && !st->done) {
+        const int32_t it = st->iter + 1;
+        st->err = e;
+        st->sum = s;
+        st->dang = d;
+        st->iter = it;
+        st->done = (e < st->tol_n || it >= st->max_iter) ? 1 : 0;
+        running = st->done ? 0 : 1;
+    }
+    const bool any = __any(running);
+    if (lane == 0) {
+        a.ctl->any_active = any ? 1 : 0;
         a.ctl->cur = cur ^ 1;
     }
 }
@@ -566,7 +625,8 @@ struct Batch {
     int device = -1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf X[2], nval, tslot, ptab, cols, ctl, partials, hub_part, hsegs, hrows, wave_rows, out;
+    DevBuf X[2], nval, tslot, ptab, cols, ctl, partials, red_part, hub_part, hsegs, hrows, wave_rows, out;
+    uint32_t n_red = 1;
     int cur = 0;
     int32_t width = 0;     // columns per X row (8, 16, 32 or 64): covers every assigned slot
     int32_t cap_cols = 0;  // columns the X buffers hold
@@ -726,6 +786,9 @@ int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     b->grid = b->nblk_h + (uint32_t)((waves + kRowWaves - 1) / kRowWaves);
     EGX_TRY(b->partials.alloc((size_t)b->grid * 3 * kCols * 8));
+    // reduce parts: about 64 hub rows + partial slots each
+    b->n_red = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, (b->n_hrows + b->grid + 63) / 64));
+    EGX_TRY(b->red_part.alloc((size_t)b->n_red * 3 * kCols * 8));
     EGX_TRY(b->out.alloc((size_t)n * 8));
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     out = std::move(b);
@@ -957,6 +1020,8 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
     a.cols = b.cols.as<ColState>();
     a.ctl = b.ctl.as<BatchCtl>();
     a.partials = b.partials.as<double>();
+    a.red_part = b.red_part.as<double>();
+    a.n_red = b.n_red;
     a.n_hrows = b.n_hrows;
     a.n = g.n;
     a.nblk_h = b.nblk_h;
@@ -1020,10 +1085,17 @@ int enqueue_iterations(Batch& b, const SpmmArgs& a, int32_t iters, hipStream_t s
             EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
         }
         switch (b.width) {
-            case 8: k_spmm_reduce<8><<<1, kRedThreads, 0, s>>>(a); break;
-            case 16: k_spmm_reduce<16><<<1, kRedThreads, 0, s>>>(a); break;
-            case 32: k_spmm_reduce<32><<<1, kRedThreads, 0, s>>>(a); break;
-            default: k_spmm_reduce<64><<<1, kRedThreads, 0, s>>>(a); break;
+            case 8: k_spmm_reduce_part<8><<<a.n_red, kRowThreads, 0, s>>>(a); break;
+            case 16: k_spmm_reduce_part<16><<<a.n_red, kRowThreads, 0, s>>>(a); break;
+            case 32: k_spmm_reduce_part<32><<<a.n_red, kRowThreads, 0, s>>>(a); break;
+            default: k_spmm_reduce_part<64><<<a.n_red, kRowThreads, 0, s>>>(a); break;
+        }
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        switch (b.width) {
+            case 8: k_spmm_reduce_final<8><<<1, kWave, 0, s>>>(a); break;
+            case 16: k_spmm_reduce_final<16><<<1, kWave, 0, s>>>(a); break;
+            case 32: k_spmm_reduce_final<32><<<1, kWave, 0, s>>>(a); break;
+            default: k_spmm_reduce_final<64><<<1, kWave, 0, s>>>(a); break;
         }
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     }

@@ -32,6 +32,7 @@ PRECISION_FP32 = 0
 PRECISION_FP64 = 1
 SOLVE_TIME_LAUNCHES = 1
 SOLVE_HIP_GRAPH = 2
+SOLVE_XCD_PLACEMENT = 4
 
 
 class NativeLibraryMissing(RuntimeError):
@@ -601,7 +602,7 @@ class Server:
 
 
 def multi_pagerank(graphs, alpha=0.85, max_iter=200, tol=1e-6, precision=PRECISION_FP32, check_every=0,
-                   hip_graph=True, download=True):
+                   hip_graph=True, download=True, xcd_placement=False):
     """Global PageRank of many device-resident graphs in shared launches.
     Returns (list of float64[n_i] or None, iterations int32[k], stats dict)."""
     k = len(graphs)
@@ -611,8 +612,9 @@ def multi_pagerank(graphs, alpha=0.85, max_iter=200, tol=1e-6, precision=PRECISI
     iters = np.zeros(max(k, 1), dtype=np.int32)
     st = SolveStats()
     o = SolveOpts(float(alpha), int(max_iter), float(tol), int(precision), int(check_every))
+    flags = (SOLVE_HIP_GRAPH if hip_graph else 0) | (SOLVE_XCD_PLACEMENT if xcd_placement else 0)
     _check(lib().egx_multi_pagerank(handles, k, ctypes.byref(o), out_ptrs, _ptr(iters, _i32p), ctypes.byref(st),
-                                    SOLVE_HIP_GRAPH if hip_graph else 0))
+                                    flags))
     return outs, iters[:k], st.as_dict()
 
 

@@ -223,6 +223,7 @@ typedef struct {
 /* flags for egx_graph_pagerank_ex / egx_multi_pagerank */
 #define EGX_SOLVE_TIME_LAUNCHES 1 /* time every launch (no hipGraph replay) */
 #define EGX_SOLVE_HIP_GRAPH 2     /* capture check_every iterations once, replay the hipGraph */
+#define EGX_SOLVE_XCD_PLACEMENT 4 /* multi-graph: put each small graph's work items on one XCD */
 
 int egx_graph_pagerank_ex(egx_graph *g, const egx_solve_opts *opts, double *PR, egx_solve_stats *stats,
                           int32_t flags);

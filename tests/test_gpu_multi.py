@@ -50,9 +50,9 @@ def test_multi_equals_single_bitwise(gpu_ready, weighted, precision):
     for i, g in enumerate(graphs):  # mixed bin settings inside one launch
         g.set_bins(tile_sort=i % 2, tile_items=(2944, 5888, 4416)[i % 3])
     alone = [g.pagerank(0.85, 100, 1e-6, precision=precision) for g in graphs]
-    for hip_graph in (False, True):
+    for hip_graph, xcd in ((False, False), (True, False), (True, True)):
         outs, iters, st = native.multi_pagerank(graphs, 0.85, 100, 1e-6, precision=precision,
-                                                hip_graph=hip_graph)
+                                                hip_graph=hip_graph, xcd_placement=xcd)
         for i, g in enumerate(graphs):
             assert iters[i] == alone[i][1], (i, hip_graph)
             assert np.array_equal(outs[i], alone[i][0]), (i, hip_graph)
