@@ -8,15 +8,16 @@
 //   c_j     = (1 - alpha_j) + alpha_j * dang_j,  nval = w / outW[u] (FP32)
 //   err_j   = sum_v |x'_j[v] - x_j[v]|,  stop column j when err_j < tol_j*n
 //
-// X is stored row-major, n x 64 FP32: the 64 values of one node are one
-// 256-byte line pair, so each arc costs one coalesced row load shared by
-// all 64 columns.  Lane j of a wavefront owns column j; the wavefront walks
-// the in-arcs of a row, broadcasting each (source, value) pair from the lane
-// that loaded it with v_readlane, 8 row loads in flight.  Row sums, c_j and
-// every reduction are FP64.  Rows are split over wavefronts by a merge-path
-// split of (arcs + 2*rows); hub rows (the prefix of the iteration layout)
-// are cut into segments whose 64-wide FP64 partials the reduce launch adds
-// in segment order.  A column whose stop flag is set is copied through
+// X is stored row-major, n x C FP32 (C in {8, 16, 32, 64}, the batch width):
+// the C values of one node are one contiguous row, so each arc costs one
+// coalesced row load shared by all C columns.  A wavefront gathers 64/C arcs
+// x C columns per instruction (lane = arc group * C + column) while walking
+// one contiguous range of rows as a single stream, arc indices loaded 64 at a
+// time and broadcast with shuffles, the next step's row loads issued before
+// the current step is summed.  Row sums, c_j and every reduction are FP64.
+// Rows are split over wavefronts by a merge-path split of (arcs + 2*rows);
+// hub rows (the prefix of the iteration layout) are cut into segments whose
+// FP64 partials the reduce launch adds in segment order.  A column whose stop flag is set is copied through
 // unchanged, so its values do not depend on which other columns share the
 // launch.  Teleport vectors are sparse: tslot[v] indexes a small table of
 // per-column seed weights (-1 when v seeds no column).
@@ -104,95 +105,6 @@ __device__ __forceinline__ double class_tree(const double (&acc)[8]) {
     return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
-// Gather the aligned batch [p0, p0+8) restricted to [beg, end) for the
-// lane's group: lanes c < 8 of the group load the arcs' (source, value) and
-// every lane of the group reads them from there; eight row loads in flight.
-template <int C>
-__device__ __forceinline__ void gather_batch(const SpmmArgs& a, int32_t p0, int32_t beg, int32_t end, float (&g)[8],
-                                             float (&w)[8]) {
-    const int lane = lane_id();
-    const int c = lane % C;
-    const int32_t pl = p0 + (c & 7);
-    const bool okl = c < 8 && pl >= beg && pl < end;
-    const int32_t u_l = okl ? __builtin_nontemporal_load(&a.col[pl]) : 0;
-    const float w_l = okl ? __builtin_nontemporal_load(&a.nval[pl]) : 0.0f;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        int32_t u;
-        float wj;
-        if constexpr (C == kWave) {
-            u = __builtin_amdgcn_readlane(u_l, j);
-            wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), j));
-        } else {
-            u = __shfl(u_l, lane - c + j, kWave);
-            wj = __shfl(w_l, lane - c + j, kWave);
-        }
-        const int32_t p = p0 + j;
-        const bool ok = p >= beg && p < end;
-        g[j] = ok ? a.x_in[(size_t)u * C + c] : 0.0f;
-        w[j] = ok ? wj : 0.0f;
-    }
-}
-
-// Hub segment quarter [q0, q1) of one row: its aligned batches are dealt to
-// 8 fixed work lines (batch i to line i mod 8), each line summed as a row
-// range, and the 8 line totals combined in the class tree.  Groups take the
-// lines g, g+G, ...; the result does not depend on G.
-template <int C>
-__device__ __forceinline__ double hub_quarter(const SpmmArgs& a, int32_t q0, int32_t q1) {
-    constexpr int G = kWave / C;
-    const int lane = lane_id();
-    const int grp = lane / C;
-    const int c = lane % C;
-    const int32_t base = q0 & ~7;
-    if constexpr (G == 1) {
-        // one group walks the 8 lines in order; the tree is built as it goes
-        double pair = 0.0, quad = 0.0, half = 0.0;
-#pragma unroll 1
-        for (int L = 0; L < 8; ++L) {
-            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (int32_t p0 = base + 8 * L; p0 < q1; p0 += 64) {
-                float g[8], w[8];
-                gather_batch<C>(a, p0, q0, q1, g, w);
-#pragma unroll
-                for (int j = 0; j < 8; ++j) acc[j] += (double)w[j] * (double)g[j];
-            }
-            const double lt = class_tree(acc);
-            if ((L & 1) == 0) {
-                pair = lt;
-            } else {
-                const double p2 = pair + lt;          // (t[L-1] + t[L])
-                if ((L & 2) == 0) {
-                    quad = p2;
-                } else {
-                    const double q4 = quad + p2;      // ((..)+(..)) over 4 lines
-                    if (L == 3) half = q4;
-                    else return half + q4;            // L == 7
-                }
-            }
-        }
-        return 0.0;  // not reached
-    }
-    double line_tot[8 / G];
-#pragma unroll
-    for (int k = 0; k < 8 / G; ++k) {
-        const int L = grp + G * k;
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int32_t p0 = base + 8 * L; p0 < q1; p0 += 64) {
-            float g[8], w[8];
-            gather_batch<C>(a, p0, q0, q1, g, w);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) acc[j] += (double)w[j] * (double)g[j];
-        }
-        line_tot[k] = class_tree(acc);
-    }
-    // line L lives in group L % G, slot L / G
-    double t[8];
-#pragma unroll
-    for (int L = 0; L < 8; ++L) t[L] = __shfl(line_tot[L / G], (L % G) * C + c, kWave);
-    return class_tree(t);
-}
-
 // Row inputs of the update that do not depend on the gathers; loaded before
 // them so the latencies overlap.
 struct RowIn {
@@ -232,98 +144,99 @@ __device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, 
     a.x_out[at] = xn;
 }
 
-// Rows [r0, r1), narrow widths: group g takes rows r0+g, r0+g+G, ... in
-// rounds of G rows, so G rows are in flight at once.  Each round's row
-// contributions are folded into the column sums in row order.
-template <int C>
-__device__ __forceinline__ void rows_grouped(const SpmmArgs& a, const ColState& cs, double cj, double inv_n,
-                                             int32_t r0, int32_t r1, double& err, double& sum, double& dang) {
-    constexpr int G = kWave / C;
-    const int lane = lane_id();
-    const int grp = lane / C;
-    const int c = lane % C;
-    for (int32_t rb = r0; rb < r1; rb += G) {
-        const int32_t v = rb + grp;
-        const bool has = v < r1;
-        RowIn in{};
-        int32_t beg = 0, end = 0;
-        if (has) {
-            in = row_inputs<C>(a, v);
-            beg = a.row[v];
-            end = a.row[v + 1];
+// Ordered fold of up to 8 leaves into the fixed tree
+// ((l0+l1)+(l2+l3))+((l4+l5)+(l6+l7)): after 1, 2, 4 or 8 leaves the value
+// of that subtree is result(1, 2, 4, 8).
+struct Fold8 {
+    double l0 = 0.0, l1 = 0.0, l2 = 0.0, l3 = 0.0;
+    int n = 0;
+    __device__ __forceinline__ void add(double x) {
+        if ((n & 1) == 0) {
+            l0 = x;
+        } else {
+            const double p = l0 + x;
+            if ((n & 2) == 0) {
+                l1 = p;
+            } else {
+                const double q = l1 + p;
+                if ((n & 4) == 0) l2 = q;
+                else l3 = l2 + q;
+            }
         }
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int32_t p0 = beg & ~7; p0 < end; p0 += 8) {
-            float g[8], w[8];
-            gather_batch<C>(a, p0, beg, end, g, w);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) acc[j] += (double)w[j] * (double)g[j];
-        }
-        double e = 0.0, su = 0.0, dg = 0.0;
-        if (has) finalize<C>(a, cs, cj, v, class_tree(acc), inv_n, in, e, su, dg);
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-            err += __shfl(e, k * C + c, kWave);
-            sum += __shfl(su, k * C + c, kWave);
-            dang += __shfl(dg, k * C + c, kWave);
-        }
+        ++n;
     }
+    __device__ __forceinline__ double result(int leaves) const {
+        return leaves == 1 ? l0 : leaves == 2 ? l1 : leaves == 4 ? l2 : l3;
+    }
+};
+
+// The upper levels of the 8-leaf tree across the G lane groups of a
+// wavefront: group g holds the subtree of leaves [g*8/G, (g+1)*8/G).
+template <int C>
+__device__ __forceinline__ double across_groups(double v) {
+#pragma unroll
+    for (int off = C; off < kWave; off <<= 1) v += __shfl_xor(v, off, kWave);
+    return v;
 }
 
-// Rows [r0, r1), width 64: one stream of aligned batches over all their arcs,
-// crossing row boundaries, so short rows do not limit the gathers in flight.
-// The arc indices of 64 arcs at a time come in one load (the next 64 are
-// prefetched), and the gathers of batch i+1 are issued before batch i is
-// consumed: 16 row loads in flight per wavefront.  Within a batch, arcs
-// [lo, hi) belong to the current row; a row that ends inside the batch is
-// finished and the rest of the batch goes to the next.  Row end offsets come
-// 64 rows at a time, one per lane.
-__device__ __forceinline__ void rows_streamed(const SpmmArgs& a, const ColState& cs, double cj, double inv_n,
-                                              int32_t r0, int32_t r1, double& err, double& sum, double& dang) {
-    if (r0 >= r1) return;
+// Stream the arcs [A0, A1) of the lane's group, cut into segments ending at
+// seg_end(0) <= seg_end(1) <= ... (nseg of them, empty ones included), and
+// call on_end(k, sum) for each segment in order, sum being its canonical
+// class-tree sum.  The arc indices of C arcs at a time come in one load (the
+// next C are prefetched), and the gathers of batch i+1 are issued before
+// batch i is consumed: 16 row loads in flight per group.  Within a batch,
+// arcs [lo, hi) belong to the current segment.
+template <int C, class SegEnd, class OnEnd>
+__device__ __forceinline__ void group_stream(const SpmmArgs& a, int32_t A0, int32_t A1, int32_t nseg, SegEnd&& seg_end,
+                                             OnEnd&& on_end) {
+    if (nseg <= 0) return;
     const int lane = lane_id();
-    const int32_t A0 = a.row[r0], A1 = a.row[r1];
+    const int c = lane % C;
+    const int gb = lane - c;
     const int32_t P0 = A0 & ~7;
-    // 64 arcs [cb, cb+64): lane l holds arc cb+l
+    // C arcs [cb, cb+C): lane c of the group holds arc cb+c
     auto load_chunk = [&](int32_t cb, int32_t& u_l, float& w_l) {
-        const int32_t p = cb + lane;
+        const int32_t p = cb + c;
         const bool ok = p >= A0 && p < A1;
         u_l = ok ? __builtin_nontemporal_load(&a.col[p]) : 0;
         w_l = ok ? __builtin_nontemporal_load(&a.nval[p]) : 0.0f;
     };
-    // the 8 gathers of batch [p0, p0+8), whose arcs sit at lanes k..k+7 of the chunk
+    // the 8 gathers of batch [p0, p0+8), whose arcs sit at lanes k..k+7 of the group's chunk
     auto issue = [&](int32_t p0, int32_t k, int32_t u_l, float w_l, float (&g)[8], float (&w)[8]) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int32_t u = __builtin_amdgcn_readlane(u_l, k + j);
-            const float wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), k + j));
+            int32_t u;
+            float wj;
+            if constexpr (C == kWave) {
+                u = __builtin_amdgcn_readlane(u_l, k + j);
+                wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), k + j));
+            } else {
+                u = __shfl(u_l, gb + k + j, kWave);
+                wj = __shfl(w_l, gb + k + j, kWave);
+            }
             const int32_t p = p0 + j;
             const bool ok = p >= A0 && p < A1;
-            g[j] = ok ? a.x_in[(size_t)u * kWave + lane] : 0.0f;
+            g[j] = ok ? a.x_in[(size_t)u * C + c] : 0.0f;
             w[j] = ok ? wj : 0.0f;
         }
     };
     int32_t cu, cn;
     float cw, cwn;
     load_chunk(P0, cu, cw);
-    load_chunk(P0 + 64, cn, cwn);
+    load_chunk(P0 + C, cn, cwn);
     float g[8], w[8];
     issue(P0, 0, cu, cw, g, w);
-
-    int32_t wbase = r0;  // rows [wbase, wbase+64) are in the lane window
-    int32_t off_l = (wbase + 1 + lane <= r1) ? a.row[wbase + 1 + lane] : A1;  // end offset of row wbase+lane
-    int32_t v = r0;
-    int32_t vend = __builtin_amdgcn_readlane(off_l, 0);
-    RowIn in = row_inputs<kWave>(a, v);
+    int32_t k = 0;
+    int32_t kend = seg_end(0);
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int32_t p0 = P0; v < r1; p0 += 8) {
+    for (int32_t p0 = P0; k < nseg; p0 += 8) {
         // next batch first, so its loads overlap this batch's use
         const int32_t p1 = p0 + 8;
-        const int32_t k1 = (p1 - P0) & 63;
+        const int32_t k1 = (p1 - P0) & (C - 1);
         if (k1 == 0) {
             cu = cn;
             cw = cwn;
-            load_chunk(p1 + 64, cn, cwn);
+            load_chunk(p1 + C, cn, cwn);
         }
         float g2[8], w2[8];
         issue(p1, k1, cu, cw, g2, w2);
@@ -332,32 +245,180 @@ __device__ __forceinline__ void rows_streamed(const SpmmArgs& a, const ColState&
         for (int j = 0; j < 8; ++j) t[j] = (double)w[j] * (double)g[j];
         int32_t lo = max(A0, p0) - p0;
         for (;;) {
-            const int32_t hi = min(vend, p0 + 8) - p0;
+            const int32_t hi = min(kend, p0 + 8) - p0;
 #pragma unroll
             for (int j = 0; j < 8; ++j)
                 if (j >= lo && j < hi) acc[j] += t[j];
-            if (vend >= p0 + 8) break;  // the row goes on past this batch
-            // row v ends inside this batch: update it and move to row v+1
-            double e, su, dg;
-            finalize<kWave>(a, cs, cj, v, class_tree(acc), inv_n, in, e, su, dg);
-            err += e;
-            sum += su;
-            dang += dg;
+            if (kend >= p0 + 8) break;  // the segment goes on past this batch
+            on_end(k, class_tree(acc));
 #pragma unroll
             for (int j = 0; j < 8; ++j) acc[j] = 0.0;
             lo = max(lo, hi);
-            if (++v >= r1) break;
-            if (v - wbase >= kWave) {
-                wbase = v;
-                off_l = (wbase + 1 + lane <= r1) ? a.row[wbase + 1 + lane] : A1;
-            }
-            vend = __builtin_amdgcn_readlane(off_l, v - wbase);
-            in = row_inputs<kWave>(a, v);
+            if (++k >= nseg) break;
+            kend = seg_end(k);
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             g[j] = g2[j];
             w[j] = w2[j];
+        }
+    }
+}
+
+// Hub segment quarter [q0, q1) of one row: 8 equal pieces, each summed as a
+// segment, combined in the 8-leaf tree.  Group g streams the pieces
+// [g*8/G, (g+1)*8/G); the result does not depend on G.
+template <int C>
+__device__ __forceinline__ double hub_quarter(const SpmmArgs& a, int32_t q0, int32_t q1) {
+    constexpr int G = kWave / C;
+    constexpr int L = 8 / G;
+    const int grp = lane_id() / C;
+    const int64_t len = q1 - q0;
+    auto piece_end = [&](int i) { return q0 + (int32_t)((len * (i + 1)) / 8); };
+    const int first = grp * L;
+    const int32_t A0 = q0 + (int32_t)((len * first) / 8);
+    const int32_t A1 = piece_end(first + L - 1);
+    Fold8 f;
+    group_stream<C>(a, A0, A1, L, [&](int32_t k) { return piece_end(first + k); },
+                    [&](int32_t, double sum) { f.add(sum); });
+    return across_groups<C>(f.result(L));
+}
+
+// Rows [r0, r1) of one wavefront, as one uniform stream over their arcs.
+// An instruction gathers G = 64/C consecutive arcs x C columns (lane =
+// group*C + column, group = arc), so every row end is wavefront-uniform.
+// Steps of S arcs (aligned to 8) keep 16 row loads in flight per lane (8 at
+// C = 8); arc p of a step lands in class p mod 8, held by its group in
+// accumulator slot (p mod 8) / G.  A row that ends inside a step is
+// finished (classes combined across groups, then across slots, in the fixed
+// tree) and the rest of the step goes to the next row.  Row ends, degrees
+// and slots come 64 rows per load, old values G rows per load.  Row
+// contributions are folded into the column sums in row order.
+template <int C>
+__device__ __forceinline__ void wave_stream(const SpmmArgs& a, const ColState& cs, double cj, double inv_n,
+                                            int32_t r0, int32_t r1, double& err, double& sum, double& dang) {
+    constexpr int G = kWave / C;               // arcs per instruction
+    constexpr int R = 8 / G;                   // class slots per lane
+    constexpr int S = (16 * G < 64) ? 16 * G : 64;  // arcs per step
+    constexpr int NI = S / G;                  // gather instructions per step
+    if (r0 >= r1) return;
+    const int lane = lane_id();
+    const int grp = lane / C;
+    const int c = lane % C;
+    const int32_t A0 = a.row[r0], A1 = a.row[r1];
+    const int32_t P0 = A0 & ~7;
+    // arc indices, 64 per load: lane l holds arc cb + l
+    auto load_chunk = [&](int32_t cb, int32_t& u_l, float& w_l) {
+        const int32_t p = cb + lane;
+        const bool ok = p >= A0 && p < A1;
+        u_l = ok ? __builtin_nontemporal_load(&a.col[p]) : 0;
+        w_l = ok ? __builtin_nontemporal_load(&a.nval[p]) : 0.0f;
+    };
+    // gathers of step [ps, ps+S), whose arcs sit at chunk lanes k..k+S-1
+    auto issue = [&](int32_t ps, int32_t k, int32_t u_l, float w_l, float (&g)[NI], float (&w)[NI]) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int off = i * G + grp;
+            int32_t u;
+            float wj;
+            if constexpr (G == 1) {
+                u = __builtin_amdgcn_readlane(u_l, k + i);
+                wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), k + i));
+            } else {
+                u = __shfl(u_l, k + off, kWave);
+                wj = __shfl(w_l, k + off, kWave);
+            }
+            const int32_t p = ps + off;
+            const bool ok = p >= A0 && p < A1;
+            g[i] = ok ? a.x_in[(size_t)u * C + c] : 0.0f;
+            w[i] = ok ? wj : 0.0f;
+        }
+    };
+    int32_t cu, cn;
+    float cw, cwn;
+    load_chunk(P0, cu, cw);
+    load_chunk(P0 + 64, cn, cwn);
+    float g[NI], w[NI];
+    issue(P0, 0, cu, cw, g, w);
+
+    // row windows: ends, degrees and slots of rows [wb, wb+64); old values of rows [xb, xb+G)
+    int32_t wb = r0;
+    int32_t end_l = (wb + 1 + lane <= r1) ? a.row[wb + 1 + lane] : A1;
+    float deg_l = (wb + lane < r1) ? a.deg[wb + lane] : 1.0f;
+    int32_t ts_l = (wb + lane < r1) ? a.tslot[wb + lane] : -1;
+    int32_t xb = r0;
+    float xo_l = (xb + grp < r1) ? a.x_in[(size_t)(xb + grp) * C + c] : 0.0f;
+    int32_t v = r0;
+    int32_t vend = __builtin_amdgcn_readlane(end_l, 0);
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    for (int32_t ps = P0; v < r1; ps += S) {
+        // next step first, so its loads overlap this step's use
+        const int32_t p1 = ps + S;
+        const int32_t k1 = (p1 - P0) & 63;
+        if (k1 == 0) {
+            cu = cn;
+            cw = cwn;
+            load_chunk(p1 + 64, cn, cwn);
+        }
+        float g2[NI], w2[NI];
+        issue(p1, k1, cu, cw, g2, w2);
+        double t[NI];
+#pragma unroll
+        for (int i = 0; i < NI; ++i) t[i] = (double)w[i] * (double)g[i];
+        int32_t lo = max(A0, ps) - ps;
+        for (;;) {
+            const int32_t hi = min(vend, ps + S) - ps;
+#pragma unroll
+            for (int i = 0; i < NI; ++i) {
+                const int off = i * G + grp;
+                if (off >= lo && off < hi) acc[i % R] += t[i];
+            }
+            if (vend >= ps + S) break;  // the row goes on past this step
+            // row v ends inside this step: classes across groups, then slots
+#pragma unroll
+            for (int sh = C; sh < kWave; sh <<= 1) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) acc[r] += __shfl_xor(acc[r], sh, kWave);
+            }
+#pragma unroll
+            for (int span = 1; span < R; span <<= 1) {
+#pragma unroll
+                for (int r = 0; r < R; r += 2 * span) acc[r] += acc[r + span];
+            }
+            const double total = acc[0];
+            RowIn in;
+            in.xo = (G == 1) ? xo_l : __shfl(xo_l, (v - xb) * C + c, kWave);
+            in.deg = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(deg_l), v - wb));
+            in.ts = __builtin_amdgcn_readlane(ts_l, v - wb);
+            if (lane < C) {
+                double e, su, dg;
+                finalize<C>(a, cs, cj, v, total, inv_n, in, e, su, dg);
+                err += e;
+                sum += su;
+                dang += dg;
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] = 0.0;
+            lo = max(lo, hi);
+            if (++v >= r1) break;
+            if (v - wb >= kWave) {
+                wb = v;
+                end_l = (wb + 1 + lane <= r1) ? a.row[wb + 1 + lane] : A1;
+                deg_l = (wb + lane < r1) ? a.deg[wb + lane] : 1.0f;
+                ts_l = (wb + lane < r1) ? a.tslot[wb + lane] : -1;
+            }
+            if (v - xb >= G) {
+                xb = v;
+                xo_l = (xb + grp < r1) ? a.x_in[(size_t)(xb + grp) * C + c] : 0.0f;
+            }
+            vend = __builtin_amdgcn_readlane(end_l, v - wb);
+        }
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            g[i] = g2[i];
+            w[i] = w2[i];
         }
     }
 }
@@ -409,11 +470,7 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     const double inv_n = 1.0 / (double)a.n;
     double err = 0.0, sum = 0.0, dang = 0.0;
     const uint32_t gw = b * kRowWaves + w;
-    if (gw < a.n_waves) {
-        const int32_t r0 = a.wave_rows[gw], r1 = a.wave_rows[gw + 1];
-        if constexpr (C == kWave) rows_streamed(a, cs, cj, inv_n, r0, r1, err, sum, dang);
-        else rows_grouped<C>(a, cs, cj, inv_n, r0, r1, err, sum, dang);
-    }
+    if (gw < a.n_waves) wave_stream<C>(a, cs, cj, inv_n, a.wave_rows[gw], a.wave_rows[gw + 1], err, sum, dang);
     sh[0][w][lane] = owner ? err : 0.0;
     sh[1][w][lane] = owner ? sum : 0.0;
     sh[2][w][lane] = owner ? dang : 0.0;
@@ -775,8 +832,9 @@ int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
     int32_t row_end_hub = 0, row_end = 0;
     EGX_HIP_TRY(hipMemcpy(&row_end_hub, g.it_row + g.n_hub, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpy(&row_end, g.it_row + n, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    // wavefront k holds the rows whose merge-path diagonal starts in its share
     const int64_t work = (int64_t)(row_end - row_end_hub) + 2 * (int64_t)(n - g.n_hub);
-    int64_t waves = std::min<int64_t>(8192, std::max<int64_t>(1, (work + 511) / 512));
+    int64_t waves = std::min<int64_t>(16384, std::max<int64_t>(1, (work + 511) / 512));
     const int64_t per_wave = std::max<int64_t>(1, (work + waves - 1) / waves);
     waves = std::max<int64_t>(1, (work + per_wave - 1) / per_wave);
     b->n_waves = (uint32_t)waves;
