@@ -218,7 +218,8 @@ def main():
             "alpha": 0.85,
             "iterate_dtype": "f32",
             "reductions": "f64 (dangling mass, L1 residual, teleport, row sums)",
-            "bins": {k: info[k] for k in ("thread_row_max", "wave_row_max", "hub_segment", "tile_items")},
+            "bins": {k: info[k] for k in ("thread_row_max", "wave_row_max", "hub_segment", "tile_items", "tile_sort",
+                                                  "hot_sources")},
             "autotune": tuned,
             "hub_rows": info["rows_hub"],
             "tiles": info["tiles"],
