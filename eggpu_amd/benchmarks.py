@@ -200,7 +200,9 @@ def _serving_requests(graphs, count, seed):
 def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, step_iters=4):
     """Poisson arrivals against a device-resident pool; p50/p99 latency at a
     fixed QPS beside the closed-loop solve rate and the one-call-per-solve
-    rate on the same request stream."""
+    rate on the same request stream.  The closed loop runs twice: with
+    request coalescing (identical requests in flight share a column; the
+    server default) and with every request on a column of its own."""
     specs = workloads.serving_pool_specs(pool_size, seed=seed)
     graphs = [s.build(native) for s in specs]
     arcs = [int(g.info()["m"]) for g in graphs]
@@ -218,8 +220,10 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
         else:
             graphs[gi].pagerank(a, 100, 1e-6, download=True)
 
-    for r in reqs[:8]:
-        plain(r)  # warm
+    # warm: every graph's plain solve and its single-column session exist before timing
+    for gi in range(len(graphs)):
+        plain((gi, (), 0.85))
+        plain((gi, (0,), 0.85))
     sample = reqs[:plain_sample]
     t0 = time.perf_counter()
     for r in sample:
@@ -228,8 +232,13 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
     for b in sessions.values():
         b.close()
 
-    def run(rate):
-        with native.Server(graphs, step_iters=step_iters) as srv:
+    def run(rate, coalesce=True):
+        with native.Server(graphs, step_iters=step_iters, coalesce=coalesce) as srv:
+            # warm: open every graph's session (steady-state serving keeps them open)
+            for gi in range(len(graphs)):
+                for rid in [srv.submit(gi, seeds, alpha=0.85, max_iter=100, tol=1e-6) for seeds in ((), (0,))]:
+                    srv.wait(rid)
+            st0 = srv.stats()
             ids = [None] * len(reqs)
             gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=len(reqs)) if rate else None
 
@@ -257,12 +266,12 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
                 widths.append(info["width"])
             wall = time.perf_counter() - t0
             st = srv.stats()
+        st = {k: (v - st0[k] if k in ("coalesced", "sessions_opened", "sessions_closed", "steps") else v)
+              for k, v in st.items()}
         return len(reqs) / wall, np.array(lat), st
 
-    # warm the server path once (sessions, code objects)
-    with native.Server(graphs) as srv:
-        srv.wait(srv.submit(0))
     closed_rate, closed_lat, closed_st = run(None)
+    solo_rate, solo_lat, solo_st = run(None, coalesce=False)  # every request on a column of its own
     target = qps if qps else 0.5 * closed_rate
     rate, lat, st = run(target)
     res = {
@@ -275,11 +284,16 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
         "closed_loop_solves_per_s": round(closed_rate, 1),
         "closed_loop_p50_ms": round(float(np.percentile(closed_lat, 50)) * 1e3, 3),
         "closed_loop_p99_ms": round(float(np.percentile(closed_lat, 99)) * 1e3, 3),
+        "closed_loop_coalesced_requests": closed_st["coalesced"],
+        "closed_loop_no_coalescing_solves_per_s": round(solo_rate, 1),
+        "closed_loop_no_coalescing_p99_ms": round(float(np.percentile(solo_lat, 99)) * 1e3, 3),
         "plain_solves_per_s": round(plain_rate, 1),
         "plain_calls_timed": len(sample),
         "speedup_vs_plain": round(closed_rate / plain_rate, 2),
+        "speedup_vs_plain_no_coalescing": round(solo_rate / plain_rate, 2),
         "peak_active_columns": st["peak_active_columns"],
         "sessions_opened": st["sessions_opened"],
+        "coalesced_requests": st["coalesced"],
     }
     for g in graphs:
         g.close()

@@ -192,5 +192,11 @@ struct egx_graph {
 namespace egx {
 // A new, empty handle bound to the current device, with its stream and events.
 int make_graph(std::unique_ptr<egx_graph>& out);
+
+// egx_batch_result without the wait: queues the normalised result of `slot`
+// on the batch's stream, copies it to host_out (pinned, n doubles; may be
+// null) and records `done` (may be null) after the copy.  host_out must stay
+// valid until `done` completes.
+int batch_result_enqueue(egx_batch* b, int32_t slot, double* host_out, hipEvent_t done);
 }  // namespace egx
 

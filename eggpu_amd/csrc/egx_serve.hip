@@ -14,15 +14,22 @@
 //     idle sessions are closed (least recently used first) to make room;
 //   * each round the worker enqueues `step_iters` iterations on every busy
 //     session before it polls any, so sessions on different graphs overlap
-//     on the device; finished columns are normalised, copied to the
-//     caller's buffer and their slots freed.
+//     on the device; a finished column is normalised and copied into a
+//     pinned staging buffer on its session's stream and its slot freed at
+//     once; delivery threads wait for the copy and move the result into the
+//     callers' buffers, so the worker never blocks on a result.
 //
 // A column's arithmetic does not depend on the other columns of its session
 // (inactive and finished columns are copied through; sums use a fixed
-// order), so each result is bit-identical alone and in the mix.
+// order), so each result is bit-identical alone and in the mix.  For the
+// same reason a request identical to one already in a slot of its graph's
+// session (same seed list, alpha, max_iter and tol) rides on that column:
+// it receives the column's result and iteration count, exactly what a
+// column of its own would have produced (request coalescing).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <list>
 #include <map>
@@ -42,6 +49,7 @@ double seconds_since(Clock::time_point t0, Clock::time_point t) {
 }
 
 constexpr int kSlots = 64;
+constexpr int kDeliveryThreads = 4;
 
 struct Request {
     int64_t id = 0;
@@ -56,6 +64,11 @@ struct Request {
     int32_t width = 0;  // columns of the session when the request finished
     int status = EGX_SUCC;
     bool done = false;
+    std::vector<std::shared_ptr<Request>> riders;  // identical requests sharing this column
+
+    bool same_solve(const Request& o) const {
+        return graph == o.graph && seeds == o.seeds && alpha == o.alpha && max_iter == o.max_iter && tol == o.tol;
+    }
 };
 
 struct Session {
@@ -74,6 +87,13 @@ int64_t session_bytes(const egx_graph_info& inf) {
     return 2 * n * kSlots * 4 + n * 4 + n * 8 + (int64_t)1024 * kSlots * 4 +
            (int64_t)(inf.tiles + inf.hub_segments + 8192) * 3 * kSlots * 8 + ((int64_t)1 << 20);
 }
+
+// A finished column on its way to the callers' buffers.
+struct Delivery {
+    std::shared_ptr<Request> r;
+    double* staged = nullptr;  // pinned copy of the result (null: nobody asked for it)
+    hipEvent_t ev = nullptr;   // recorded after the copy into `staged`
+};
 
 }  // namespace
 
@@ -97,6 +117,15 @@ struct egx_server {
     egx_server_stats stats{};
     std::thread worker;
 
+    // result delivery (guarded by mu)
+    std::deque<Delivery> deliveries;
+    std::condition_variable cv_deliver;
+    bool deliver_stop = false;
+    std::vector<std::thread> deliverers;
+    std::vector<std::vector<double*>> staged_free;  // per graph: pinned buffers of n doubles
+    std::vector<double*> staged_all;
+    std::vector<hipEvent_t> events_free, events_all;
+
     ~egx_server() {
         {
             std::lock_guard<std::mutex> lk(mu);
@@ -104,17 +133,105 @@ struct egx_server {
         }
         cv_work.notify_all();
         if (worker.joinable()) worker.join();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            deliver_stop = true;
+        }
+        cv_deliver.notify_all();
+        for (auto& t : deliverers)
+            if (t.joinable()) t.join();  // every queued delivery is completed first
         for (Session& s : sessions)
             if (s.b) egx_batch_close(s.b);
+        for (hipEvent_t e : events_all) (void)hipEventDestroy(e);
+        for (double* p : staged_all) (void)hipHostFree(p);
     }
 
+    // A pinned buffer for one result of `graph` and an event, from the pools.
+    int take_staging(int32_t graph, double** buf, hipEvent_t* ev) {
+        *buf = nullptr;
+        *ev = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto& fl = staged_free[(size_t)graph];
+            if (!fl.empty()) {
+                *buf = fl.back();
+                fl.pop_back();
+            }
+            if (!events_free.empty()) {
+                *ev = events_free.back();
+                events_free.pop_back();
+            }
+        }
+        if (!*buf) {
+            void* p = nullptr;
+            const size_t bytes = (size_t)info[(size_t)graph].n * sizeof(double);
+            if (hipHostMalloc(&p, bytes) != hipSuccess) return EGX_FAILED_TO_ALLOCATE_HOST_MEM;
+            *buf = static_cast<double*>(p);
+            std::lock_guard<std::mutex> lk(mu);
+            staged_all.push_back(*buf);
+        }
+        if (!*ev) {
+            if (hipEventCreateWithFlags(ev, hipEventDisableTiming) != hipSuccess) {
+                give_back(graph, *buf, nullptr);
+                *buf = nullptr;
+                return EGX_DEVICE_ERR;
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            events_all.push_back(*ev);
+        }
+        return EGX_SUCC;
+    }
+
+    void give_back(int32_t graph, double* buf, hipEvent_t ev) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (buf) staged_free[(size_t)graph].push_back(buf);
+        if (ev) events_free.push_back(ev);
+    }
+
+    // Delivery thread: wait for a staged result, copy it to the request's
+    // buffer and to those of the requests riding on its column, complete them.
+    void deliver_loop() {
+        (void)hipSetDevice(device);
+        for (;;) {
+            Delivery d;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_deliver.wait(lk, [&] { return deliver_stop || !deliveries.empty(); });
+                if (deliveries.empty()) return;  // stopping, and nothing left to deliver
+                d = std::move(deliveries.front());
+                deliveries.pop_front();
+            }
+            int st = EGX_SUCC;
+            if (d.ev && hipEventSynchronize(d.ev) != hipSuccess) st = EGX_DEVICE_ERR;
+            if (st == EGX_SUCC && d.staged) {
+                const size_t bytes = (size_t)info[(size_t)d.r->graph].n * sizeof(double);
+                if (d.r->out) std::memcpy(d.r->out, d.staged, bytes);
+                for (auto& q : d.r->riders)
+                    if (q->out) std::memcpy(q->out, d.staged, bytes);
+            }
+            give_back(d.r->graph, d.staged, d.ev);
+            finish(d.r, st);
+        }
+    }
+
+    // Completes r and the requests riding on its column.
     void finish(const std::shared_ptr<Request>& r, int status) {
         std::lock_guard<std::mutex> lk(mu);
-        r->status = status;
-        r->t_done = Clock::now();
-        r->done = true;
-        stats.completed += 1;
-        if (status != EGX_SUCC) stats.failed += 1;
+        const Clock::time_point now = Clock::now();
+        auto one = [&](Request& q) {
+            q.status = status;
+            q.t_done = now;
+            q.done = true;
+            stats.completed += 1;
+            if (status != EGX_SUCC) stats.failed += 1;
+        };
+        one(*r);
+        for (auto& q : r->riders) {
+            q->iterations = r->iterations;
+            q->width = r->width;
+            one(*q);
+        }
+        r->riders.clear();
         cv_done.notify_all();
     }
 
@@ -186,6 +303,21 @@ struct egx_server {
                     continue;
                 }
             }
+            if (!opts.coalesce_off) {
+                std::shared_ptr<Request> host;
+                for (int32_t k = 0; k < kSlots && !host; ++k) {
+                    const auto& q = s->slot[(size_t)k];
+                    if (q && q->same_solve(*r)) host = q;
+                }
+                if (host) {
+                    r->t_start = Clock::now();
+                    host->riders.push_back(r);
+                    s->last_used = r->t_start;
+                    std::lock_guard<std::mutex> lk(mu);
+                    stats.coalesced += 1;
+                    continue;
+                }
+            }
             int32_t j = -1;
             for (int32_t k = 0; k < kSlots && j < 0; ++k)
                 if (!s->slot[(size_t)k]) j = k;
@@ -242,9 +374,28 @@ struct egx_server {
             if (!r || !done[j]) continue;
             r->iterations = iters[j];
             r->width = width;
-            int st = egx_batch_result(s.b, j, r->out);
+            bool wanted = r->out != nullptr;
+            for (auto& q : r->riders) wanted = wanted || q->out != nullptr;
+            Delivery d;
+            d.r = r;
+            int st = EGX_SUCC;
+            if (wanted) {
+                st = take_staging(s.graph, &d.staged, &d.ev);
+                if (st == EGX_SUCC) st = egx::batch_result_enqueue(s.b, j, d.staged, d.ev);
+            }
+            // stream order: the release applies after the result has been read
             if (st == EGX_SUCC) st = egx_batch_release(s.b, j);
-            finish(r, st);
+            if (st == EGX_SUCC) {
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    deliveries.push_back(std::move(d));
+                }
+                cv_deliver.notify_one();
+            } else {
+                if (d.ev) (void)hipEventSynchronize(d.ev);  // the copy may be queued: let it land first
+                give_back(s.graph, d.staged, d.ev);
+                finish(r, st);
+            }
             r = nullptr;
             s.busy -= 1;
             s.last_used = Clock::now();
@@ -337,6 +488,7 @@ int egx_server_open(egx_graph* const* pool, int32_t count, const egx_server_opts
         }
     }
     sv->device = pool[0]->g.device;
+    sv->staged_free.resize((size_t)count);
     if (opts) sv->opts = *opts;
     if (sv->opts.max_sessions <= 0) sv->opts.max_sessions = 64;
     if (sv->opts.step_iters <= 0) sv->opts.step_iters = 4;
@@ -358,10 +510,11 @@ int egx_server_open(egx_graph* const* pool, int32_t count, const egx_server_opts
         sv->stats.largest_session_bytes = std::max<int64_t>(sv->stats.largest_session_bytes, session_bytes(inf));
     egx_server* raw = sv.get();
     try {
+        for (int i = 0; i < kDeliveryThreads; ++i) sv->deliverers.emplace_back([raw] { raw->deliver_loop(); });
         sv->worker = std::thread([raw] { raw->run(); });
     } catch (...) {
-        egx::set_last_error("could not start the server thread");
-        return EGX_UNKNOW_ERROR;
+        egx::set_last_error("could not start the server threads");
+        return EGX_UNKNOW_ERROR;  // the destructor stops whatever started
     }
     *out = sv.release();
     return EGX_SUCC;

@@ -1250,6 +1250,30 @@ int run_until_done(egx::Batch& b, int32_t max_launches, int32_t check_every, dou
 
 }  // namespace
 
+int egx::batch_result_enqueue(egx_batch* bh, int32_t j, double* host_out, hipEvent_t done) {
+    using namespace egx;
+    if (!bh || !bh->b) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    Batch& b = *bh->b;
+    Graph& g = *b.g;
+    hipStream_t s = b.stream;
+    if (j < 0 || j >= b.width) {
+        set_last_error("batch slot holds no request");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    // b.out is reused by the next result on this stream only after this copy
+    k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[0].as<float>(), b.X[1].as<float>(), b.ctl.as<BatchCtl>(),
+                                                  b.width, b.cols.as<ColState>(),
+                                                  g.reordered ? g.perm.as<int32_t>() : nullptr, g.n, j,
+                                                  b.out.as<double>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    if (host_out)
+        EGX_HIP_TRY(hipMemcpyAsync(host_out, b.out.p, (size_t)g.n * 8, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    if (done) EGX_HIP_TRY(hipEventRecord(done, s), EGX_DEVICE_ERR);
+    return EGX_SUCC;
+}
+
 extern "C" {
 
 int egx_batch_open(egx_graph* gh, egx_batch** out) {

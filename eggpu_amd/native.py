@@ -128,7 +128,7 @@ class IngestReport(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_sessions", ctypes.c_int32),
-                ("step_iters", ctypes.c_int32)]
+                ("step_iters", ctypes.c_int32), ("coalesce_off", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -142,7 +142,7 @@ class RequestInfo(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in ("submitted", "completed", "failed", "steps", "sessions_opened",
                                               "sessions_closed", "peak_active_columns", "peak_session_bytes",
-                                              "budget_bytes", "largest_session_bytes")]
+                                              "budget_bytes", "largest_session_bytes", "coalesced")]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -546,13 +546,14 @@ class Batch:
 class Server:
     """Continuous-batching PageRank server over a pool of DeviceGraphs
     (egx_server_*).  submit() queues a request and returns its id; wait()
-    returns (ranks float64[n] or None, info dict)."""
+    returns (ranks float64[n] or None, info dict).  Identical requests in
+    flight together share one column unless coalesce=False."""
 
-    def __init__(self, graphs, hbm_budget_bytes=0, max_sessions=0, step_iters=0):
+    def __init__(self, graphs, hbm_budget_bytes=0, max_sessions=0, step_iters=0, coalesce=True):
         self._graphs = list(graphs)
         k = len(self._graphs)
         handles = (_vp * max(k, 1))(*[g.handle for g in self._graphs])
-        o = ServerOpts(int(hbm_budget_bytes), int(max_sessions), int(step_iters))
+        o = ServerOpts(int(hbm_budget_bytes), int(max_sessions), int(step_iters), 0 if coalesce else 1, 0)
         h = _vp()
         _check(lib().egx_server_open(handles, k, ctypes.byref(o), ctypes.byref(h)))
         self._h = h.value

@@ -308,7 +308,10 @@ int egx_probe_gather(int64_t working_set_floats, int64_t n_gathers, int32_t reps
 /* stream); queued requests join running sessions at iteration-step          */
 /* boundaries; sessions open while the HBM budget holds them and idle ones   */
 /* close least-recently-used first; busy sessions on different graphs run    */
-/* concurrently.  A result is bit-identical alone and in any mix.            */
+/* concurrently.  A result is bit-identical alone and in any mix.  A request */
+/* identical to one still in flight on the same graph (same seed list,       */
+/* alpha, max_iter and tol) shares that request's column and receives the    */
+/* same result (request coalescing; coalesce_off = 1 turns it off).          */
 /* ------------------------------------------------------------------------ */
 typedef struct egx_server egx_server; /* opaque */
 
@@ -316,6 +319,8 @@ typedef struct {
     int64_t hbm_budget_bytes; /* session memory the server may hold; <= 0: 80% of free HBM at open */
     int32_t max_sessions;     /* concurrently open sessions (streams); <= 0: 64 */
     int32_t step_iters;       /* iterations enqueued per session between polls; <= 0: 4 */
+    int32_t coalesce_off;     /* 1: every request takes a column of its own */
+    int32_t pad;
 } egx_server_opts;
 
 typedef struct {
@@ -337,6 +342,7 @@ typedef struct {
     int64_t peak_session_bytes;
     int64_t budget_bytes;
     int64_t largest_session_bytes; /* footprint charged for the pool's largest graph */
+    int64_t coalesced;           /* requests answered by a column they shared with an identical one */
 } egx_server_stats;
 
 /* The pool's graphs must stay alive (and not be re-binned) until close. */

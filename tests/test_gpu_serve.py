@@ -131,3 +131,23 @@ def test_budget_admission(pool):
         st = srv.stats()
     _say(f"one-session budget: opened {st['sessions_opened']}, closed {st['sessions_closed']}")
     assert st["sessions_closed"] >= 2 and st["peak_session_bytes"] <= one
+
+
+def test_identical_requests_share_a_column(pool):
+    graphs, _ = pool
+    cases = [(1, (), 0.85)] * 16 + [(0, (5, 9, 40), 0.9)] * 8
+    results = {}
+    for coalesce in (True, False):
+        with native.Server(graphs, coalesce=coalesce) as srv:
+            ids = [srv.submit(g, s, alpha=a, max_iter=100, tol=1e-6) for g, s, a in cases]
+            outs = [srv.wait(rid) for rid in ids]
+            st = srv.stats()
+        results[coalesce] = outs
+        _say(f"coalesce={coalesce}: coalesced {st['coalesced']}, completed {st['completed']}")
+        assert st["completed"] == len(cases) and st["failed"] == 0
+        if coalesce:
+            assert st["coalesced"] >= 1  # submitted back to back: later ones ride on the first column
+        else:
+            assert st["coalesced"] == 0
+    for (a, ia), (b, ib) in zip(results[True], results[False]):
+        assert np.array_equal(a, b) and ia["iterations"] == ib["iterations"]
