@@ -49,7 +49,9 @@ namespace egx {
 
 namespace {
 
-constexpr int kThreads = 256;
+// 512 threads over one LDS tile: 8 wavefronts per SIMD at FP32 (LDS-limited to
+// 5 with 256-thread blocks); the many small tiles of batched graphs gain most
+constexpr int kThreads = 512;
 constexpr int kWavesPerBlock = kThreads / kWave;
 constexpr int kReduceThreads = 256;  // per reduce-part workgroup
 constexpr int kRedItems = 512;       // hub rows + partial slots per reduce-part workgroup (target)
