@@ -112,6 +112,50 @@ def test_graph_out_csr_keeps_directed_self_loops_drops_undirected_ones():
     assert list(V) == [0, 1, 3, 4] and list(E) == [1, 0, 2, 1]
 
 
+@pytest.mark.parametrize("labels", ["identity", "shuffled", "floaty"])
+@pytest.mark.parametrize("weight", [None, "weight"])
+def test_graph_out_csr_integer_labels_and_unsorted_rows(labels, weight):
+    # nodes 0..n-1 in order take the no-lookup path; neighbours are inserted
+    # in descending order so every row needs sorting
+    rng = np.random.default_rng(11)
+    n = 40
+    nodes = list(range(n))
+    if labels == "shuffled":
+        rng.shuffle(nodes)
+    elif labels == "floaty":
+        nodes[3] = 3.5  # not an int: the lookup path
+    G = DiGraph()
+    G.add_nodes_from(nodes)
+    for u in nodes:
+        for v in sorted(rng.choice(n, size=int(rng.integers(0, 6)), replace=False), reverse=True):
+            G.add_edge(u, nodes[int(v)], weight=float(rng.uniform(0.5, 2)))
+    prep = _prep(G)
+    assert gpu_backend._identity_labels(prep) == (labels == "identity")
+    V, E, W = gpu_backend.graph_out_csr(G, prep, weight)
+    Vo, Eo, Wo, _ = O.csr_from_graph(G, weight)
+    assert np.array_equal(V, Vo) and np.array_equal(E, Eo)
+    if weight is not None:
+        assert np.array_equal(W, Wo)
+
+
+def test_graph_stamp_sees_changes_behind_the_generation_counter():
+    G = DiGraph()
+    G.add_edges_from([(0, 1), (1, 2), (2, 0), (2, 3)])
+    s0 = gpu_backend._graph_stamp(G)
+    H = DiGraph()
+    H.add_edges_from([(0, 1), (1, 2), (2, 0), (2, 3)])
+    assert gpu_backend._graph_stamp(H) == s0  # same graph, same stamp
+    G._adj[1][3] = {}  # an arc added without touching the generation counter
+    assert gpu_backend._graph_stamp(G) != s0
+    K = DiGraph()
+    K.add_edges_from([(0, 1), (1, 2), (2, 0), (2, 4)])  # same counts, different labels
+    assert gpu_backend._graph_stamp(K) != s0
+    P = DiGraph()
+    P.add_nodes_from([1, 0, 2, 3])  # same nodes and degrees, another order
+    P.add_edges_from([(0, 1), (1, 2), (2, 0), (2, 3)])
+    assert gpu_backend._graph_stamp(P) != s0
+
+
 def test_networkx_graph_is_accepted():
     G = nx.DiGraph()
     G.add_edges_from([(0, 1), (1, 2), (2, 0), (2, 3)])
