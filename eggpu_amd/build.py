@@ -24,6 +24,7 @@ SOURCES = [
     "egx_gen.hip",
     "egx_pagerank.hip",
     "egx_spmm.hip",
+    "egx_katz.hip",
     "egx_api.hip",
     "egx_serve.hip",
     "egx_ingest.hip",

@@ -109,7 +109,7 @@ __global__ void k_fill_csc(const uint32_t* __restrict__ vals, const int32_t* __r
     const uint32_t val = vals[q];
     if (weighted) {
         const int32_t u = src[val];
-        double w = W[val];
+        double w = W ? W[val] : 1.0;
         if (!isfinite(w) || w < 0.0) w = 0.0;
         const double denom = out_w[u];
         in_col[q] = u;
@@ -385,11 +385,17 @@ int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
     return build_tile_sort(g, cfg);
 }
 
+// out_w[u] = 1: adjacency values keep alpha * w undivided.
+__global__ void k_unit_out_w(double* __restrict__ out_w, int32_t n) {
+    const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) out_w[u] = 1.0;
+}
+
 int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const double* dW, int32_t n, int64_t m,
                           double alpha) {
     hipStream_t s = g.stream;
     g.n = n;
-    g.weighted = dW != nullptr;
+    g.weighted = dW != nullptr || g.adjacency;
     g.alpha_vals = alpha;
     g.alpha_orig = alpha;
     const auto t_start = std::chrono::steady_clock::now();
@@ -426,6 +432,10 @@ int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const 
         else
             k_row_out<<<nblocks(n, 64), 64, 0, s>>>(dV, dE, dW, n, g.out_w.as<double>(), outcnt_p);
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        if (g.adjacency) {
+            k_unit_out_w<<<nblocks(n, 256), 256, 0, s>>>(g.out_w.as<double>(), n);
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        }
         k_node_arrays<<<nblocks(n, 256), 256, 0, s>>>(g.out_w.as<double>(), outcnt_p, n,
                                                        g.weighted ? 1 : 0, g.deg.as<float>(), ctr.as<uint32_t>());
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);

@@ -92,6 +92,10 @@ struct Graph {
     int64_t m = 0;          // stored inbound arcs
     int64_t n_dangling = 0;
     bool weighted = false;
+    // adjacency values: every arc carries fl32(alpha * w) with unit weights
+    // and no division by the source's out-weight (Katz: x' = alpha*A*x + b);
+    // set before build_from_device_csr
+    bool adjacency = false;
     double alpha_vals = 0.0;   // alpha baked into it_val
     double alpha_orig = 0.0;   // alpha baked into in_val
     int32_t max_in_degree = 0;
@@ -143,7 +147,7 @@ struct Graph {
     size_t device_bytes() const;
 };
 
-// egx_build.hip
+// egx_build.hip.  With g.adjacency set, dW may be null (unit weights).
 int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const double* dW, int32_t n,
                           int64_t m, double alpha);
 int build_layout(Graph& g, const BinCfg& cfg);
@@ -168,6 +172,11 @@ struct SolveParams {
     bool time_each_launch = false;  // hipEvent pair around every launch
     bool use_graph = false;         // replay captured runs of check_every iterations
     bool xcd_placement = false;     // multi-graph: keep a small graph's work items on one XCD
+    // Katz mode (katz_centrality.py:81-110): x0 = 1, the teleport and
+    // dangling terms are replaced by bias[row] (device, iteration-row order),
+    // the stop test is sum|x' - x| < tol (not tol*n), and the result is the
+    // iterate before the last update, unnormalised.
+    const double* bias = nullptr;
 };
 struct SolveStats {
     double kernel_seconds = 0.0;  // first to last iteration launch

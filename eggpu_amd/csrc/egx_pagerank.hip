@@ -89,6 +89,7 @@ struct IterArgs {
     uint32_t grid;
     uint32_t n_red;    // reduce-part workgroups
     int32_t cold_from; // first source label gathered non-temporally (n: none)
+    const double* bias;  // per-row constant term replacing the teleport (Katz); null: PageRank
     double alpha;
     double tol_n;
     int32_t max_iter;
@@ -118,6 +119,7 @@ __device__ __forceinline__ T term(const IterArgs<T>& a, int32_t u, float w) {
 template <typename T, bool W>
 __device__ __forceinline__ void finalize_row(const IterArgs<T>& a, int32_t v, double acc, double jump, double& err,
                                              double& sum, double& dang) {
+    if (a.bias) jump = a.bias[v];  // uniform branch: a kernel argument
     const double xd = W ? (acc + jump) : (a.alpha * acc + jump);
     const T xn = (T)xd;
     const T xo = a.x_in[v];
@@ -358,12 +360,13 @@ __global__ __launch_bounds__(kWave) void k_pr_final_multi(const IterArgs<T>* __r
     reduce_final_wave<T>(a);
 }
 
-// x0 = 1/n (as T), s0 = x0/deg.
+// x0 = init (1/n for PageRank, 1 for Katz; as T), s0 = x0/deg.
 template <typename T>
-__global__ void k_pr_init(const float* __restrict__ deg, int32_t n, T* __restrict__ x, T* __restrict__ s) {
+__global__ void k_pr_init(const float* __restrict__ deg, int32_t n, double init, T* __restrict__ x,
+                          T* __restrict__ s) {
     const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= n) return;
-    const T x0 = (T)(1.0 / (double)n);
+    const T x0 = (T)init;
     x[v] = x0;
     const float d = deg[v];
     s[v] = (d > 0.0f) ? (T)(x0 / (T)d) : (T)0;
@@ -380,6 +383,18 @@ __global__ void k_pr_output(const T* __restrict__ x0, const T* __restrict__ x1, 
     const T* x = (st->iter & 1) ? x1 : x0;
     const int32_t v = perm ? perm[r] : r;
     out[v] = (norm > 0.0) ? (double)x[r] * (1.0 / norm) : 1.0 / (double)n;
+}
+
+// Katz result: out[perm[r]] = the iterate before the last update
+// (katz_centrality.py:95-99 keeps x when the test on x_new passes), in FP64.
+template <typename T>
+__global__ void k_katz_output(const T* __restrict__ x0, const T* __restrict__ x1, const PrState* __restrict__ st,
+                              const int32_t* __restrict__ perm, int32_t n, double* __restrict__ out) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const T* x = (st->iter & 1) ? x0 : x1;
+    const int32_t v = perm ? perm[r] : r;
+    out[v] = (double)x[r];
 }
 
 inline unsigned nblk(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
@@ -441,7 +456,8 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
     EGX_TRY(refresh_alpha(g, p.alpha));
     EGX_TRY(ensure_workspace(g, grid));
     const int32_t n = g.n;
-    k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, g.x[0].as<T>(), g.s[0].as<T>());
+    k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, p.bias ? 1.0 : 1.0 / (double)n, g.x[0].as<T>(),
+                                              g.s[0].as<T>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     // staged through the pinned state buffer: the copy may run after return
     PrState* st0 = static_cast<PrState*>(g.host_state);
@@ -477,18 +493,25 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
     a.nblk_h = (uint32_t)g.n_hseg;
     a.grid = grid;
     a.n_red = (uint32_t)std::min<int64_t>(kMaxRed, std::max<int64_t>(1, (g.n_hrows + grid + kRedItems - 1) / kRedItems));
+    a.bias = p.bias;
     a.alpha = p.alpha;
-    a.tol_n = std::max(0.0, p.tol) * (double)n;
+    a.tol_n = std::max(0.0, p.tol) * (p.bias ? 1.0 : (double)n);  // Katz: absolute L1 test
     a.max_iter = p.max_iter;
     return EGX_SUCC;
 }
 
-// Writes the normalised result of graph g (and copies it to host_out).
+// Writes the result of graph g (and copies it to host_out): PageRank's
+// normalised iterate, or Katz's unnormalised previous iterate.
 template <typename T>
-int emit_result(Graph& g, hipStream_t s, double* host_out) {
-    k_pr_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
-                                                   g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
-                                                   g.out.as<double>());
+int emit_result(Graph& g, hipStream_t s, double* host_out, bool katz = false) {
+    if (katz)
+        k_katz_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
+                                                         g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
+                                                         g.out.as<double>());
+    else
+        k_pr_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
+                                                       g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
+                                                       g.out.as<double>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     if (host_out)
         EGX_HIP_TRY(hipMemcpyAsync(host_out, g.out.p, (size_t)g.n * 8, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
@@ -572,7 +595,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
             st->spmv_launches = iters;
         }
     }
-    EGX_TRY(emit_result<T>(g, s, host_out));
+    EGX_TRY(emit_result<T>(g, s, host_out, p.bias != nullptr));
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
@@ -762,10 +785,12 @@ int solve_pagerank(Graph& g, const SolveParams& p_in, double* host_out, SolveSta
             for (int32_t i = 0; i < g.n; ++i) host_out[i] = 1.0 / (double)g.n;
         return EGX_SUCC;
     }
+    // adjacency graphs have out-weight 1 everywhere, so the unweighted path
+    // (s = x/1, x' = alpha * row sum + b) applies alpha in FP64 to each sum
+    const bool w = g.weighted && !g.adjacency;
     if (p.precision == EGX_PRECISION_FP64)
-        return g.weighted ? solve_typed<double, true>(g, p, host_out, st)
-                          : solve_typed<double, false>(g, p, host_out, st);
-    return g.weighted ? solve_typed<float, true>(g, p, host_out, st) : solve_typed<float, false>(g, p, host_out, st);
+        return w ? solve_typed<double, true>(g, p, host_out, st) : solve_typed<double, false>(g, p, host_out, st);
+    return w ? solve_typed<float, true>(g, p, host_out, st) : solve_typed<float, false>(g, p, host_out, st);
 }
 
 int solve_pagerank_multi(std::vector<Graph*>& gs, const SolveParams& p_in, double* const* host_out, int32_t* iters,

@@ -391,6 +391,32 @@ def _identity_labels(prepared):
     return flag
 
 
+def adjacency_csr(G, prepared):
+    """(V, E) of the adjacency as Katz centrality reads it
+    (``katz_centrality.py:73-81``): row u lists every adjacency key of u in
+    adjacency order, self-loops included, both directions of an undirected
+    edge; no sorting and no weights."""
+    if _is_bulk_graph(G):
+        V, E, _ = G.csr_arrays(None)
+        return V, E
+    idx = prepared["node_to_idx"]
+    nodes = prepared["nodes"]
+    adj = _adj_dict(G)
+    identity = _identity_labels(prepared)
+    get = idx.__getitem__
+    counts, dst = [], []
+    for u in nodes:
+        nbrs = adj[u]
+        counts.append(len(nbrs))
+        dst.extend(nbrs if identity else map(get, nbrs))
+    V = np.zeros(len(nodes) + 1, dtype=np.int64)
+    if counts:
+        V[1:] = np.cumsum(np.asarray(counts, dtype=np.int64))
+    if V[-1] >= 2**31 - 1:
+        raise ValueError("adjacency entry count exceeds the signed 32-bit EGGPU CSR limit")
+    return V.astype(np.int32), np.fromiter(dst, dtype=np.int64, count=len(dst)).astype(np.int32)
+
+
 def graph_out_csr(G, prepared, weight=None):
     """Out-CSR of a protocol graph in the reference's order.
 

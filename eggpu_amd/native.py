@@ -221,6 +221,9 @@ SIGNATURES = {
     "egx_batch_close": (None, [_vp]),
     "egx_graph_ppr": (ctypes.c_int, [_vp, ctypes.c_int32, _i32p, _i32p, _f64p, ctypes.c_double, ctypes.c_int32,
                                      ctypes.c_double, _f64p, _i32p, _f64p]),
+    "egx_katz_csr": (ctypes.c_int, [_i32p, _i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _f64p,
+                                    ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _f64p, _i32p,
+                                    _i32p, _f64p]),
     "egx_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _f64p]),
     "egx_probe_gather": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _f64p]),
 }
@@ -633,6 +636,30 @@ def pagerank_csr(V, E, W=None, alpha=0.85, max_iter=200, tol=1e-6, graph_id=0):
                               int(max_iter), float(tol), int(graph_id) & (2**64 - 1), _ptr(out, _f64p),
                               ctypes.byref(it), ctypes.byref(ks)))
     return out, int(it.value), float(ks.value)
+
+
+def katz_csr(V, E, alpha=0.1, beta=1.0, max_iter=1000, tol=1e-6, normalized=True):
+    """``egx_katz_csr`` on an out-CSR: (float64[n], iterations, converged,
+    kernel_seconds).  beta is a scalar or n values in node order."""
+    V = _i32(V)
+    E = _i32(E)
+    n = len(V) - 1
+    vec = None
+    scalar = 1.0
+    if np.ndim(beta) == 0:
+        scalar = float(beta)
+    else:
+        vec = np.ascontiguousarray(beta, dtype=np.float64)
+        if len(vec) != n:
+            raise ValueError(f"beta has {len(vec)} values for {n} nodes")
+    out = np.empty(max(n, 0), dtype=np.float64)
+    it = ctypes.c_int32(0)
+    conv = ctypes.c_int32(0)
+    ks = ctypes.c_double(0.0)
+    _check(lib().egx_katz_csr(_ptr(V, _i32p), _ptr(E, _i32p), n, len(E), float(alpha), _ptr(vec, _f64p), scalar,
+                              int(max_iter), float(tol), 1 if normalized else 0, _ptr(out, _f64p),
+                              ctypes.byref(it), ctypes.byref(conv), ctypes.byref(ks)))
+    return out, int(it.value), bool(conv.value), float(ks.value)
 
 
 def cache_stats():

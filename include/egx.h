@@ -301,6 +301,22 @@ int egx_probe_copy(int64_t bytes, int32_t reps, double *seconds_per_copy);
 int egx_probe_gather(int64_t working_set_floats, int64_t n_gathers, int32_t reps, double *seconds_per_pass);
 
 /* ------------------------------------------------------------------------ */
+/* Katz centrality on the same SpMV engine.  No counterpart on the reference's */
+/* GPU path; it restates the CPU function katz_centrality.py:7-110 (the     */
+/* Python route of its @hybrid dispatch): A[u][v] = 1 for every arc u->v of */
+/* the CSR (V, E), x = 1, x_new = alpha*A*x + b with b = beta[v] (beta may  */
+/* be NULL: beta_scalar everywhere); the loop stops when sum|x_new - x| <   */
+/* tol and keeps the iterate before that update; normalized != 0 divides by */
+/* the L2 norm.  FP64 iterates and sums, alpha applied to each row sum.  *converged */
+/* = 0 when max_iter passes without the test holding (the reference raises  */
+/* RuntimeError); out (n doubles) is then unspecified.  Returns EGX_SUCC    */
+/* whenever the iteration ran.                                              */
+/* ------------------------------------------------------------------------ */
+int egx_katz_csr(const int32_t *V, const int32_t *E, int32_t n, int32_t m, double alpha, const double *beta,
+                 double beta_scalar, int32_t max_iter, double tol, int32_t normalized, double *out,
+                 int32_t *iterations, int32_t *converged, double *kernel_seconds);
+
+/* ------------------------------------------------------------------------ */
 /* Serving (no counterpart in the reference, whose pagerank.py:141-179 solves */
 /* one graph per call): a worker thread answers global and personalised      */
 /* PageRank requests against a pool of device-resident graphs by continuous  */

@@ -21,6 +21,8 @@ Python-side helpers around ``egx_oracle.c``:
   (``graph_convert.cpp:110-134``).
 * :func:`rmat_csr` / :func:`er_csr` are this project's seeded generators,
   the definitions the HIP generator must reproduce bit for bit.
+* :func:`katz_rows` / :func:`katz_graph` restate the CPU Katz centrality
+  ``katz_centrality.py:7-110`` in FP64 with a sparse adjacency.
 
 Functions with no independent check are marked "parity unpinned" in their
 docstring; every other function is pinned in ``tests/test_oracle.py``.
@@ -370,3 +372,54 @@ def rmat_n_csr(n, nedges, seed=1, a=0.57, b=0.19, c=0.19):
 def er_csr(n, nedges, seed):
     src, dst = er_coo(n, nedges, seed)
     return simple_csr_from_coo(n, src, dst)
+
+
+def adjacency_csr_from_graph(G):
+    """A of ``katz_centrality.py:73-81``: node order ``list(G.nodes)``, one
+    entry A[u][v] = 1 per adjacency key v of u (self-loops included, both
+    directions of an undirected edge).  Returns (n, src, dst) index arrays."""
+    nodes = list(G.nodes)
+    idx = {node: i for i, node in enumerate(nodes)}
+    src, dst = [], []
+    for u in nodes:
+        for v in G.adj[u]:
+            src.append(idx[u])
+            dst.append(idx[v])
+    return len(nodes), np.asarray(src, dtype=np.int64), np.asarray(dst, dtype=np.int64)
+
+
+def katz_rows(n, src, dst, alpha=0.1, beta=1.0, max_iter=1000, tol=1e-6, normalized=True):
+    """``katz_centrality.py:83-110`` in FP64: x = 1; x_new = alpha*A@x + b;
+    stop when ||x_new - x||_1 < tol keeping the previous x; L2-normalise.
+    beta is a scalar or an array of n values.  Returns (x, iterations) or
+    (None, max_iter) when the loop ends without the test holding (the
+    reference raises RuntimeError)."""
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((np.ones(len(src)), (np.asarray(src), np.asarray(dst))), shape=(n, n))
+    x = np.ones(n, dtype=np.float64)
+    b = np.asarray(beta, dtype=np.float64) * np.ones(n) if np.ndim(beta) else np.ones(n) * float(beta)
+    for it in range(max_iter):
+        x_new = alpha * (A @ x) + b
+        if np.linalg.norm(x_new - x, ord=1) < tol:
+            break
+        x = x_new
+    else:
+        return None, max_iter
+    if normalized:
+        norm = np.linalg.norm(x)
+        if norm > 0:
+            x = x / norm
+    return x, it + 1
+
+
+def katz_graph(G, alpha=0.1, beta=1.0, max_iter=1000, tol=1e-6, normalized=True):
+    """:func:`katz_rows` on a graph; beta may be a dict node -> value
+    (default 1.0).  Returns ({node: value} or None, iterations)."""
+    nodes = list(G.nodes)
+    n, src, dst = adjacency_csr_from_graph(G)
+    b = np.array([beta.get(v, 1.0) for v in nodes]) if isinstance(beta, dict) else beta
+    x, it = katz_rows(n, src, dst, alpha, b, max_iter, tol, normalized)
+    if x is None:
+        return None, it
+    return {v: float(x[i]) for i, v in enumerate(nodes)}, it

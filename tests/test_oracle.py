@@ -315,3 +315,69 @@ def test_hetero_workload_specs():
     pool = workloads.serving_pool_specs(32)
     assert len(pool) == 32 and any(s.weighted for s in pool) and not all(s.weighted for s in pool)
     assert min(s.nedges for s in pool) >= 1e4 and max(s.nedges for s in pool) <= 5e7
+
+
+# -- Katz centrality (katz_centrality.py:7-110) --------------------------------
+@pytest.mark.parametrize("directed", [False, True])
+def test_katz_matches_networkx_at_convergence(directed):
+    # networkx sums in-arcs (A^T x), starts from 0 and keeps the last update;
+    # at a tight tolerance both reach the fixed point, so the reference's
+    # out-arc sum equals networkx on the reversed graph.  (The docstring
+    # example of katz_centrality.py:64-70, 0.370/0.447, is not unit-norm and
+    # is not what the function computes: 0.4877/0.5120 here and in networkx.)
+    import networkx as nx
+
+    V, E = O.rmat_csr(7, 6, seed=3)
+    n = len(V) - 1
+    src, dst = O.csr_rows(V, E)
+    H = nx.DiGraph() if directed else nx.Graph()
+    H.add_nodes_from(range(n))
+    H.add_edges_from(zip(src.tolist(), dst.tolist()))
+    G = DiGraph() if directed else Graph()
+    G.add_nodes_from(range(n))
+    G.add_edges_from(zip(src.tolist(), dst.tolist()))
+    x, _ = O.katz_graph(G, alpha=0.02, tol=1e-13)
+    ref = nx.katz_centrality(H.reverse() if directed else H, alpha=0.02, tol=1e-15, max_iter=5000)
+    assert max(abs(x[v] - ref[v]) for v in range(n)) < 1e-11
+    path = Graph()
+    path.add_edges_from([(0, 1), (1, 2), (2, 3)])
+    xp, _ = O.katz_graph(path, alpha=0.05, tol=1e-13)
+    rp = nx.katz_centrality(nx.path_graph(4), alpha=0.05, tol=1e-15, max_iter=5000)
+    assert max(abs(xp[v] - rp[v]) for v in range(4)) < 1e-11 and abs(xp[0] - 0.48766) < 1e-5
+
+
+def test_katz_keeps_the_iterate_before_the_passing_update_and_reads_out_arcs():
+    # directed star 0 -> 1..5, alpha 0.1, beta 2: x1 = alpha*outdeg + beta =
+    # (2.5, 2, ..., 2), |x1 - x0|_1 = 6.5; x2 = (3.0, 2, ..., 2), |x2 - x1|_1 = 0.5.
+    # tol 1.0 passes on the second update and returns x1, not x2; in-arcs
+    # would give (2, 2.2, ...) instead.
+    G = DiGraph()
+    G.add_edges_from([(0, k) for k in range(1, 6)])
+    x, it = O.katz_graph(G, alpha=0.1, beta=2.0, tol=1.0, normalized=False)
+    assert it == 2
+    assert [x[v] for v in range(6)] == [2.5, 2.0, 2.0, 2.0, 2.0, 2.0]
+    # a test that passes at once returns x0 = 1, normalised to 1/sqrt(n)
+    x, it = O.katz_graph(G, alpha=0.1, beta=2.0, tol=1e9)
+    assert it == 1 and np.allclose([x[v] for v in range(6)], 1.0 / np.sqrt(6.0), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_katz_fixed_point_closed_form(seed):
+    # converged Katz solves (I - alpha A) x = b; beta varies per node
+    V, E = O.rmat_csr(8, 8, seed=seed)
+    n = len(V) - 1
+    src, dst = O.csr_rows(V, E)
+    A = sp.csr_matrix((np.ones(len(src)), (src, dst)), shape=(n, n))
+    alpha = 0.5 / max(1.0, float(abs(sp.linalg.eigs(A.astype(float), k=1, return_eigenvectors=False)[0])))
+    b = 1.0 + np.random.default_rng(seed).random(n)
+    x, it = O.katz_rows(n, src, dst, alpha, b, max_iter=2000, tol=1e-13, normalized=False)
+    exact = sp.linalg.spsolve((sp.identity(n) - alpha * A).tocsc(), b)
+    assert it > 3 and np.max(np.abs(x - exact)) < 1e-11
+
+
+def test_katz_not_converging_is_reported():
+    # a directed cycle with alpha 1 grows without bound; max_iter 0 never tests
+    G = DiGraph()
+    G.add_edges_from([(0, 1), (1, 2), (2, 0)])
+    assert O.katz_graph(G, alpha=1.0, max_iter=50)[0] is None
+    assert O.katz_graph(G, alpha=0.1, max_iter=0)[0] is None
