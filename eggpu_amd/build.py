@@ -1,8 +1,8 @@
 """Compile the HIP sources of eggpu_amd into ``eggpu_amd/libegx.so`` (gfx950).
 
-Run ``python -m eggpu_amd.build`` (or ``__graft_entry__.build()``).  Every
-call recompiles; objects go to ``eggpu_amd/_build/`` and the library is
-written in-tree so that it travels with the source tree.
+Run ``python -m eggpu_amd.build``.  Every call recompiles; objects go to
+``eggpu_amd/_build/`` and the library is written in-tree, beside the
+package that loads it.
 """
 from __future__ import annotations
 
