@@ -460,6 +460,33 @@ int egx_graph_pagerank_ex(egx_graph* gh, const egx_solve_opts* opts, double* PR,
     return rc;
 }
 
+int egx_graph_ppr_single(egx_graph* gh, const int32_t* seeds, int32_t nseeds, const egx_solve_opts* opts,
+                         double* PR, egx_solve_stats* stats, int32_t flags) {
+    if (!gh || !opts || nseeds < 1 || !seeds) {
+        egx::set_last_error("egx_graph_ppr_single needs a graph, options and at least one seed");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    Graph& g = gh->g;
+    std::lock_guard<std::mutex> lk(g.mu);
+    egx::DeviceGuard dg(g.device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    std::vector<int32_t> uniq(seeds, seeds + nseeds);
+    for (int32_t v : uniq)
+        if (v < 0 || v >= g.n) {
+            egx::set_last_error("seed node out of range");
+            return EGX_UNSUPPORTED_GRAPH;
+        }
+    std::sort(uniq.begin(), uniq.end());
+    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    EGX_TRY(egx::ensure_tele(g, uniq));
+    egx::SolveParams p = solve_params(opts, flags);
+    p.tele = g.tele.as<double>();
+    egx::SolveStats st;
+    int rc = after_launches(egx::solve_pagerank(g, p, PR, &st));
+    copy_stats(st, stats);
+    return rc;
+}
+
 int egx_multi_pagerank(egx_graph* const* graphs, int32_t count, const egx_solve_opts* opts, double* const* PR,
                        int32_t* iterations, egx_solve_stats* stats, int32_t flags) {
     if (!opts || count < 0 || (count > 0 && !graphs)) {

@@ -114,6 +114,7 @@ struct Graph {
     DevBuf perm, rank;
     DevBuf p_row, p_col, p_val, p_w, p_outw, p_deg;
     DevBuf p_nval;                // w/outW per arc (no alpha), batched solves
+    DevBuf tele;                  // personalised teleport weights (n doubles), single-vector solves
     int32_t open_batches = 0;     // batch sessions reading this graph
     const int32_t* it_row = nullptr;
     const int32_t* it_col = nullptr;
@@ -177,6 +178,10 @@ struct SolveParams {
     // the stop test is sum|x' - x| < tol (not tol*n), and the result is the
     // iterate before the last update, unnormalised.
     const double* bias = nullptr;
+    // personalised PageRank (networkx 3.4.2 semantics): per-row teleport
+    // weights (device, iteration-row order, summing to 1) that receive both
+    // the teleport and the dangling mass; null: uniform 1/n
+    const double* tele = nullptr;
 };
 struct SolveStats {
     double kernel_seconds = 0.0;  // first to last iteration launch
@@ -187,6 +192,8 @@ struct SolveStats {
     int64_t spmv_grid = 0;
 };
 int solve_pagerank(Graph& g, const SolveParams& p, double* host_out, SolveStats* stats);
+// g.tele := 1/k on the iteration rows of the k (distinct, in-range) seeds, 0 elsewhere.
+int ensure_tele(Graph& g, const std::vector<int32_t>& seeds);
 // Many graphs (same device, all weighted or all unweighted) in one launch per
 // iteration on stream s; each graph stops on its own.
 int solve_pagerank_multi(std::vector<Graph*>& gs, const SolveParams& p, double* const* host_out, int32_t* iters,

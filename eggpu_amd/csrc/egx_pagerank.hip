@@ -90,6 +90,7 @@ struct IterArgs {
     uint32_t n_red;    // reduce-part workgroups
     int32_t cold_from; // first source label gathered non-temporally (n: none)
     const double* bias;  // per-row constant term replacing the teleport (Katz); null: PageRank
+    const double* tele;  // per-row teleport weights summing to 1 (personalised); null: 1/n
     double alpha;
     double tol_n;
     int32_t max_iter;
@@ -119,7 +120,10 @@ __device__ __forceinline__ T term(const IterArgs<T>& a, int32_t u, float w) {
 template <typename T, bool W>
 __device__ __forceinline__ void finalize_row(const IterArgs<T>& a, int32_t v, double acc, double jump, double& err,
                                              double& sum, double& dang) {
-    if (a.bias) jump = a.bias[v];  // uniform branch: a kernel argument
+    // uniform branches (kernel arguments): Katz bias, or personalised teleport
+    // where jump holds (1-alpha) + alpha*dang
+    if (a.bias) jump = a.bias[v];
+    else if (a.tele) jump *= a.tele[v];
     const double xd = W ? (acc + jump) : (a.alpha * acc + jump);
     const T xn = (T)xd;
     const T xo = a.x_in[v];
@@ -131,10 +135,14 @@ __device__ __forceinline__ void finalize_row(const IterArgs<T>& a, int32_t v, do
     if (!W) a.s_out[v] = (d > 0.0f) ? (T)(xn / (T)d) : (T)0;
 }
 
+// (1-alpha)/n + alpha*dang/n (pagerank.py:125); with a personalised teleport
+// p the teleport and the dangling mass both go to p (networkx 3.4.2), and
+// this returns (1-alpha) + alpha*dang, scaled by p[v] per row.
 template <typename T>
 __device__ __forceinline__ double jump_of(const IterArgs<T>& a) {
+    if (a.tele) return (1.0 - a.alpha) + a.alpha * a.state->dang;
     const double n_d = (double)a.n;
-    return (1.0 - a.alpha) / n_d + a.alpha * a.state->dang / n_d;  // pagerank.py:125
+    return (1.0 - a.alpha) / n_d + a.alpha * a.state->dang / n_d;
 }
 
 // Gather the terms of a tile's arcs [a0, a0+na) into lds[] in row order.
@@ -494,6 +502,7 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
     a.grid = grid;
     a.n_red = (uint32_t)std::min<int64_t>(kMaxRed, std::max<int64_t>(1, (g.n_hrows + grid + kRedItems - 1) / kRedItems));
     a.bias = p.bias;
+    a.tele = p.tele;
     a.alpha = p.alpha;
     a.tol_n = std::max(0.0, p.tol) * (p.bias ? 1.0 : (double)n);  // Katz: absolute L1 test
     a.max_iter = p.max_iter;
@@ -771,7 +780,33 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     return EGX_SUCC;
 }
 
+// tele[rank[v]] = 1/k for the k seeds v (caller ids), 0 elsewhere.
+__global__ void k_tele_seeds(const int32_t* __restrict__ seeds, int32_t k, const int32_t* __restrict__ rank,
+                             double w, double* __restrict__ tele) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    const int32_t v = seeds[i];
+    tele[rank ? rank[v] : v] = w;
+}
+
 }  // namespace
+
+int ensure_tele(Graph& g, const std::vector<int32_t>& seeds) {
+    hipStream_t s = g.stream;
+    const size_t nb = (size_t)std::max(g.n, 1) * 8;
+    if (g.tele.bytes < nb) EGX_TRY(g.tele.alloc(nb));
+    EGX_HIP_TRY(hipMemsetAsync(g.tele.p, 0, nb, s), EGX_DEVICE_ERR);
+    const int32_t k = (int32_t)seeds.size();
+    if (k == 0) return EGX_SUCC;
+    DevBuf d;
+    EGX_TRY(d.alloc((size_t)k * 4));
+    EGX_HIP_TRY(hipMemcpyAsync(d.p, seeds.data(), (size_t)k * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    k_tele_seeds<<<nblk(k, 256), 256, 0, s>>>(d.as<int32_t>(), k, g.reordered ? g.rank.as<int32_t>() : nullptr,
+                                              1.0 / (double)k, g.tele.as<double>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);  // d is freed on return
+    return EGX_SUCC;
+}
 
 int solve_pagerank(Graph& g, const SolveParams& p_in, double* host_out, SolveStats* st) {
     SolveParams p = p_in;

@@ -199,6 +199,8 @@ SIGNATURES = {
     "egx_graph_pagerank": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, _i32p, _f64p]),
     "egx_graph_pagerank_ex": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, ctypes.POINTER(SolveStats),
                                              ctypes.c_int32]),
+    "egx_graph_ppr_single": (ctypes.c_int, [_vp, _i32p, ctypes.c_int32, ctypes.POINTER(SolveOpts), _f64p,
+                                            ctypes.POINTER(SolveStats), ctypes.c_int32]),
     "egx_multi_pagerank": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(SolveOpts),
                                           ctypes.POINTER(_f64p), _i32p, ctypes.POINTER(SolveStats), ctypes.c_int32]),
     "egx_graph_free": (None, [_vp]),
@@ -432,6 +434,19 @@ class DeviceGraph:
         _check(lib().egx_graph_pagerank_ex(self.handle, ctypes.byref(o), _ptr(out, _f64p), ctypes.byref(st),
                                            flags))
         return out, st.as_dict()
+
+    def ppr_single(self, seeds, alpha=0.85, max_iter=200, tol=1e-6, precision=PRECISION_FP32, download=True):
+        """One personalised vector on the plain SpMV (egx_graph_ppr_single):
+        teleport and dangling mass uniform over the distinct seeds.
+        Returns (float64[n] or None, iterations, kernel_seconds)."""
+        seeds = np.ascontiguousarray(seeds, dtype=np.int32)
+        out = np.empty(self.n, dtype=np.float64) if download else None
+        st = SolveStats()
+        o = SolveOpts(float(alpha), int(max_iter), float(tol), int(precision), 0)
+        _check(lib().egx_graph_ppr_single(self.handle, _ptr(seeds, _i32p), len(seeds), ctypes.byref(o),
+                                          _ptr(out, _f64p), ctypes.byref(st), 0))
+        d = st.as_dict()
+        return out, int(d["iterations"]), float(d["kernel_seconds"])
 
     def ppr(self, seed_sets, alpha=0.85, max_iter=200, tol=1e-6, alphas=None, download=True):
         """Batched PageRank: one column per entry of ``seed_sets`` (a list of

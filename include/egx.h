@@ -228,6 +228,15 @@ typedef struct {
 int egx_graph_pagerank_ex(egx_graph *g, const egx_solve_opts *opts, double *PR, egx_solve_stats *stats,
                           int32_t flags);
 
+/* One personalised PageRank vector on the plain SpMV (no counterpart in the
+ * reference, which has no personalisation, pagerank.py:14): the teleport and
+ * the dangling mass go uniformly to the distinct seeds (networkx 3.4.2
+ * semantics, as egx_graph_ppr), x0 = 1/n, stop when sum|x' - x| < tol*n,
+ * result sum-normalised.  The single-vector form of egx_graph_ppr, with the
+ * gather cost of one FP32 vector per arc. */
+int egx_graph_ppr_single(egx_graph *g, const int32_t *seeds, int32_t nseeds, const egx_solve_opts *opts,
+                         double *PR, egx_solve_stats *stats, int32_t flags);
+
 /* Global PageRank on many device-resident graphs of one device at once (no
  * counterpart in the reference, which solves one graph per call,
  * pagerank.cu:591-664).  All graphs must be weighted or all unweighted; each

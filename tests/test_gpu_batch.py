@@ -201,3 +201,39 @@ def test_scale20_ppr_properties(gpu_ready):
     assert (it == 50).all()
     assert np.abs(pr.sum(axis=1) - 1.0).max() < 1e-9
     assert (pr >= 0).all()
+
+
+def test_single_vector_ppr_matches_networkx_fixture(gpu_ready):
+    # the plain-SpMV form of one personalised vector (egx_graph_ppr_single)
+    z = np.load(os.path.join(GOLDEN, "ppr_rmat10.npz"))
+    V, E, W = z["V"], z["E"], z["W"]
+    sets = [[int(s) for s in S if s >= 0] for S in z["seed_sets"]]
+    g = native.DeviceGraph.from_csr(V, E, W)
+    for j, S in enumerate(sets):
+        out, it, _ = g.ppr_single(S, alpha=0.85, max_iter=200, tol=1e-6)
+        ref = z["nx_values"][j] / z["nx_values"][j].sum()
+        _close(out, ref, f"single-vector nx ppr set {j}")
+        assert it == int(z["nx_iterations"][j])
+    g.close()
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("precision", [native.PRECISION_FP32, native.PRECISION_FP64])
+def test_single_vector_ppr_vs_oracle_and_batch(gpu_ready, weighted, precision):
+    V, E = O.rmat_csr(12, 16, seed=33)
+    W = O.edge_weights(V, E, 9) if weighted else None
+    n = len(V) - 1
+    src, dst = O.csr_rows(V, E)
+    g = native.DeviceGraph.from_csr(V, E, W)
+    rng = np.random.default_rng(6)
+    for k in (1, 3, 16):
+        S = sorted(rng.choice(n, k, replace=False).tolist())
+        for alpha in (0.85, 0.9):
+            out, it, _ = g.ppr_single(S + S[:1], alpha=alpha, tol=1e-6, precision=precision)  # a repeated seed counts once
+            ref, it_ref, _ = O.ppr_rows(n, src, dst, W, _p(n, S), alpha, 200, 1e-6)
+            _close(out, ref, f"single ppr k={k} alpha={alpha} weighted={weighted} fp64={precision}")
+            assert it == it_ref
+            col, it_col, _ = g.ppr([S], alpha=alpha, tol=1e-6)
+            _close(out, col[0], f"single vs batch column k={k}")
+            assert it == int(it_col[0])
+    g.close()
