@@ -244,7 +244,7 @@ size_t Graph::device_bytes() const {
                             &csr_E,  &csr_W,  &perm,   &rank,   &p_row,  &p_col,   &p_val,  &p_w,
                             &p_outw, &p_deg,  &p_nval, &tiles,  &t_col,  &t_pos,  &t_val,  &hsegs,
                             &hrows,  &hub_part, &x[0],  &x[1],
-                            &s[0],   &s[1],   &state,  &partials, &red_part, &out};
+                            &s[0],   &s[1],   &state,  &partials, &red_part, &out,    &tele};
     size_t total = 0;
     for (const DevBuf* b : bufs) total += b->bytes;
     return total;
