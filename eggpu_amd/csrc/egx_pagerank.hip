@@ -91,6 +91,9 @@ struct IterArgs {
     int32_t cold_from; // first source label gathered non-temporally (n: none)
     const double* bias;  // per-row constant term replacing the teleport (Katz); null: PageRank
     const double* tele;  // per-row teleport weights summing to 1 (personalised); null: 1/n
+    const int32_t* perm; // iteration row -> caller id (null: identity); output only
+    double* out;         // normalised result (n doubles); output only
+    int64_t n_dangling;  // dangling rows, for the initial dangling mass
     double alpha;
     double tol_n;
     int32_t max_iter;
@@ -368,6 +371,61 @@ __global__ __launch_bounds__(kWave) void k_pr_final_multi(const IterArgs<T>* __r
     reduce_final_wave<T>(a);
 }
 
+// Node-wise kernels over a table of graphs: graph i owns the blocks
+// [nbase[i], nbase[i+1]) of the launch, 256 nodes each.
+__device__ __forceinline__ int32_t graph_of_block(const uint32_t* __restrict__ nbase, int32_t count, uint32_t b) {
+    int32_t lo = 0, hi = count;  // last i with nbase[i] <= b
+    while (hi - lo > 1) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (nbase[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// x0 = 1/n, s0 = x0/deg and the initial state of every graph of the table
+// (the multi-graph form of k_pr_init plus the host-written PrState).
+template <typename T>
+__global__ void k_pr_init_multi(const IterArgs<T>* __restrict__ table, const uint32_t* __restrict__ nbase,
+                                int32_t count) {
+    const int32_t gi = graph_of_block(nbase, count, blockIdx.x);
+    const IterArgs<T>& a = table[gi];
+    const int32_t v = (int32_t)(blockIdx.x - nbase[gi]) * blockDim.x + threadIdx.x;
+    const T x0 = (T)(1.0 / (double)a.n);
+    if (v < a.n) {
+        a.xb[0][v] = x0;
+        const float d = a.deg[v];
+        a.sb[0][v] = (d > 0.0f) ? (T)(x0 / (T)d) : (T)0;
+    }
+    if (blockIdx.x == nbase[gi] && threadIdx.x == 0) {
+        PrState st{};
+        st.dang = (double)a.n_dangling * (double)x0;
+        st.sum = 1.0;
+        *a.state = st;
+    }
+}
+
+// states[i] = *table[i].state: one copy to the host instead of one per graph.
+template <typename T>
+__global__ void k_gather_states(const IterArgs<T>* __restrict__ table, int32_t count, PrState* __restrict__ states) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) states[i] = *table[i].state;
+}
+
+// k_pr_output for every graph of the table.
+template <typename T>
+__global__ void k_pr_output_multi(const IterArgs<T>* __restrict__ table, const uint32_t* __restrict__ nbase,
+                                  int32_t count) {
+    const int32_t gi = graph_of_block(nbase, count, blockIdx.x);
+    const IterArgs<T>& a = table[gi];
+    const int32_t r = (int32_t)(blockIdx.x - nbase[gi]) * blockDim.x + threadIdx.x;
+    if (r >= a.n) return;
+    const double norm = a.state->sum;
+    const T* x = (a.state->iter & 1) ? a.xb[1] : a.xb[0];
+    const int32_t v = a.perm ? a.perm[r] : r;
+    a.out[v] = (norm > 0.0) ? (double)x[r] * (1.0 / norm) : 1.0 / (double)a.n;
+}
+
 // x0 = init (1/n for PageRank, 1 for Katz; as T), s0 = x0/deg.
 template <typename T>
 __global__ void k_pr_init(const float* __restrict__ deg, int32_t n, double init, T* __restrict__ x,
@@ -453,26 +511,20 @@ struct CapturedRun {
     }
 };
 
-// Fills the launch arguments of graph g and resets its iterate to x0 on stream s.
+// Fills the launch arguments of graph g (refreshing its values for p.alpha
+// on the graph's own stream; *refreshed reports whether that queued work).
 template <typename T>
-int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a) {
+int fill_args(Graph& g, const SolveParams& p, IterArgs<T>& a, bool* refreshed = nullptr) {
     const uint32_t grid = (uint32_t)g.n_hseg + (uint32_t)g.n_tiles;
     if (grid == 0) {
         set_last_error("graph has no work items");
         return EGX_UNKNOW_ERROR;
     }
+    const double before = g.alpha_vals;
     EGX_TRY(refresh_alpha(g, p.alpha));
+    if (refreshed) *refreshed = g.weighted && before != g.alpha_vals;
     EGX_TRY(ensure_workspace(g, grid));
     const int32_t n = g.n;
-    k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, p.bias ? 1.0 : 1.0 / (double)n, g.x[0].as<T>(),
-                                              g.s[0].as<T>());
-    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-    // staged through the pinned state buffer: the copy may run after return
-    PrState* st0 = static_cast<PrState*>(g.host_state);
-    *st0 = PrState{};
-    st0->dang = (double)g.n_dangling * (double)(T)(1.0 / (double)n);
-    st0->sum = 1.0;
-    EGX_HIP_TRY(hipMemcpyAsync(g.state.p, st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
     a = IterArgs<T>{};
     a.row = g.it_row;
     a.col = g.it_col;
@@ -503,9 +555,29 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
     a.n_red = (uint32_t)std::min<int64_t>(kMaxRed, std::max<int64_t>(1, (g.n_hrows + grid + kRedItems - 1) / kRedItems));
     a.bias = p.bias;
     a.tele = p.tele;
+    a.perm = g.reordered ? g.perm.as<int32_t>() : nullptr;
+    a.out = g.out.as<double>();
+    a.n_dangling = g.n_dangling;
     a.alpha = p.alpha;
     a.tol_n = std::max(0.0, p.tol) * (p.bias ? 1.0 : (double)n);  // Katz: absolute L1 test
     a.max_iter = p.max_iter;
+    return EGX_SUCC;
+}
+
+// fill_args, then reset graph g's iterate to x0 and its state on stream s.
+template <typename T>
+int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a) {
+    EGX_TRY(fill_args<T>(g, p, a));
+    const int32_t n = g.n;
+    k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, p.bias ? 1.0 : 1.0 / (double)n, g.x[0].as<T>(),
+                                              g.s[0].as<T>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    // staged through the pinned state buffer: the copy may run after return
+    PrState* st0 = static_cast<PrState*>(g.host_state);
+    *st0 = PrState{};
+    st0->dang = (double)g.n_dangling * (double)(T)(1.0 / (double)n);
+    st0->sum = 1.0;
+    EGX_HIP_TRY(hipMemcpyAsync(g.state.p, st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
 
@@ -616,11 +688,27 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     std::vector<IterArgs<T>> table((size_t)count);
     std::vector<uint32_t> base((size_t)count);
     uint64_t total = 0;
-    for (Graph* g : gs) EGX_HIP_TRY(hipStreamSynchronize(g->stream), EGX_DEVICE_ERR);
+    // work still queued on a graph's own stream (none after the synchronous
+    // entry points) must land before this launch reads the graph
+    for (Graph* g : gs) {
+        const hipError_t q = hipStreamQuery(g->stream);
+        if (q == hipErrorNotReady) {
+            (void)hipGetLastError();
+            EGX_HIP_TRY(hipStreamSynchronize(g->stream), EGX_DEVICE_ERR);
+        } else if (q != hipSuccess) {
+            set_last_error(hipGetErrorString(q));
+            return EGX_DEVICE_ERR;
+        }
+    }
+    std::vector<Graph*> refreshed;
+    std::vector<uint32_t> nbase((size_t)count + 1, 0);  // node blocks of the init / output launches
     for (int32_t i = 0; i < count; ++i) {
-        EGX_TRY(prepare_graph<T>(*gs[(size_t)i], p, s, table[(size_t)i]));
+        bool r = false;
+        EGX_TRY(fill_args<T>(*gs[(size_t)i], p, table[(size_t)i], &r));
+        if (r) refreshed.push_back(gs[(size_t)i]);
         base[(size_t)i] = (uint32_t)total;
         total += table[(size_t)i].grid;
+        nbase[(size_t)i + 1] = nbase[(size_t)i] + nblk(gs[(size_t)i]->n, 256);
     }
     if (total > 0x7FFFFFFFull) {
         set_last_error("too many work items for one launch");
@@ -689,8 +777,11 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     std::vector<int32_t> rblk((size_t)rtotal);
     for (int32_t i = 0; i < count; ++i)
         std::fill(rblk.begin() + rbase[(size_t)i], rblk.begin() + rbase[(size_t)i] + table[(size_t)i].n_red, i);
-    DevBuf d_table, d_blk, d_base, d_rblk, d_rbase;
+    DevBuf d_table, d_blk, d_base, d_rblk, d_rbase, d_nbase, d_states;
     EGX_TRY(d_table.alloc(sizeof(IterArgs<T>) * (size_t)count));
+    EGX_TRY(d_nbase.alloc(nbase.size() * 4));
+    EGX_TRY(d_states.alloc(sizeof(PrState) * (size_t)count));
+    EGX_HIP_TRY(hipMemcpyAsync(d_nbase.p, nbase.data(), nbase.size() * 4, hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
     EGX_TRY(d_blk.alloc(blk.size() * 4));
     EGX_TRY(d_base.alloc(blk_local.size() * 4));
     EGX_TRY(d_rblk.alloc(rblk.size() * 4));
@@ -703,7 +794,6 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     EGX_HIP_TRY(hipMemcpyAsync(d_base.p, blk_local.data(), blk_local.size() * 4, hipMemcpyHostToDevice, s),
                 EGX_DEVICE_ERR);
     // per-graph states are gathered into one pinned copy for polling
-    std::vector<PrState> hstates((size_t)count);
     void* pinned = nullptr;
     EGX_HIP_TRY(hipHostMalloc(&pinned, sizeof(PrState) * (size_t)count), EGX_FAILED_TO_ALLOCATE_HOST_MEM);
     struct PinnedGuard {
@@ -716,8 +806,11 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     CapturedRun cap;
     cap.stream = s;
     // alpha refreshes run on each graph's own stream
-    for (Graph* g : gs) EGX_HIP_TRY(hipStreamSynchronize(g->stream), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    for (Graph* g : refreshed) EGX_HIP_TRY(hipStreamSynchronize(g->stream), EGX_DEVICE_ERR);
+    // every iterate and state reset in one launch
+    k_pr_init_multi<T><<<nbase[(size_t)count], 256, 0, s>>>(d_table.as<IterArgs<T>>(), d_nbase.as<uint32_t>(),
+                                                              count);
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipEventRecord(ev0, s), EGX_DEVICE_ERR);
     int32_t launched = 0;
     while (launched < p.max_iter) {
@@ -750,18 +843,30 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
         }
         launched += todo;
         if (launched >= p.max_iter) break;
-        for (int32_t i = 0; i < count; ++i)
-            EGX_HIP_TRY(hipMemcpyAsync(hs + i, gs[(size_t)i]->state.p, sizeof(PrState), hipMemcpyDeviceToHost, s),
-                        EGX_DEVICE_ERR);
+        k_gather_states<T><<<nblk(count, 256), 256, 0, s>>>(d_table.as<IterArgs<T>>(), count,
+                                                             d_states.as<PrState>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipMemcpyAsync(hs, d_states.p, sizeof(PrState) * (size_t)count, hipMemcpyDeviceToHost, s),
+                    EGX_DEVICE_ERR);
         EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
         bool all_done = true;
         for (int32_t i = 0; i < count && all_done; ++i) all_done = hs[i].done != 0;
         if (all_done) break;
     }
     EGX_HIP_TRY(hipEventRecord(ev1, s), EGX_DEVICE_ERR);
+    k_gather_states<T><<<nblk(count, 256), 256, 0, s>>>(d_table.as<IterArgs<T>>(), count, d_states.as<PrState>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(hs, d_states.p, sizeof(PrState) * (size_t)count, hipMemcpyDeviceToHost, s),
+                EGX_DEVICE_ERR);
+    // every normalised result in one launch; copies only where asked for
+    k_pr_output_multi<T><<<nbase[(size_t)count], 256, 0, s>>>(d_table.as<IterArgs<T>>(), d_nbase.as<uint32_t>(),
+                                                                count);
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     for (int32_t i = 0; i < count; ++i)
-        EGX_HIP_TRY(hipMemcpyAsync(hs + i, gs[(size_t)i]->state.p, sizeof(PrState), hipMemcpyDeviceToHost, s),
-                    EGX_DEVICE_ERR);
+        if (host_out && host_out[i])
+            EGX_HIP_TRY(hipMemcpyAsync(host_out[i], gs[(size_t)i]->out.p, (size_t)gs[(size_t)i]->n * 8,
+                                       hipMemcpyDeviceToHost, s),
+                        EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     if (st) {
         float ms = 0.0f;
@@ -772,11 +877,8 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
         st->iterations = 0;
         for (int32_t i = 0; i < count; ++i) st->iterations = std::max(st->iterations, hs[i].iter);
     }
-    for (int32_t i = 0; i < count; ++i) {
-        if (iters) iters[i] = hs[i].iter;
-        EGX_TRY(emit_result<T>(*gs[(size_t)i], s, host_out ? host_out[i] : nullptr));
-    }
-    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    if (iters)
+        for (int32_t i = 0; i < count; ++i) iters[i] = hs[i].iter;
     return EGX_SUCC;
 }
 
