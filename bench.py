@@ -194,7 +194,10 @@ def main():
         for name in wanted:
             log(f"secondary workload: {name}")
             t1 = time.perf_counter()
-            secondary[name] = getattr(benchmarks, name)()
+            try:
+                secondary[name] = getattr(benchmarks, name)()
+            except Exception as exc:  # reported in the line; the headline stands on its own
+                secondary[name] = {"error": f"{type(exc).__name__}: {exc}"}
             secondary[name]["section_wall_s"] = round(time.perf_counter() - t1, 1)
             log(f"{name}: {json.dumps(secondary[name])}")
 

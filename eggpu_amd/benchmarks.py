@@ -292,12 +292,14 @@ def _serving_requests(graphs, count, seed):
     return reqs
 
 
-def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, step_iters=4):
+def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, step_iters=4, repeats=3):
     """Poisson arrivals against a device-resident pool; p50/p99 latency at a
     fixed QPS beside the closed-loop solve rate and the one-call-per-solve
-    rate on the same request stream.  The closed loop runs twice: with
-    request coalescing (identical requests in flight share a column; the
-    server default) and with every request on a column of its own."""
+    rate on the same request stream.  The closed loop runs with request
+    coalescing (identical requests in flight share a column; the server
+    default) and with every request on a column of its own; closed loops and
+    the one-call-per-request sample each report the best of `repeats`
+    passes on warm state."""
     specs = workloads.serving_pool_specs(pool_size, seed=seed)
     graphs = [s.build(native) for s in specs]
     arcs = [int(g.info()["m"]) for g in graphs]
@@ -315,51 +317,64 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
         plain((gi, (), 0.85))
         plain((gi, (0,), 0.85))
     sample = reqs[:plain_sample]
-    t0 = time.perf_counter()
-    for r in sample:
-        plain(r)
-    plain_rate = len(sample) / (time.perf_counter() - t0)
+    plain_walls = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        for r in sample:
+            plain(r)
+        plain_walls.append(time.perf_counter() - t0)
+    plain_rate = len(sample) / min(plain_walls)
 
-    def run(rate, coalesce=True):
+    def run(rate, coalesce=True, passes=1):
+        """The request stream against a warm server, `passes` times; returns
+        the best pass (solves/s, latencies, stats of that pass)."""
+        best = None
         with native.Server(graphs, step_iters=step_iters, coalesce=coalesce) as srv:
             # warm: open every graph's session (steady-state serving keeps them open)
             for gi in range(len(graphs)):
                 for rid in [srv.submit(gi, seeds, alpha=0.85, max_iter=100, tol=1e-6) for seeds in ((), (0,))]:
                     srv.wait(rid)
-            st0 = srv.stats()
-            ids = [None] * len(reqs)
-            gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=len(reqs)) if rate else None
+            for _ in range(passes):
+                res = _one_pass(srv, rate)
+                if best is None or res[0] > best[0]:
+                    best = res
+        return best
 
-            def feed():
-                t = time.perf_counter()
-                for k, (gi, seeds, a) in enumerate(reqs):
-                    if gaps is not None:
-                        t += gaps[k]
-                        while True:
-                            d = t - time.perf_counter()
-                            if d <= 0:
-                                break
-                            if d > 3e-4:
-                                time.sleep(d - 2e-4)
-                    ids[k] = srv.submit(gi, seeds, alpha=a, max_iter=100, tol=1e-6)
+    def _one_pass(srv, rate):
+        st0 = srv.stats()
+        ids = [None] * len(reqs)
+        gaps = np.random.default_rng(seed + 2).exponential(1.0 / rate, size=len(reqs)) if rate else None
 
-            t0 = time.perf_counter()
-            th = threading.Thread(target=feed)
-            th.start()
-            th.join()
-            lat, widths = [], []
-            for rid in ids:
-                _, info = srv.wait(rid)
-                lat.append(info["latency_seconds"])
-                widths.append(info["width"])
-            wall = time.perf_counter() - t0
-            st = srv.stats()
+        def feed():
+            t = time.perf_counter()
+            for k, (gi, seeds, a) in enumerate(reqs):
+                if gaps is not None:
+                    t += gaps[k]
+                    while True:
+                        d = t - time.perf_counter()
+                        if d <= 0:
+                            break
+                        if d > 3e-4:
+                            time.sleep(d - 2e-4)
+                ids[k] = srv.submit(gi, seeds, alpha=a, max_iter=100, tol=1e-6)
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        lat, widths = [], []
+        for rid in ids:
+            _, info = srv.wait(rid)
+            lat.append(info["latency_seconds"])
+            widths.append(info["width"])
+        wall = time.perf_counter() - t0
+        st = srv.stats()
         st = {k: (v - st0[k] if k in ("coalesced", "sessions_opened", "sessions_closed", "steps") else v)
               for k, v in st.items()}
         return len(reqs) / wall, np.array(lat), st
 
-    closed_rate, closed_lat, closed_st = run(None)
-    solo_rate, solo_lat, solo_st = run(None, coalesce=False)  # every request on a column of its own
+    closed_rate, closed_lat, closed_st = run(None, passes=repeats)
+    solo_rate, solo_lat, solo_st = run(None, coalesce=False, passes=repeats)  # every request on its own column
     target = qps if qps else 0.5 * closed_rate
     rate, lat, st = run(target)
     res = {
@@ -377,6 +392,7 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
         "closed_loop_no_coalescing_p99_ms": round(float(np.percentile(solo_lat, 99)) * 1e3, 3),
         "plain_solves_per_s": round(plain_rate, 1),
         "plain_calls_timed": len(sample),
+        "best_of_passes": repeats,
         "speedup_vs_plain": round(closed_rate / plain_rate, 2),
         "speedup_vs_plain_no_coalescing": round(solo_rate / plain_rate, 2),
         "peak_active_columns": st["peak_active_columns"],

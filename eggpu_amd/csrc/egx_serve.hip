@@ -35,6 +35,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -63,6 +64,7 @@ struct Request {
     int32_t iterations = 0;
     int32_t width = 0;  // columns of the session when the request finished
     int status = EGX_SUCC;
+    std::string msg;  // failure detail, from the thread where the failure happened
     bool done = false;
     std::vector<std::shared_ptr<Request>> riders;  // identical requests sharing this column
 
@@ -202,7 +204,13 @@ struct egx_server {
                 deliveries.pop_front();
             }
             int st = EGX_SUCC;
-            if (d.ev && hipEventSynchronize(d.ev) != hipSuccess) st = EGX_DEVICE_ERR;
+            if (d.ev) {
+                const hipError_t e = hipEventSynchronize(d.ev);
+                if (e != hipSuccess) {
+                    egx::set_last_error(std::string("result copy event: ") + hipGetErrorString(e));
+                    st = EGX_DEVICE_ERR;
+                }
+            }
             if (st == EGX_SUCC && d.staged) {
                 const size_t bytes = (size_t)info[(size_t)d.r->graph].n * sizeof(double);
                 if (d.r->out) std::memcpy(d.r->out, d.staged, bytes);
@@ -214,12 +222,15 @@ struct egx_server {
         }
     }
 
-    // Completes r and the requests riding on its column.
+    // Completes r and the requests riding on its column.  A failure keeps
+    // this thread's last error text, which egx_server_wait hands to the caller.
     void finish(const std::shared_ptr<Request>& r, int status) {
+        const std::string why = status != EGX_SUCC ? std::string(egx_last_error()) : std::string();
         std::lock_guard<std::mutex> lk(mu);
         const Clock::time_point now = Clock::now();
         auto one = [&](Request& q) {
             q.status = status;
+            q.msg = why;
             q.t_done = now;
             q.done = true;
             stats.completed += 1;
@@ -570,6 +581,7 @@ int egx_server_wait(egx_server* sv, int64_t request_id, double timeout_seconds, 
         return EGX_UNKNOW_ERROR;
     }
     sv->all.erase(request_id);
+    if (r->status != EGX_SUCC) egx::set_last_error(r->msg.empty() ? "request failed in the server" : r->msg);
     if (info) {
         info->iterations = r->iterations;
         info->width = r->width;
