@@ -26,6 +26,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "egx_graph.h"
@@ -1056,6 +1057,33 @@ int batch_release(Batch& b, int32_t j) {
 
 int enqueue_iterations(Batch& b, const SpmmArgs& a, int32_t iters, hipStream_t s);
 
+// Step graphs are captured on one stream per device, shared by every batch
+// there, never on a batch's own stream: other threads may be waiting on events
+// recorded on a batch's stream (served results), and a wait on a stream that
+// is capturing invalidates its capture.  One shared stream rather than one per
+// batch because HIP gives each new stream the least-used of a few hardware
+// queues: a capture stream created beside every batch stream would leave the
+// batch streams on half of them.  Captures hold `mu` from begin to end.  The
+// streams live until the process ends.
+struct CaptureStreams {
+    std::mutex mu;
+    std::vector<hipStream_t> per_device;
+};
+
+CaptureStreams& capture_streams() {
+    static CaptureStreams* c = new CaptureStreams();
+    return *c;
+}
+
+// The current device's capture stream (created on first use); `mu` is held.
+int capture_stream_locked(CaptureStreams& c, int device, hipStream_t& out) {
+    if (device < 0) return EGX_DEVICE_ERR;
+    if (c.per_device.size() <= static_cast<size_t>(device)) c.per_device.resize(device + 1, nullptr);
+    if (!c.per_device[device]) EGX_HIP_TRY(hipStreamCreate(&c.per_device[device]), EGX_DEVICE_ERR);
+    out = c.per_device[device];
+    return EGX_SUCC;
+}
+
 // Enqueue `iters` iterations (launches after every column stops return at
 // once).  Asynchronous: returns before the work completes.
 int batch_step(Batch& b, int32_t iters, bool record_time) {
@@ -1093,10 +1121,16 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
             // the previous replay may still be running: let it finish before its exec goes away
             if (b.step_exec) EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
             b.drop_step_graph();
-            EGX_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
-            const int rc = enqueue_iterations(b, a, iters, s);
+            // captured on the device's capture stream, replayed on `s` below
+            CaptureStreams& cap = capture_streams();
+            std::unique_lock<std::mutex> lk(cap.mu);
+            hipStream_t cs = nullptr;
+            EGX_TRY(capture_stream_locked(cap, b.device, cs));
+            EGX_HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
+            const int rc = enqueue_iterations(b, a, iters, cs);
             hipGraph_t graph = nullptr;
-            const hipError_t e = hipStreamEndCapture(s, &graph);
+            const hipError_t e = hipStreamEndCapture(cs, &graph);
+            lk.unlock();
             if (rc != EGX_SUCC) {
                 if (graph) (void)hipGraphDestroy(graph);
                 return rc;
