@@ -11,6 +11,7 @@ from .bulk import EGGPUBulkGraph, read_eggpu_csr, write_eggpu_csr
 from .classes import DiGraph, Graph, MultiDiGraph, MultiGraph
 from .exception import EasyGraphError, EasyGraphException, EasyGraphNotImplemented
 from .native import DeviceGraph, NativeLibraryMissing
+from .eigenvector import eigenvector_centrality, set_cpu_eigenvector
 from .katz import katz_centrality, set_cpu_katz
 from .pagerank import pagerank, set_cpu_pagerank
 
@@ -19,7 +20,9 @@ __version__ = "0.1.0"
 __all__ = [
     "pagerank",
     "set_cpu_pagerank",
+    "eigenvector_centrality",
     "katz_centrality",
+    "set_cpu_eigenvector",
     "set_cpu_katz",
     "DeviceGraph",
     "NativeLibraryMissing",

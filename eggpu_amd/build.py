@@ -25,6 +25,7 @@ SOURCES = [
     "egx_pagerank.hip",
     "egx_spmm.hip",
     "egx_katz.hip",
+    "egx_eigen.hip",
     "egx_api.hip",
     "egx_serve.hip",
     "egx_ingest.hip",

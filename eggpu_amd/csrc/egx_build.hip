@@ -396,6 +396,7 @@ int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const 
     hipStream_t s = g.stream;
     g.n = n;
     g.weighted = dW != nullptr || g.adjacency;
+    g.adjacency_weights = g.adjacency && dW != nullptr;
     g.alpha_vals = alpha;
     g.alpha_orig = alpha;
     const auto t_start = std::chrono::steady_clock::now();

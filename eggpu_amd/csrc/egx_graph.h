@@ -64,8 +64,12 @@ struct PrState {
     double sum;       // sum of the last iterate
     int32_t iter;     // iterations performed
     int32_t done;     // stop flag, set on the device
-    int32_t pad0;
-    int32_t pad1;
+    // eigenvector mode: the iterate buffers hold y = A x unnormalised, and
+    // x = scale * y for the input buffer, scale_prev * y for the other one
+    double scale;
+    double scale_prev;
+    int32_t eig_iters;  // iterations the reference loop counts at the stop
+    int32_t eig_stop;   // 1 delta below tol, 2 max_iter, 3 norm below the floor
 };
 
 // LDS entries (of the iterate type) a tile may stage: tile_items + hub
@@ -96,6 +100,9 @@ struct Graph {
     // and no division by the source's out-weight (Katz: x' = alpha*A*x + b);
     // set before build_from_device_csr
     bool adjacency = false;
+    // adjacency graph built with weights: the weighted kernels gather
+    // fl32(alpha * w) (eigenvector centrality); false: unit weights
+    bool adjacency_weights = false;
     double alpha_vals = 0.0;   // alpha baked into it_val
     double alpha_orig = 0.0;   // alpha baked into in_val
     int32_t max_in_degree = 0;
@@ -182,6 +189,17 @@ struct SolveParams {
     // weights (device, iteration-row order, summing to 1) that receive both
     // the teleport and the dangling mass; null: uniform 1/n
     const double* tele = nullptr;
+    // eigenvector mode (eigenvector.py:110-131, eigenvector.cpp:33-97): FP64,
+    // adjacency graph, alpha 1.  x <- (A + eig_shift I) x / ||.||_2 from eig_x0
+    // (device, caller order, already normalised); stop once the delta between
+    // normalised iterates (eig_delta 1: L1, 2: L2) is below tol, after
+    // max_iter iterations, or when ||A x|| < eig_floor (keeping x).  Each
+    // iteration's delta is found in the following launch, so up to
+    // max_iter + 1 launches run.
+    int32_t eig_delta = 0;  // 0: not eigenvector mode
+    double eig_floor = 0.0;
+    double eig_shift = 0.0;
+    const double* eig_x0 = nullptr;
 };
 struct SolveStats {
     double kernel_seconds = 0.0;  // first to last iteration launch

@@ -91,6 +91,9 @@ struct IterArgs {
     int32_t cold_from; // first source label gathered non-temporally (n: none)
     const double* bias;  // per-row constant term replacing the teleport (Katz); null: PageRank
     const double* tele;  // per-row teleport weights summing to 1 (personalised); null: 1/n
+    int32_t eig;         // eigenvector mode: 0 off, 1 L1 delta, 2 L2 delta
+    double eig_floor;    // eigenvector: stop when ||A x||_2 < eig_floor
+    double eig_shift;    // eigenvector: iterate A + eig_shift I
     const int32_t* perm; // iteration row -> caller id (null: identity); output only
     double* out;         // normalised result (n doubles); output only
     int64_t n_dangling;  // dangling rows, for the initial dangling mass
@@ -123,6 +126,22 @@ __device__ __forceinline__ T term(const IterArgs<T>& a, int32_t u, float w) {
 template <typename T, bool W>
 __device__ __forceinline__ void finalize_row(const IterArgs<T>& a, int32_t v, double acc, double jump, double& err,
                                              double& sum, double& dang) {
+    if (a.eig) {
+        // eigenvector: jump holds the input's scale; x = scale * y is the
+        // normalised iterate, y' = A x + shift x is written unnormalised, and
+        // the other buffer's old y (the iterate before x) gives the delta
+        const double xc = jump * (double)a.x_in[v];
+        const double yn = jump * acc + a.eig_shift * xc;
+        const PrState* st = a.state;
+        if (st->iter > 0) {
+            const double d = xc - st->scale_prev * (double)a.x_out[v];
+            err += (a.eig == 1) ? fabs(d) : d * d;
+        }
+        sum += yn * yn;
+        a.x_out[v] = (T)yn;
+        if (!W) a.s_out[v] = (T)yn;  // out-weight 1 on adjacency graphs
+        return;
+    }
     // uniform branches (kernel arguments): Katz bias, or personalised teleport
     // where jump holds (1-alpha) + alpha*dang
     if (a.bias) jump = a.bias[v];
@@ -143,6 +162,7 @@ __device__ __forceinline__ void finalize_row(const IterArgs<T>& a, int32_t v, do
 // this returns (1-alpha) + alpha*dang, scaled by p[v] per row.
 template <typename T>
 __device__ __forceinline__ double jump_of(const IterArgs<T>& a) {
+    if (a.eig) return a.state->scale;
     if (a.tele) return (1.0 - a.alpha) + a.alpha * a.state->dang;
     const double n_d = (double)a.n;
     return (1.0 - a.alpha) / n_d + a.alpha * a.state->dang / n_d;
@@ -302,7 +322,33 @@ __device__ __forceinline__ void reduce_final_wave(const IterArgs<T>& a) {
     e = wave_sum(e);
     su = wave_sum(su);
     dg = wave_sum(dg);
-    if (l == 0) {
+    if (l == 0 && a.eig) {
+        // launch p found delta_p (x_p against x_{p-1}, p >= 1) and ||A x_p||;
+        // the reference's iteration p tests the delta, iteration p+1 the norm
+        PrState* st = a.state;
+        const int32_t p = st->iter;
+        const double delta = (a.eig == 2) ? sqrt(e) : e;
+        const double nrm = sqrt(su);
+        st->err = delta;
+        st->sum = nrm;
+        int32_t stop = 0, counted = p;
+        if (p >= 1 && delta < a.tol_n) stop = 1;
+        else if (p >= a.max_iter) stop = 2;
+        else if (!(nrm >= a.eig_floor) || !(nrm > 0.0)) {
+            stop = 3;
+            counted = p + 1;
+        }
+        if (stop) {
+            // x_p (input buffer, scale) is the result; iter keeps its parity
+            st->eig_stop = stop;
+            st->eig_iters = counted;
+            st->done = 1;
+        } else {
+            st->scale_prev = st->scale;
+            st->scale = 1.0 / nrm;
+            st->iter = p + 1;
+        }
+    } else if (l == 0) {
         PrState* st = a.state;
         const int32_t it = st->iter + 1;
         st->err = e;
@@ -451,6 +497,30 @@ __global__ void k_pr_output(const T* __restrict__ x0, const T* __restrict__ x1, 
     out[v] = (norm > 0.0) ? (double)x[r] * (1.0 / norm) : 1.0 / (double)n;
 }
 
+// eigenvector: x[r] = x0[perm[r]] (caller order, normalised), s = x
+// (out-weight 1); state: no iterations, scale 1.
+template <typename T>
+__global__ void k_eig_init(const double* __restrict__ x0, const int32_t* __restrict__ perm, int32_t n,
+                           T* __restrict__ x, T* __restrict__ s) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const T v = (T)x0[perm ? perm[r] : r];
+    x[r] = v;
+    s[r] = v;
+}
+
+// eigenvector result: out[perm[r]] = scale * y[r] for the stopping launch's
+// input buffer (its iterate x_p).
+template <typename T>
+__global__ void k_eig_output(const T* __restrict__ x0, const T* __restrict__ x1, const PrState* __restrict__ st,
+                             const int32_t* __restrict__ perm, int32_t n, double* __restrict__ out) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const T* x = (st->iter & 1) ? x1 : x0;
+    const int32_t v = perm ? perm[r] : r;
+    out[v] = st->scale * (double)x[r];
+}
+
 // Katz result: out[perm[r]] = the iterate before the last update
 // (katz_centrality.py:95-99 keeps x when the test on x_new passes), in FP64.
 template <typename T>
@@ -555,11 +625,15 @@ int fill_args(Graph& g, const SolveParams& p, IterArgs<T>& a, bool* refreshed = 
     a.n_red = (uint32_t)std::min<int64_t>(kMaxRed, std::max<int64_t>(1, (g.n_hrows + grid + kRedItems - 1) / kRedItems));
     a.bias = p.bias;
     a.tele = p.tele;
+    a.eig = p.eig_delta;
+    a.eig_floor = p.eig_floor;
+    a.eig_shift = p.eig_shift;
     a.perm = g.reordered ? g.perm.as<int32_t>() : nullptr;
     a.out = g.out.as<double>();
     a.n_dangling = g.n_dangling;
     a.alpha = p.alpha;
-    a.tol_n = std::max(0.0, p.tol) * (p.bias ? 1.0 : (double)n);  // Katz: absolute L1 test
+    // Katz and eigenvector: absolute test
+    a.tol_n = std::max(0.0, p.tol) * ((p.bias || p.eig_delta) ? 1.0 : (double)n);
     a.max_iter = p.max_iter;
     return EGX_SUCC;
 }
@@ -569,6 +643,17 @@ template <typename T>
 int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a) {
     EGX_TRY(fill_args<T>(g, p, a));
     const int32_t n = g.n;
+    if (p.eig_delta) {
+        k_eig_init<T><<<nblk(n, 256), 256, 0, s>>>(p.eig_x0, g.reordered ? g.perm.as<int32_t>() : nullptr, n,
+                                                   g.x[0].as<T>(), g.s[0].as<T>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        PrState* st0 = static_cast<PrState*>(g.host_state);
+        *st0 = PrState{};
+        st0->scale = 1.0;
+        st0->scale_prev = 1.0;
+        EGX_HIP_TRY(hipMemcpyAsync(g.state.p, st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+        return EGX_SUCC;
+    }
     k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, p.bias ? 1.0 : 1.0 / (double)n, g.x[0].as<T>(),
                                               g.s[0].as<T>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
@@ -584,8 +669,12 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
 // Writes the result of graph g (and copies it to host_out): PageRank's
 // normalised iterate, or Katz's unnormalised previous iterate.
 template <typename T>
-int emit_result(Graph& g, hipStream_t s, double* host_out, bool katz = false) {
-    if (katz)
+int emit_result(Graph& g, hipStream_t s, double* host_out, bool katz = false, bool eig = false) {
+    if (eig)
+        k_eig_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
+                                                        g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
+                                                        g.out.as<double>());
+    else if (katz)
         k_katz_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
                                                          g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
                                                          g.out.as<double>());
@@ -606,17 +695,19 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
     EGX_TRY(prepare_graph<T>(g, p, s, a));
     const uint32_t grid = a.grid;
 
+    // eigenvector mode finds iteration k's delta in launch k+1
+    const int32_t max_launch = p.max_iter + (p.eig_delta ? 1 : 0);
     LaunchTimer timer;
-    if (p.time_each_launch) EGX_TRY(timer.reserve((size_t)4 * p.max_iter));
+    if (p.time_each_launch) EGX_TRY(timer.reserve((size_t)4 * max_launch));
     CapturedRun cap;
     cap.stream = s;
     EGX_HIP_TRY(hipEventRecord(g.ev0, s), EGX_DEVICE_ERR);
     int32_t launched = 0;
     int32_t chunk = p.check_every > 0 ? p.check_every : 4;
-    const bool can_stop_early = p.tol > 0.0;
+    const bool can_stop_early = p.tol > 0.0 || p.eig_delta != 0;
     PrState* hs = static_cast<PrState*>(g.host_state);
-    while (launched < p.max_iter) {
-        const int32_t todo = can_stop_early ? std::min(chunk, p.max_iter - launched) : (p.max_iter - launched);
+    while (launched < max_launch) {
+        const int32_t todo = can_stop_early ? std::min(chunk, max_launch - launched) : (max_launch - launched);
         if (p.use_graph && !p.time_each_launch && can_stop_early && todo == p.check_every) {
             // replay one captured chunk of `check_every` iterations
             if (!cap.exec) {
@@ -647,7 +738,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
                 if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 3], s), EGX_DEVICE_ERR);
             }
         }
-        if (launched >= p.max_iter) break;
+        if (launched >= max_launch) break;
         EGX_HIP_TRY(hipMemcpyAsync(hs, g.state.p, sizeof(PrState), hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
         EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
         if (hs->done) break;
@@ -676,7 +767,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
             st->spmv_launches = iters;
         }
     }
-    EGX_TRY(emit_result<T>(g, s, host_out, p.bias != nullptr));
+    EGX_TRY(emit_result<T>(g, s, host_out, p.bias != nullptr, p.eig_delta != 0));
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
@@ -924,7 +1015,12 @@ int solve_pagerank(Graph& g, const SolveParams& p_in, double* host_out, SolveSta
     }
     // adjacency graphs have out-weight 1 everywhere, so the unweighted path
     // (s = x/1, x' = alpha * row sum + b) applies alpha in FP64 to each sum
-    const bool w = g.weighted && !g.adjacency;
+    // unless they carry weights (eigenvector centrality with a weight key)
+    const bool w = g.weighted && (!g.adjacency || g.adjacency_weights);
+    if (p.eig_delta && (p.precision != EGX_PRECISION_FP64 || !g.adjacency)) {
+        set_last_error("eigenvector mode runs in FP64 on an adjacency graph");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
     if (p.precision == EGX_PRECISION_FP64)
         return w ? solve_typed<double, true>(g, p, host_out, st) : solve_typed<double, false>(g, p, host_out, st);
     return w ? solve_typed<float, true>(g, p, host_out, st) : solve_typed<float, false>(g, p, host_out, st);

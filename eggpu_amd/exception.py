@@ -8,6 +8,7 @@ try:  # pragma: no cover - depends on the host environment
         EasyGraphError,
         EasyGraphException,
         EasyGraphNotImplemented,
+        EasyGraphPointlessConcept,
     )
 except Exception:  # EasyGraph not installed: standalone definitions
 
@@ -20,5 +21,8 @@ except Exception:  # EasyGraph not installed: standalone definitions
     class EasyGraphNotImplemented(EasyGraphException):
         """Raised by algorithms not implemented for a type of graph."""
 
+    class EasyGraphPointlessConcept(EasyGraphException):
+        """Raised when a concept is undefined for the graph (the null graph)."""
 
-__all__ = ["EasyGraphException", "EasyGraphError", "EasyGraphNotImplemented"]
+
+__all__ = ["EasyGraphException", "EasyGraphError", "EasyGraphNotImplemented", "EasyGraphPointlessConcept"]

@@ -419,6 +419,20 @@ def adjacency_csr(G, prepared):
     return V.astype(np.int32), np.fromiter(dst, dtype=np.int64, count=len(dst)).astype(np.int32)
 
 
+def adjacency_weights(G, prepared, weight):
+    """Weights aligned with :func:`adjacency_csr`: ``attrs.get(weight, 1.0)``
+    for every adjacency entry, as eigenvector centrality reads them
+    (``eigenvector.py:80-89``)."""
+    if _is_bulk_graph(G):
+        _, _, W = G.csr_arrays(weight)
+        return np.asarray(W, dtype=np.float64)
+    adj = _adj_dict(G)
+    vals = []
+    for u in prepared["nodes"]:
+        vals.extend(float(d.get(weight, 1.0)) for d in adj[u].values())
+    return np.asarray(vals, dtype=np.float64)
+
+
 def graph_out_csr(G, prepared, weight=None):
     """Out-CSR of a protocol graph in the reference's order.
 

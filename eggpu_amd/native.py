@@ -148,6 +148,18 @@ class ServerStats(ctypes.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class EigOpts(ctypes.Structure):
+    _fields_ = [
+        ("max_iter", ctypes.c_int32),
+        ("delta", ctypes.c_int32),
+        ("tol", ctypes.c_double),
+        ("norm_floor", ctypes.c_double),
+        ("shift", ctypes.c_double),
+        ("start_norm", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+    ]
+
+
 class CacheStats(ctypes.Structure):
     _fields_ = [
         ("hits", ctypes.c_uint64),
@@ -226,6 +238,8 @@ SIGNATURES = {
     "egx_katz_csr": (ctypes.c_int, [_i32p, _i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _f64p,
                                     ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, _f64p, _i32p,
                                     _i32p, _f64p]),
+    "egx_eigenvector_csr": (ctypes.c_int, [_i32p, _i32p, _f64p, ctypes.c_int32, ctypes.c_int32, _f64p,
+                                           ctypes.POINTER(EigOpts), _f64p, _i32p, _i32p, _f64p]),
     "egx_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _f64p]),
     "egx_probe_gather": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _f64p]),
 }
@@ -675,6 +689,37 @@ def katz_csr(V, E, alpha=0.1, beta=1.0, max_iter=1000, tol=1e-6, normalized=True
                               int(max_iter), float(tol), 1 if normalized else 0, _ptr(out, _f64p),
                               ctypes.byref(it), ctypes.byref(conv), ctypes.byref(ks)))
     return out, int(it.value), bool(conv.value), float(ks.value)
+
+
+EIG_CONVERGED, EIG_MAX_ITER, EIG_NORM_FLOOR = 1, 2, 3
+
+
+def eigenvector_csr(V, E, W=None, x0=None, *, max_iter=100, tol=1e-6, delta=2, norm_floor=1e-10, shift=0.0,
+                    start_norm=1):
+    """``egx_eigenvector_csr`` on an out-CSR (row u lists the heads of u's
+    arcs; row v of A x sums over v's in-arcs): (float64[n] last normalised
+    iterate, iterations, stop, kernel_seconds).  stop is EIG_CONVERGED,
+    EIG_MAX_ITER or EIG_NORM_FLOOR.  x0 defaults to ones."""
+    V = _i32(V)
+    E = _i32(E)
+    n = len(V) - 1
+    if W is not None:
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        if len(W) != len(E):
+            raise ValueError(f"{len(W)} weights for {len(E)} arcs")
+    x0 = np.ones(max(n, 0), dtype=np.float64) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+    if len(x0) != max(n, 0):
+        raise ValueError(f"start vector has {len(x0)} values for {n} nodes")
+    opts = EigOpts(max_iter=int(max_iter), delta=int(delta), tol=float(tol), norm_floor=float(norm_floor),
+                   shift=float(shift), start_norm=int(start_norm), pad=0)
+    out = np.empty(max(n, 0), dtype=np.float64)
+    it = ctypes.c_int32(0)
+    stop = ctypes.c_int32(0)
+    ks = ctypes.c_double(0.0)
+    _check(lib().egx_eigenvector_csr(_ptr(V, _i32p), _ptr(E, _i32p), _ptr(W, _f64p), n, len(E), _ptr(x0, _f64p),
+                                     ctypes.byref(opts), _ptr(out, _f64p), ctypes.byref(it), ctypes.byref(stop),
+                                     ctypes.byref(ks)))
+    return out, int(it.value), int(stop.value), float(ks.value)
 
 
 def cache_stats():

@@ -326,6 +326,41 @@ int egx_katz_csr(const int32_t *V, const int32_t *E, int32_t n, int32_t m, doubl
                  int32_t *iterations, int32_t *converged, double *kernel_seconds);
 
 /* ------------------------------------------------------------------------ */
+/* Eigenvector centrality on the same SpMV engine.  No counterpart on the    */
+/* reference's GPU path; it restates the power iterations of the CPU         */
+/* functions eigenvector.py:110-131 (Python route, nstart given) and         */
+/* eigenvector.cpp:33-97 (C++ twin, cpp_eigenvector_centrality).  A[v][u] =  */
+/* W[p] (1 when W is NULL) for every arc p: u -> v of the CSR (V, E), so row */
+/* v of A x sums over v's in-arcs.  From the normalised start x_0:          */
+/*   y = A x_{k-1} + shift x_{k-1};  if ||y||_2 < norm_floor: stop, keep     */
+/*   x_{k-1};  x_k = y / ||y||_2;  if delta(x_k, x_{k-1}) < tol: stop        */
+/* for k = 1..max_iter.  FP64 iterates and sums; weights are gathered as    */
+/* FP32 values (the pull layout's value array).                              */
+/* ------------------------------------------------------------------------ */
+typedef struct egx_eig_opts {
+    int32_t max_iter;   /* iterations k = 1..max_iter (0: x_0 is returned) */
+    int32_t delta;      /* 1: sum|x_k - x_{k-1}| (eigenvector.cpp:86-93);
+                           2: ||x_k - x_{k-1}||_2 (eigenvector.py:128)      */
+    double tol;         /* >= 0 */
+    double norm_floor;  /* > 0: 1e-12 (eigenvector.cpp:83), 1e-10 (eigenvector.py:122) */
+    double shift;       /* >= 0: iterate A + shift I (0: the reference iteration) */
+    int32_t start_norm; /* x_0 = x0 / sum|x0| (1, eigenvector.py:113), or
+                           x0 / ||x0||_2, uniform 1/sqrt(n) when ||x0||_2 < 1e-12
+                           (2, eigenvector.cpp:44-56)                       */
+    int32_t pad;
+} egx_eig_opts;
+
+/* x0: n start values (caller order).  out: n doubles, the last normalised
+ * iterate kept (x_k, or x_{k-1} on a norm stop).  *iterations: the k of the
+ * stop (the reference loop's iteration count); *stop: 1 delta below tol,
+ * 2 max_iter reached without it (eigenvector.py raises EasyGraphError,
+ * eigenvector.cpp returns x), 3 norm below the floor (both return x_{k-1}).
+ * Weights must be finite and non-negative; n = 0 writes nothing (*stop 0). */
+int egx_eigenvector_csr(const int32_t *V, const int32_t *E, const double *W, int32_t n, int32_t m,
+                        const double *x0, const egx_eig_opts *opts, double *out, int32_t *iterations,
+                        int32_t *stop, double *kernel_seconds);
+
+/* ------------------------------------------------------------------------ */
 /* Serving (no counterpart in the reference, whose pagerank.py:141-179 solves */
 /* one graph per call): a worker thread answers global and personalised      */
 /* PageRank requests against a pool of device-resident graphs by continuous  */

@@ -23,6 +23,9 @@ Python-side helpers around ``egx_oracle.c``:
   the definitions the HIP generator must reproduce bit for bit.
 * :func:`katz_rows` / :func:`katz_graph` restate the CPU Katz centrality
   ``katz_centrality.py:7-110`` in FP64 with a sparse adjacency.
+* :func:`eigenvector_rows` / :func:`eigenvector_graph` restate the CPU
+  eigenvector centrality (``eigenvector.py:62-154`` and the power iteration
+  of ``eigenvector.cpp:33-97``) in FP64.
 
 Functions with no independent check are marked "parity unpinned" in their
 docstring; every other function is pinned in ``tests/test_oracle.py``.
@@ -423,3 +426,93 @@ def katz_graph(G, alpha=0.1, beta=1.0, max_iter=1000, tol=1e-6, normalized=True)
     if x is None:
         return None, it
     return {v: float(x[i]) for i, v in enumerate(nodes)}, it
+
+
+def eigenvector_rows(n, src, dst, w=None, x0=None, max_iter=100, tol=1e-6, delta=2, norm_floor=1e-10, shift=0.0,
+                     start_norm=1):
+    """The power iterations of eigenvector centrality in FP64, on
+    A[v][u] = w for every arc u -> v (duplicates summed, as
+    ``eigenvector.py:92`` builds it):
+
+    * ``eigenvector.py:110-131`` (delta=2, start_norm=1, norm_floor=1e-10):
+      v = x0 / sum|x0|; y = A v; stop keeping v when ||y|| < floor; v' = y/||y||;
+      stop when ||v' - v||_2 < tol;
+    * ``eigenvector.cpp:33-97`` (delta=1, start_norm=2, norm_floor=1e-12):
+      x = x0 * (1/||x0||) (uniform 1/sqrt(n) below 1e-12); x' = y * (1/||y||);
+      stop when sum|x' - x| < tol.
+
+    shift > 0 iterates A + shift*I.  Returns (x, iterations, stop): stop 1 the
+    delta test held, 2 max_iter iterations ran without it, 3 the norm test
+    stopped at that iteration (x is the iterate before it)."""
+    import scipy.sparse as sp
+
+    w = np.ones(len(src)) if w is None else np.asarray(w, dtype=np.float64)
+    A = sp.csr_matrix((w, (np.asarray(dst), np.asarray(src))), shape=(n, n))
+    v = np.ones(n, dtype=np.float64) if x0 is None else np.array(x0, dtype=np.float64)
+    if start_norm == 1:
+        v = v / np.sum(np.abs(v))
+    else:
+        nrm = np.sqrt(np.sum(v * v))
+        v = np.full(n, 1.0 / np.sqrt(n)) if nrm < 1e-12 else v * (1.0 / nrm)
+    for k in range(1, max_iter + 1):
+        last = v
+        y = A @ last
+        if shift:
+            y = y + shift * last
+        nrm = np.linalg.norm(y)
+        if nrm < norm_floor:
+            return last, k, 3
+        v = y * (1.0 / nrm) if delta == 1 else y / nrm
+        d = np.sum(np.abs(v - last)) if delta == 1 else np.linalg.norm(v - last)
+        if d < tol:
+            return v, k, 1
+    return v, max_iter, 2
+
+
+def eigenvector_graph(G, max_iter=100, tol=1e-6, nstart=None, weight=None):
+    """``eigenvector_centrality`` of ``eigenvector.py:62-154`` on a graph:
+    A from ``G[u].items()`` in ``list(G.nodes)`` order with weight
+    ``attrs.get(weight, 1.0)``, a 1e-4 self-loop on every node whose row and
+    column sums are both 0, the power iteration when nstart is given
+    (:func:`eigenvector_rows`, raising RuntimeError where the reference
+    raises EasyGraphError), else ``scipy.sparse.linalg.eigs(A, k=1,
+    which='LR', maxiter=max_iter, tol=tol)`` as the reference calls it; then
+    the sign fix, L2 normalisation, and isolated nodes set to 0 with a
+    renormalisation.  Raises ValueError for fewer than two nodes."""
+    import scipy.sparse as sp
+    from scipy.sparse import linalg as sla
+
+    nodes = list(G.nodes)
+    n = len(nodes)
+    if n < 2:
+        raise ValueError("eigenvector centrality needs at least two nodes")
+    idx = {node: i for i, node in enumerate(nodes)}
+    src, dst, w = [], [], []
+    for u in nodes:
+        for v, attrs in G[u].items():
+            src.append(idx[u])
+            dst.append(idx[v])
+            w.append(float(attrs.get(weight, 1.0)) if weight else 1.0)
+    A = sp.csr_matrix((w, (dst, src)), shape=(n, n))
+    isolated = np.where((np.asarray(A.sum(axis=1)).ravel() == 0) & (np.asarray(A.sum(axis=0)).ravel() == 0))[0]
+    if len(isolated):
+        src = src + [int(i) for i in isolated]
+        dst = dst + [int(i) for i in isolated]
+        w = w + [1.0e-4] * len(isolated)
+        A = sp.csr_matrix((w, (dst, src)), shape=(n, n))
+    if nstart is not None:
+        x0 = np.array([nstart.get(v, 1.0) for v in nodes], dtype=float)
+        c, _, stop = eigenvector_rows(n, np.asarray(src), np.asarray(dst), np.asarray(w), x0, max_iter, tol)
+        if stop == 2:
+            raise RuntimeError(f"Eigenvector calculation did not converge in {max_iter} iterations")
+    else:
+        _, vecs = sla.eigs(A, k=1, which="LR", maxiter=max_iter, tol=tol)
+        c = np.real(vecs[:, 0])
+    if c.sum() < 0:
+        c = -c
+    c = c / np.linalg.norm(c)
+    if len(isolated):
+        c[isolated] = 0.0
+        if np.sum(c) > 0:
+            c = c / np.linalg.norm(c)
+    return {nodes[i]: float(c[i]) for i in range(n)}

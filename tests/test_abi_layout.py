@@ -23,6 +23,7 @@ PAIRS = {
     "egx_request_info": native.RequestInfo,
     "egx_server_stats": native.ServerStats,
     "egx_cache_stats": native.CacheStats,
+    "egx_eig_opts": native.EigOpts,
 }
 
 

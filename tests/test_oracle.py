@@ -381,3 +381,157 @@ def test_katz_not_converging_is_reported():
     G.add_edges_from([(0, 1), (1, 2), (2, 0)])
     assert O.katz_graph(G, alpha=1.0, max_iter=50)[0] is None
     assert O.katz_graph(G, alpha=0.1, max_iter=0)[0] is None
+
+
+# -- Eigenvector centrality (eigenvector.py:62-154, eigenvector.cpp:33-97) -----
+_EIG_ROUTES = {
+    "py": dict(delta=2, start_norm=1, norm_floor=1e-10),
+    "cpp": dict(delta=1, start_norm=2, norm_floor=1e-12),
+}
+
+
+def _dense_adj(n, src, dst, w=None):
+    A = np.zeros((n, n))
+    for k, (u, v) in enumerate(zip(src.tolist(), dst.tolist())):
+        A[v, u] += 1.0 if w is None else w[k]
+    return A
+
+
+def _eig_case(seed):
+    V, E = O.rmat_csr(6, 6, seed=seed)
+    n = len(V) - 1
+    src, dst = O.csr_rows(V, E)
+    rng = np.random.default_rng(seed)
+    return n, src, dst, rng.uniform(0.1, 1.0, len(src)), rng.uniform(0.5, 2.0, n)
+
+
+@pytest.mark.parametrize("route", ["py", "cpp"])
+@pytest.mark.parametrize("shift", [0.0, 1.0])
+def test_eigenvector_iterates_are_normalised_matrix_powers(route, shift):
+    # x_k = (A + s I)^k x_0 / ||.||_2 whatever the start scaling: the loop run
+    # with tol 0 for k iterations must land on the dense closed form (a
+    # transposed A, a dropped weight or a missed shift all fail it)
+    n, src, dst, w, x0 = _eig_case(5)
+    M = _dense_adj(n, src, dst, w) + shift * np.eye(n)
+    for k in (1, 2, 7, 20):
+        x, it, stop = O.eigenvector_rows(n, src, dst, w, x0, max_iter=k, tol=0.0, shift=shift, **_EIG_ROUTES[route])
+        ref = np.linalg.matrix_power(M, k) @ x0
+        ref /= np.linalg.norm(ref)
+        assert (it, stop) == (k, 2)
+        assert np.max(np.abs(x - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("route", ["py", "cpp"])
+def test_eigenvector_stops_at_the_first_small_delta(route):
+    # the stop iteration from the dense sequence: the first k with
+    # delta(x_k, x_{k-1}) < tol, L2 (Python route) or L1 (C++ twin), where
+    # x_0 is the start scaled by sum|x0| or ||x0||_2
+    n, src, dst, w, x0 = _eig_case(9)
+    M = _dense_adj(n, src, dst, w)
+    kw = _EIG_ROUTES[route]
+    xs = [x0 / np.sum(np.abs(x0)) if kw["start_norm"] == 1 else x0 / np.linalg.norm(x0)]
+    for _ in range(400):
+        y = M @ xs[-1]
+        xs.append(y / np.linalg.norm(y))
+    d = np.array([(np.sum(np.abs(xs[k] - xs[k - 1])) if kw["delta"] == 1 else np.linalg.norm(xs[k] - xs[k - 1]))
+                  for k in range(1, len(xs))])
+    for tol in (1e-3, 1e-6, 1e-9):
+        k_ref = int(np.argmax(d < tol)) + 1
+        assert d[k_ref - 1] < tol and np.min(np.abs(d / tol - 1.0)) > 1e-6
+        x, it, stop = O.eigenvector_rows(n, src, dst, w, x0, max_iter=1000, tol=tol, **kw)
+        assert (it, stop) == (k_ref, 1)
+        assert np.max(np.abs(x - xs[k_ref])) < 1e-12
+
+
+def _karate(weighted):
+    import networkx as nx
+
+    H = nx.karate_club_graph()
+    G = Graph()
+    G.add_nodes_from(H.nodes)
+    for u, v, a in H.edges(data=True):
+        if weighted:
+            G.add_edge(u, v, weight=float(a["weight"]))
+        else:
+            G.add_edge(u, v)
+    return H, G
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_eigenvector_matches_networkx_at_convergence(weighted):
+    # networkx iterates A^T + I with an L1 test; ARPACK (the reference's
+    # default), the nstart power iteration and networkx all reach the
+    # principal eigenvector of the (connected, non-bipartite) karate graph
+    import networkx as nx
+
+    H, G = _karate(weighted)
+    wk = "weight" if weighted else None
+    ref = nx.eigenvector_centrality(H, max_iter=100000, tol=1e-15, weight=wk)
+    arpack = O.eigenvector_graph(G, tol=1e-14, max_iter=1000, weight=wk)
+    power = O.eigenvector_graph(G, nstart={v: 1.0 for v in G.nodes}, tol=1e-13, max_iter=100000, weight=wk)
+    for got in (arpack, power):
+        assert max(abs(got[v] - ref[v]) for v in H.nodes) < 1e-9
+
+
+def test_eigenvector_closed_forms():
+    # complete graph: uniform; star K_{1,4} (bipartite): ARPACK gives centre
+    # 1/sqrt(2) and leaves 1/(2 sqrt(2)) while the plain power iteration
+    # oscillates and raises; on a directed cycle the uniform direction is
+    # fixed: the C++ twin's L2-scaled start passes at the first iteration,
+    # the Python route's L1-scaled start (1/3 each) only at the second
+    K = Graph()
+    K.add_edges_from([(i, j) for i in range(5) for j in range(i + 1, 5)])
+    for x in (O.eigenvector_graph(K), O.eigenvector_graph(K, nstart={})):
+        assert max(abs(x[v] - 1 / np.sqrt(5)) for v in range(5)) < 1e-12
+    S = Graph()
+    S.add_edges_from([(0, k) for k in range(1, 5)])
+    x = O.eigenvector_graph(S, tol=1e-12)
+    assert abs(x[0] - 1 / np.sqrt(2)) < 1e-10 and max(abs(x[k] - 1 / np.sqrt(8)) for k in range(1, 5)) < 1e-10
+    with pytest.raises(RuntimeError):
+        O.eigenvector_graph(S, nstart={})
+    n, src, dst = 3, np.array([0, 1, 2]), np.array([1, 2, 0])
+    for route, k in (("cpp", 1), ("py", 2)):
+        x, it, stop = O.eigenvector_rows(n, src, dst, max_iter=100, tol=1e-6, **_EIG_ROUTES[route])
+        assert (it, stop) == (k, 1) and np.allclose(x, 1 / np.sqrt(3), rtol=0, atol=1e-15)
+
+
+def test_eigenvector_norm_stop_and_isolated_nodes():
+    # directed path 0 -> 1 -> 2 is nilpotent: x_1 = (0, 1, 1)/sqrt(2),
+    # x_2 = (0, 0, 1), A x_2 = 0, so the third iteration's norm test keeps x_2
+    src, dst = np.array([0, 1]), np.array([1, 2])
+    for route in ("py", "cpp"):
+        x, it, stop = O.eigenvector_rows(3, src, dst, max_iter=100, tol=1e-6, **_EIG_ROUTES[route])
+        assert (it, stop) == (3, 3) and np.array_equal(x, [0.0, 0.0, 1.0])
+    # a triangle plus an isolated node: the node gets a 1e-4 self-loop and
+    # then 0; the triangle stays uniform
+    G = Graph()
+    G.add_nodes_from(range(4))
+    G.add_edges_from([(0, 1), (1, 2), (0, 2)])
+    for x in (O.eigenvector_graph(G, nstart={}), O.eigenvector_graph(G)):
+        assert x[3] == 0.0 and max(abs(x[v] - 1 / np.sqrt(3)) for v in range(3)) < 1e-9
+
+
+def test_isolated_loops_host_rewrite():
+    # the product's CSR rewrite against a dense construction: A plus 1e-4 on
+    # the diagonal of every node with zero in- and out-weight (zero-weight
+    # arcs included), existing arcs unchanged
+    from eggpu_amd.eigenvector import with_isolated_loops
+
+    rng = np.random.default_rng(3)
+    n = 40
+    V, E = O.er_csr(n, 50, seed=4)
+    src, dst = O.csr_rows(V, E)
+    W = rng.uniform(0.1, 1.0, len(E))
+    W[src == 7] = 0.0
+    W[dst == 7] = 0.0
+    V2, E2, W2, iso = with_isolated_loops(np.asarray(V, dtype=np.int64), np.asarray(E, dtype=np.int32), W)
+    A = _dense_adj(n, src, dst, W)
+    expect = np.flatnonzero((A.sum(axis=0) == 0) & (A.sum(axis=1) == 0))
+    assert 7 in expect and np.array_equal(iso, expect)
+    s2, d2 = O.csr_rows(V2, E2)
+    B = _dense_adj(n, s2, d2, W2)
+    assert np.array_equal(B, A + np.diag(np.where(np.isin(np.arange(n), expect), 1e-4, 0.0)))
+    # unweighted with nothing isolated: returned as given
+    Vc = np.array([0, 1, 2], dtype=np.int64)
+    Ec = np.array([1, 0], dtype=np.int32)
+    assert with_isolated_loops(Vc, Ec, None)[2] is None
