@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default bins")
     ap.add_argument("--bins", help='fixed bins instead of the tuner, JSON of egx_graph_set_bins fields')
-    ap.add_argument("--sections", default="batched_ppr,multi_graph,serving,katz,ingest",
+    ap.add_argument("--sections", default="batched_ppr,multi_graph,serving,katz,eigenvector,ingest",
                     help="secondary workloads reported beside the headline line ('' for none)")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -175,6 +175,32 @@ def katz(scale=22, tol=1e-6):
     }
 
 
+def eigenvector(scale=22, tol=1e-9):
+    """Eigenvector centrality (the power iteration of eigenvector.cpp:33-97:
+    L2-scaled start, L1 delta, FP64) on a directed R-MAT graph through
+    egx_eigenvector_csr, from the all-ones start.  Each iteration's delta
+    is found in the following launch, so iterations + 1 SpMV launches run
+    when the delta test stops the loop; GTEPS counts every launch."""
+    g = native.DeviceGraph.rmat(scale, 16, seed=3, keep_csr=True)
+    V, E, _ = g.download_csr(with_weights=False)
+    g.close()
+    n, m = len(V) - 1, len(E)
+    t0 = time.perf_counter()
+    x, it, stop, ks = native.eigenvector_csr(V, E, None, None, max_iter=200, tol=tol, delta=1, norm_floor=1e-12,
+                                             start_norm=2)
+    wall = time.perf_counter() - t0
+    assert stop == native.EIG_CONVERGED
+    launches = it + 1
+    return {
+        "workload": f"eigenvector centrality, R-MAT scale {scale} directed (n={n}, m={m}), tol={tol:g} (L1), FP64",
+        "iterations": it,
+        "launches": launches,
+        "kernel_ms_per_launch": round(ks / launches * 1e3, 4),
+        "fp64_gteps": round(m * launches / ks / 1e9, 2),
+        "call_wall_s": round(wall, 3),
+    }
+
+
 def ingest(scale=24, weighted=True):
     """eggpu-csr-v1 artifact files (R-MAT, FP64 weights) streamed into HBM by
     egx_graph_from_csr_files: pinned double-buffered reads overlapped with
@@ -405,16 +431,17 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
 
 
 def main(argv=None):
-    """python -m eggpu_amd.benchmarks [batched_ppr|multi_graph|serving] ... -- one JSON line each."""
+    """python -m eggpu_amd.benchmarks [batched_ppr|multi_graph|serving|katz|eigenvector|ingest] ... -- one JSON line each."""
     import argparse
     import json
 
     ap = argparse.ArgumentParser(description=main.__doc__)
-    ap.add_argument("sections", nargs="*", default=["batched_ppr", "multi_graph", "serving", "katz", "ingest"])
+    ap.add_argument("sections", nargs="*",
+                    default=["batched_ppr", "multi_graph", "serving", "katz", "eigenvector", "ingest"])
     args = ap.parse_args(argv)
     native.lib()
     for name in args.sections:
-        if name not in ("batched_ppr", "multi_graph", "serving", "katz", "ingest"):
+        if name not in ("batched_ppr", "multi_graph", "serving", "katz", "eigenvector", "ingest"):
             raise SystemExit(f"unknown section {name}")
         print(json.dumps({name: globals()[name]()}), flush=True)
 
