@@ -244,7 +244,9 @@ size_t Graph::device_bytes() const {
                             &csr_E,  &csr_W,  &perm,   &rank,   &p_row,  &p_col,   &p_val,  &p_w,
                             &p_outw, &p_deg,  &p_nval, &tiles,  &t_col,  &t_pos,  &t_val,  &hsegs,
                             &hrows,  &hub_part, &x[0],  &x[1],
-                            &s[0],   &s[1],   &state,  &partials, &red_part, &out,    &tele};
+                            &s[0],   &s[1],   &state,  &partials, &red_part, &out,    &tele,
+                            &pb_src, &pb_dst, &pb_vals, &pb_vgrp, &pb_scatter, &pb_sum, &pb_hrows,
+                            &pb_hub_part};
     size_t total = 0;
     for (const DevBuf* b : bufs) total += b->bytes;
     return total;
@@ -349,6 +351,7 @@ int egx_graph_info_get(const egx_graph* gh, egx_graph_info* out) {
     out->hub_rows_first = g.n_hub;
     out->tile_sort = g.tsorted ? 1 : 0;
     out->hot_sources = g.bins.hot_sources;
+    out->blocked = g.pb_ready ? 1 : 0;
     out->reordered = g.reordered ? 1 : 0;
     out->layout_seconds = g.layout_seconds;
     out->rows_thread = g.rows_thread;
@@ -379,6 +382,7 @@ int egx_graph_set_bins(egx_graph* gh, const egx_bin_opts* opts) {
     if (opts->reorder >= 0) cfg.reorder = opts->reorder ? 1 : 0;
     if (opts->tile_sort >= 0) cfg.tile_sort = opts->tile_sort ? 1 : 0;
     if (opts->hot_sources != 0) cfg.hot_sources = opts->hot_sources > 0 ? opts->hot_sources : 0;
+    if (opts->blocked >= 0) cfg.blocked = opts->blocked ? 1 : 0;
     return egx::rebuild_bins(g, cfg);
 }
 

@@ -313,6 +313,167 @@ int build_tile_sort(Graph& g, const BinCfg& cfg) {
     return EGX_SUCC;
 }
 
+// Propagation-blocked layout of the iteration CSC (it_row, it_col), built on
+// the host by two stable counting sorts of the arcs: by (sub-bin, chunk) for
+// the value slots and by (chunk, sub-bin) for the scatter's source list.
+// Both keep CSC order (row, then source) inside each (sub-bin, chunk) run,
+// so run k of chunk c fills value slots [val0, val0 + len) in order.
+int build_blocked(Graph& g, const BinCfg& cfg) {
+    g.pb_ready = false;
+    for (DevBuf* b : {&g.pb_src, &g.pb_dst, &g.pb_vals, &g.pb_vgrp, &g.pb_scatter, &g.pb_sum, &g.pb_hrows,
+                      &g.pb_hub_part})
+        b->release();
+    g.n_pb_scatter = g.n_pb_sum = g.n_pb_hrows = 0;
+    if (!cfg.blocked || g.n <= 0 || g.m <= 0) return EGX_SUCC;
+    if (g.weighted) {
+        set_last_error("propagation-blocked iterations take unweighted graphs");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    const int32_t n = g.n;
+    const int64_t m = g.m;
+    const int64_t S = (n + kPbChunk - 1) / kPbChunk;
+    const int64_t B = (n + kPbRows - 1) / kPbRows;
+    if (S * B > ((int64_t)1 << 28)) {
+        set_last_error("graph too large for the propagation-blocked layout");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    std::vector<int32_t> row((size_t)n + 1), col((size_t)m);
+    EGX_HIP_TRY(hipStreamSynchronize(g.stream), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(row.data(), g.it_row, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(col.data(), g.it_col, (size_t)m * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+
+    // run lengths, bin-major (sub-bin b, chunk c) at b*S + c, then padded
+    std::vector<int32_t> cnt((size_t)(S * B), 0);
+    for (int32_t r = 0; r < n; ++r) {
+        const int64_t kb = (int64_t)(r / kPbRows) * S;
+        for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e) ++cnt[(size_t)(kb + col[(size_t)e] / kPbChunk)];
+    }
+    for (int32_t& k : cnt) k = (k + kPbVec - 1) / kPbVec * kPbVec;
+    // value-slot starts (bin-major) and source-list starts (chunk-major)
+    std::vector<int32_t> voff((size_t)(S * B)), soff((size_t)(S * B));
+    int64_t acc = 0;
+    for (int64_t k = 0; k < S * B; ++k) {
+        voff[(size_t)k] = (int32_t)acc;
+        acc += cnt[(size_t)k];
+    }
+    const int64_t slots = acc;
+    if (slots > INT32_MAX) {
+        set_last_error("propagation-blocked layout: too many slots");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    acc = 0;
+    for (int64_t c = 0; c < S; ++c)
+        for (int64_t b = 0; b < B; ++b) {
+            soff[(size_t)(c * B + b)] = (int32_t)acc;
+            acc += cnt[(size_t)(b * S + c)];
+        }
+    // slots, in CSC order within each run; padding slots keep source
+    // kPbChunk (the zero LDS entry) and row 0
+    std::vector<uint16_t> src16((size_t)slots, (uint16_t)kPbChunk), dst16((size_t)slots, 0);
+    {
+        std::vector<int32_t> vcur(voff), scur(soff);
+        for (int32_t r = 0; r < n; ++r) {
+            const int64_t b = r / kPbRows;
+            const uint16_t rl = (uint16_t)(r - b * kPbRows);
+            for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e) {
+                const int32_t u = col[(size_t)e];
+                const int64_t c = u / kPbChunk;
+                dst16[(size_t)vcur[(size_t)(b * S + c)]++] = rl;
+                src16[(size_t)scur[(size_t)(c * B + b)]++] = (uint16_t)(u - c * kPbChunk);
+            }
+        }
+    }
+    // scatter work: the value-slot group of every source-slot group, and
+    // contiguous group ranges of at most kPbScatterItem slots per chunk
+    const int64_t groups = slots / kPbVec;
+    std::vector<int32_t> vgrp((size_t)std::max<int64_t>(groups, 1));
+    for (int64_t c = 0; c < S; ++c)
+        for (int64_t b = 0; b < B; ++b) {
+            const int32_t len = cnt[(size_t)(b * S + c)];
+            const int32_t sg = soff[(size_t)(c * B + b)] / kPbVec, vg = voff[(size_t)(b * S + c)] / kPbVec;
+            for (int32_t o = 0; o < len / kPbVec; ++o) vgrp[(size_t)(sg + o)] = vg + o;
+        }
+    std::vector<PbScatter> scat;
+    for (int64_t c = 0; c < S; ++c) {
+        const int32_t c0 = soff[(size_t)(c * B)] / kPbVec;
+        const int32_t c1 = (c + 1 < S) ? soff[(size_t)((c + 1) * B)] / kPbVec : (int32_t)groups;
+        for (int32_t g0 = c0; g0 < c1; g0 += kPbScatterItem / kPbVec)
+            scat.push_back(PbScatter{(int32_t)c, g0, std::min(c1, g0 + kPbScatterItem / kPbVec), 0});
+    }
+    // sum work: a sub-bin per item, or pieces of at most kPbItem slots whose
+    // row sums go through hub_part (rows `pieces` apart) to the reduce launch
+    std::vector<PbSum> sums;
+    std::vector<HubRow> hrows;
+    int64_t parts = 0;
+    for (int64_t b = 0; b < B; ++b) {
+        const int32_t e0 = voff[(size_t)(b * S)];
+        const int32_t e1 = (b + 1 < B) ? voff[(size_t)((b + 1) * S)] : (int32_t)slots;
+        const int32_t r0 = (int32_t)(b * kPbRows);
+        const int32_t nr = std::min<int32_t>(kPbRows, n - r0);
+        if (e1 - e0 <= kPbItem) {
+            sums.push_back(PbSum{(int32_t)b, e0, e1, -1, 1, 0, 0, 0});
+            continue;
+        }
+        const int32_t np = (e1 - e0 + kPbItem - 1) / kPbItem;
+        if (parts + (int64_t)nr * np > INT32_MAX) {
+            set_last_error("propagation-blocked layout: too many split rows");
+            return EGX_UNSUPPORTED_GRAPH;
+        }
+        const int64_t vecs = (e1 - e0) / kPbVec;
+        for (int32_t q = 0; q < np; ++q) {
+            const int32_t a0 = e0 + (int32_t)((vecs * q) / np) * kPbVec;
+            const int32_t a1 = e0 + (int32_t)((vecs * (q + 1)) / np) * kPbVec;
+            sums.push_back(PbSum{(int32_t)b, a0, a1, q, np, (int32_t)parts, 0, 0});
+        }
+        for (int32_t i = 0; i < nr; ++i) hrows.push_back(HubRow{r0 + i, (int32_t)(parts + (int64_t)i * np), np, 0});
+        parts += (int64_t)nr * np;
+    }
+    // every slot range the kernels will touch, checked before any upload
+    bool ok = true;
+    for (int64_t k = 0; k < groups; ++k) ok = ok && vgrp[(size_t)k] >= 0 && vgrp[(size_t)k] < groups;
+    for (const PbScatter& it : scat)
+        ok = ok && it.chunk >= 0 && (int64_t)it.chunk * kPbChunk < n && it.g0 >= 0 && it.g0 < it.g1 &&
+             it.g1 <= groups;
+    for (const PbSum& it : sums) {
+        ok = ok && it.sub_bin >= 0 && (int64_t)it.sub_bin * kPbRows < n && 0 <= it.e0 && it.e0 <= it.e1 &&
+             (int64_t)it.e1 <= slots && it.e0 % kPbVec == 0 && it.e1 % kPbVec == 0;
+        if (ok && it.piece >= 0) {
+            // the last row's slot of this piece: part0 + (nr - 1) * pieces + piece
+            const int64_t nr = std::min<int64_t>(kPbRows, (int64_t)n - (int64_t)it.sub_bin * kPbRows);
+            ok = it.piece < it.pieces && it.part0 >= 0 && (int64_t)it.part0 + (nr - 1) * it.pieces + it.piece < parts;
+        }
+    }
+    for (const HubRow& h : hrows) ok = ok && h.row >= 0 && h.row < n && h.first >= 0 && (int64_t)h.first + h.nseg <= parts;
+    if (!ok) {
+        set_last_error("propagation-blocked layout failed its range check");
+        return EGX_UNKNOW_ERROR;
+    }
+    EGX_TRY(g.pb_src.alloc((size_t)slots * 2));
+    EGX_TRY(g.pb_dst.alloc((size_t)slots * 2));
+    EGX_TRY(g.pb_vals.alloc((size_t)slots * 4));
+    EGX_TRY(g.pb_vgrp.alloc(vgrp.size() * 4));
+    EGX_TRY(g.pb_scatter.alloc(std::max<size_t>(scat.size(), 1) * sizeof(PbScatter)));
+    EGX_TRY(g.pb_sum.alloc(std::max<size_t>(sums.size(), 1) * sizeof(PbSum)));
+    EGX_TRY(g.pb_hrows.alloc(std::max<size_t>(hrows.size(), 1) * sizeof(HubRow)));
+    EGX_TRY(g.pb_hub_part.alloc((size_t)std::max<int64_t>(parts, 1) * 8));
+    EGX_HIP_TRY(hipMemcpy(g.pb_src.p, src16.data(), (size_t)slots * 2, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(g.pb_dst.p, dst16.data(), (size_t)slots * 2, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(g.pb_vgrp.p, vgrp.data(), vgrp.size() * 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    if (!scat.empty())
+        EGX_HIP_TRY(hipMemcpy(g.pb_scatter.p, scat.data(), scat.size() * sizeof(PbScatter), hipMemcpyHostToDevice),
+                    EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(g.pb_sum.p, sums.data(), sums.size() * sizeof(PbSum), hipMemcpyHostToDevice),
+                EGX_DEVICE_ERR);
+    if (!hrows.empty())
+        EGX_HIP_TRY(hipMemcpy(g.pb_hrows.p, hrows.data(), hrows.size() * sizeof(HubRow), hipMemcpyHostToDevice),
+                    EGX_DEVICE_ERR);
+    g.n_pb_scatter = (int64_t)scat.size();
+    g.n_pb_sum = (int64_t)sums.size();
+    g.n_pb_hrows = (int64_t)hrows.size();
+    g.pb_ready = true;
+    return EGX_SUCC;
+}
+
 int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
     BinCfg cfg = cfg_in;
     if (cfg.thread_row_max < 0) cfg.thread_row_max = 0;
@@ -323,6 +484,7 @@ int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
     if (cfg.hub_segment < 256) cfg.hub_segment = 256;
     if (cfg.reorder != 0) cfg.reorder = 1;
     if (cfg.tile_sort != 0) cfg.tile_sort = 1;
+    if (cfg.blocked != 0) cfg.blocked = 1;
     if (!g.layout_built || g.layout_hub != cfg.wave_row_max || g.layout_reorder != cfg.reorder)
         EGX_TRY(build_layout(g, cfg));
     g.bins = cfg;
@@ -382,7 +544,10 @@ int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
         if (arcs > (int64_t)cfg.thread_row_max * r) g.rows_wave += r;
         else g.rows_thread += r;
     }
-    return build_tile_sort(g, cfg);
+    EGX_TRY(build_tile_sort(g, cfg));
+    const int rc = build_blocked(g, cfg);
+    if (rc != EGX_SUCC) g.bins.blocked = 0;  // the pull layout stays usable
+    return rc;
 }
 
 // out_w[u] = 1: adjacency values keep alpha * w undivided.

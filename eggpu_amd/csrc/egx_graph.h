@@ -84,6 +84,39 @@ struct BinCfg {
     int32_t reorder = 1;          // relabel by out-degree (hub rows always first)
     int32_t tile_sort = 0;        // gather each tile's arcs in source order (LDS scatter back to row order)
     int32_t hot_sources = 0;      // > 0: sources past this label are gathered non-temporally
+    int32_t blocked = 0;          // propagation-blocked iterations (unweighted, FP32)
+};
+
+// Propagation-blocked iteration (bins.blocked).  Sources are cut into chunks
+// of kPbChunk labels and destination rows into sub-bins of kPbRows rows.
+// Every arc has a slot in a value array grouped by (sub-bin, chunk) and, in
+// the same order inside each such run, (row, source).  Runs are padded to a
+// multiple of kPbVec slots (source kPbChunk, a zero LDS entry; row 0, value
+// 0), so every run starts 16-byte aligned in both index arrays.  A scatter
+// launch stages one chunk of s = x/deg in LDS and writes a contiguous range
+// of that chunk's source slots, kPbVec at a time, each group to the value
+// slot its run gives it (pb_vgrp); a sum launch gives each sub-bin (or piece
+// of a large one, at most kPbItem slots) to one wavefront, which adds its
+// slots into FP64 LDS row accumulators in slot order.  Sub-bins cut into
+// several pieces hand per-piece row sums to the reduce launch as hub rows.
+constexpr int32_t kPbChunk = 16384;
+constexpr int32_t kPbRows = 2048;
+constexpr int32_t kPbVec = 8;
+constexpr int32_t kPbScatterItem = 262144;
+constexpr int32_t kPbItem = 262144;
+
+struct PbScatter {
+    int32_t chunk;
+    int32_t g0, g1;  // kPbVec-slot groups [g0, g1) of the chunk-major source array, all of this chunk
+    int32_t pad;
+};
+struct PbSum {
+    int32_t sub_bin;
+    int32_t e0, e1;  // value slots [e0, e1)
+    int32_t piece;   // -1: the whole sub-bin; else this piece's index among `pieces`
+    int32_t pieces;
+    int32_t part0;   // hub_part index of row 0 of the sub-bin (rows are `pieces` apart)
+    int32_t pad0, pad1;
 };
 
 struct Graph {
@@ -103,6 +136,14 @@ struct Graph {
     // adjacency graph built with weights: the weighted kernels gather
     // fl32(alpha * w) (eigenvector centrality); false: unit weights
     bool adjacency_weights = false;
+    // propagation-blocked layout (bins.blocked): pb_src uint16 chunk-local
+    // source per slot in (chunk, sub-bin, row, source) order; pb_dst uint16
+    // sub-bin-local row per slot in (sub-bin, chunk, row, source) order;
+    // pb_vals the per-iteration terms in the second order; pb_vgrp int32 per
+    // kPbVec-slot group of pb_src: the value-slot group it is written to
+    bool pb_ready = false;
+    DevBuf pb_src, pb_dst, pb_vals, pb_vgrp, pb_scatter, pb_sum, pb_hrows, pb_hub_part;
+    int64_t n_pb_scatter = 0, n_pb_sum = 0, n_pb_hrows = 0;
     double alpha_vals = 0.0;   // alpha baked into it_val
     double alpha_orig = 0.0;   // alpha baked into in_val
     int32_t max_in_degree = 0;

@@ -41,6 +41,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "egx_graph.h"
@@ -382,6 +383,144 @@ __global__ __launch_bounds__(kWave) void k_pr_final(IterArgs<T> a) {
     reduce_final_wave<T>(a);
 }
 
+// Propagation-blocked iteration (unweighted, FP32 iterates): the layout of
+// PbPiece / PbScatter / PbSum in egx_graph.h.  No random global gathers:
+// sources are read in chunks into LDS and their terms written out as runs,
+// then every destination sub-bin sums its runs in LDS.
+struct PbArgs {
+    const uint16_t* src;
+    const uint16_t* dst;
+    float* vals;
+    const int32_t* vgrp;
+    const PbScatter* scatter;
+    const PbSum* sums;
+};
+
+constexpr int kPbThreads = 512;
+
+// One scatter item: s of its chunk into LDS (one coalesced read; entry
+// kPbChunk is the zero that padding slots read), then the item's groups of
+// kPbVec source slots, one per lane: a 16-byte index load, kPbVec LDS reads
+// and two 16-byte stores to the value-slot group vgrp gives, kPbGroups
+// groups in flight per lane.
+constexpr int kPbGroups = 4;
+typedef uint32_t pb_u32x4 __attribute__((ext_vector_type(4)));
+typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs pb) {
+    __shared__ float sc[kPbChunk + 4];
+    if (a.state->done) return;
+    bind(a);
+    const PbScatter it = pb.scatter[blockIdx.x];
+    const int32_t base = it.chunk * kPbChunk;
+    const int32_t cnt = min(kPbChunk, a.n - base);
+    for (int32_t i = threadIdx.x; i < cnt; i += kPbThreads) sc[i] = (float)a.s_in[base + i];
+    if (threadIdx.x == 0) sc[kPbChunk] = 0.0f;
+    __syncthreads();
+    const pb_u32x4* src = reinterpret_cast<const pb_u32x4*>(pb.src);
+    pb_f32x4* out = reinterpret_cast<pb_f32x4*>(pb.vals);
+    for (int32_t g0 = it.g0 + (int32_t)threadIdx.x; g0 < it.g1; g0 += kPbGroups * kPbThreads) {
+        pb_u32x4 u[kPbGroups];
+        int32_t vg[kPbGroups];
+#pragma unroll
+        for (int q = 0; q < kPbGroups; ++q) {
+            const int32_t gi = g0 + q * kPbThreads;
+            const bool ok = gi < it.g1;
+            u[q] = ok ? __builtin_nontemporal_load(&src[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
+            vg[q] = ok ? __builtin_nontemporal_load(&pb.vgrp[gi]) : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < kPbGroups; ++q) {
+            if (vg[q] >= 0) {
+                const uint32_t w[4] = {u[q].x, u[q].y, u[q].z, u[q].w};
+                float v[8];
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    v[2 * h] = sc[w[h] & 0xffffu];
+                    v[2 * h + 1] = sc[w[h] >> 16];
+                }
+                out[2 * (size_t)vg[q]] = pb_f32x4{v[0], v[1], v[2], v[3]};
+                out[2 * (size_t)vg[q] + 1] = pb_f32x4{v[4], v[5], v[6], v[7]};
+            }
+        }
+    }
+}
+
+// One sum item (one wavefront): its value slots added into FP64 LDS row
+// accumulators in slot order (the pull kernel also sums rows in FP64), then
+// the sub-bin's rows finalised (or, for a piece of a split sub-bin, its row
+// sums left in hub_part for the reduce launch).  Only this wavefront touches
+// its accumulators, and a lane adds its kPbVec slots in slot order (each
+// stretch of one row summed in registers first), so the order of the adds
+// is fixed.  Padding slots add 0 to row 0.
+template <typename T>
+__global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
+    __shared__ double acc[kPbRows];
+    if (a.state->done) return;
+    bind(a);
+    const PbSum it = pb.sums[blockIdx.x];
+    const int l = threadIdx.x;
+    for (int32_t i = l; i < kPbRows; i += kWave) acc[i] = 0.0;
+    __syncthreads();
+    const pb_u32x4* dst = reinterpret_cast<const pb_u32x4*>(pb.dst + it.e0);
+    const pb_f32x4* val = reinterpret_cast<const pb_f32x4*>(pb.vals + it.e0);
+    const int32_t nv = (it.e1 - it.e0) / kPbVec;
+    for (int32_t g0 = 0; g0 < nv; g0 += kPbGroups * kWave) {
+        pb_u32x4 d[kPbGroups];
+        pb_f32x4 va[kPbGroups], vb[kPbGroups];
+#pragma unroll
+        for (int q = 0; q < kPbGroups; ++q) {
+            const int32_t gi = g0 + q * kWave + l;
+            const bool ok = gi < nv;
+            d[q] = ok ? __builtin_nontemporal_load(&dst[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
+            va[q] = ok ? __builtin_nontemporal_load(&val[2 * gi]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+            vb[q] = ok ? __builtin_nontemporal_load(&val[2 * gi + 1]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int q = 0; q < kPbGroups; ++q) {
+            if (g0 + q * kWave + l < nv) {
+                const uint32_t w[4] = {d[q].x, d[q].y, d[q].z, d[q].w};
+                const uint32_t r[8] = {w[0] & 0xffffu, w[0] >> 16, w[1] & 0xffffu, w[1] >> 16,
+                                       w[2] & 0xffffu, w[2] >> 16, w[3] & 0xffffu, w[3] >> 16};
+                const float v[8] = {va[q].x, va[q].y, va[q].z, va[q].w, vb[q].x, vb[q].y, vb[q].z, vb[q].w};
+                // slots of one row are consecutive inside a run: add each
+                // stretch in registers, one LDS add per stretch
+                double run = (double)v[0];
+#pragma unroll
+                for (int j = 0; j < 7; ++j) {
+                    if (r[j + 1] == r[j]) {
+                        run += (double)v[j + 1];
+                    } else {
+                        atomicAdd(&acc[r[j]], run);
+                        run = (double)v[j + 1];
+                    }
+                }
+                atomicAdd(&acc[r[7]], run);
+            }
+        }
+    }
+    __syncthreads();
+    const int32_t r0 = it.sub_bin * kPbRows;
+    const int32_t nr = min(kPbRows, a.n - r0);
+    double err = 0.0, sum = 0.0, dang = 0.0;
+    if (it.piece < 0) {
+        const double jump = jump_of(a);
+        for (int32_t i = l; i < nr; i += kWave) finalize_row<T, false>(a, r0 + i, acc[i], jump, err, sum, dang);
+        err = wave_sum(err);
+        sum = wave_sum(sum);
+        dang = wave_sum(dang);
+    } else {
+        for (int32_t i = l; i < nr; i += kWave)
+            a.hub_part[(size_t)it.part0 + (size_t)i * it.pieces + it.piece] = acc[i];
+    }
+    if (l == 0) {
+        a.partials[blockIdx.x] = err;
+        a.partials[(size_t)a.grid + blockIdx.x] = sum;
+        a.partials[2 * (size_t)a.grid + blockIdx.x] = dang;
+    }
+}
+
 // Many graphs, one launch: block b works for graph blk_graph[b], as its
 // local block blk_local[b]; blk_graph[b] < 0 is padding.
 template <typename T, bool W>
@@ -693,7 +832,40 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
     hipStream_t s = g.stream;
     IterArgs<T> a;
     EGX_TRY(prepare_graph<T>(g, p, s, a));
+    // propagation-blocked iterations: FP32 iterates on an unweighted graph
+    // with the blocked layout; split sub-bins go through the hub-row reduce
+    constexpr bool kBlockable = std::is_same<T, float>::value && !W;
+    const bool blocked = kBlockable && g.pb_ready;
+    PbArgs pb{};
+    if (blocked) {
+        EGX_TRY(ensure_workspace(g, (uint32_t)g.n_pb_sum));
+        a.partials = g.partials.as<double>();
+        a.hrows = g.pb_hrows.as<HubRow>();
+        a.hub_part = g.pb_hub_part.as<double>();
+        a.n_hrows = g.n_pb_hrows;
+        a.nblk_h = 0;
+        a.grid = (uint32_t)g.n_pb_sum;
+        a.n_red = (uint32_t)std::min<int64_t>(
+            kMaxRed, std::max<int64_t>(1, (a.n_hrows + a.grid + kRedItems - 1) / kRedItems));
+        pb.src = g.pb_src.as<uint16_t>();
+        pb.dst = g.pb_dst.as<uint16_t>();
+        pb.vals = g.pb_vals.as<float>();
+        pb.vgrp = g.pb_vgrp.as<int32_t>();
+        pb.scatter = g.pb_scatter.as<PbScatter>();
+        pb.sums = g.pb_sum.as<PbSum>();
+    }
     const uint32_t grid = a.grid;
+    // the SpMV of one iteration: pull kernel, or scatter + sum
+    auto launch_spmv = [&]() {
+        if constexpr (kBlockable) {
+            if (blocked) {
+                k_pb_scatter<T><<<(uint32_t)g.n_pb_scatter, kPbThreads, 0, s>>>(a, pb);
+                k_pb_sum<T><<<(uint32_t)g.n_pb_sum, kWave, 0, s>>>(a, pb);
+                return;
+            }
+        }
+        k_pr_spmv<T, W><<<grid, kThreads, 0, s>>>(a);
+    };
 
     // eigenvector mode finds iteration k's delta in launch k+1
     const int32_t max_launch = p.max_iter + (p.eig_delta ? 1 : 0);
@@ -713,7 +885,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
             if (!cap.exec) {
                 EGX_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
                 for (int32_t q = 0; q < todo; ++q) {
-                    k_pr_spmv<T, W><<<grid, kThreads, 0, s>>>(a);
+                    launch_spmv();
                     k_pr_reduce<T, W><<<a.n_red, kReduceThreads, 0, s>>>(a);
                     k_pr_final<T><<<1, kWave, 0, s>>>(a);
                 }
@@ -725,7 +897,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
         } else {
             for (int32_t q = 0; q < todo; ++q, ++launched) {
                 if (p.time_each_launch) EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched], s), EGX_DEVICE_ERR);
-                k_pr_spmv<T, W><<<grid, kThreads, 0, s>>>(a);
+                launch_spmv();
                 EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
                 if (p.time_each_launch) {
                     EGX_HIP_TRY(hipEventRecord(timer.ev[4 * launched + 1], s), EGX_DEVICE_ERR);

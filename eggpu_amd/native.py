@@ -65,7 +65,7 @@ class GraphInfo(ctypes.Structure):
         ("hub_rows_first", ctypes.c_int32),
         ("tile_sort", ctypes.c_int32),
         ("hot_sources", ctypes.c_int32),
-        ("pad0", ctypes.c_int32),
+        ("blocked", ctypes.c_int32),
         ("rows_thread", ctypes.c_int64),
         ("rows_wave", ctypes.c_int64),
         ("rows_hub", ctypes.c_int64),
@@ -89,6 +89,7 @@ class BinOpts(ctypes.Structure):
         ("reorder", ctypes.c_int32),
         ("tile_sort", ctypes.c_int32),
         ("hot_sources", ctypes.c_int32),
+        ("blocked", ctypes.c_int32),
     ]
 
 
@@ -390,12 +391,12 @@ class DeviceGraph:
         return int(self.info()["n"])
 
     def set_bins(self, thread_row_max=0, wave_row_max=0, hub_segment=0, tile_items=0, reorder=-1, tile_sort=-1,
-                 hot_sources=0):
-        """Re-bin rows (values <= 0 keep the current setting; reorder and
-        tile_sort < 0 keep theirs; hot_sources 0 keeps it, < 0 turns
-        non-temporal cold gathers off)."""
+                 hot_sources=0, blocked=-1):
+        """Re-bin rows (values <= 0 keep the current setting; reorder,
+        tile_sort and blocked < 0 keep theirs; hot_sources 0 keeps it, < 0
+        turns non-temporal cold gathers off)."""
         o = BinOpts(int(thread_row_max), int(wave_row_max), int(hub_segment), int(tile_items), int(reorder),
-                    int(tile_sort), int(hot_sources))
+                    int(tile_sort), int(hot_sources), int(blocked))
         _check(lib().egx_graph_set_bins(self.handle, ctypes.byref(o)))
 
     def download_csc(self):

@@ -105,7 +105,7 @@ typedef struct {
     int32_t hub_rows_first;  /* hub rows, stored first in the iteration layout */
     int32_t tile_sort;       /* 1: tile arcs gathered in source order */
     int32_t hot_sources;     /* gathers past this relabelled id are non-temporal (0: none) */
-    int32_t pad0;
+    int32_t blocked;         /* 1: iterations run propagation-blocked (egx_bin_opts.blocked) */
     int64_t rows_thread;     /* rows reduced one thread per row */
     int64_t rows_wave;       /* rows reduced one wavefront per row */
     int64_t rows_hub;        /* rows of the hub bin (segmented, one workgroup per segment) */
@@ -131,6 +131,10 @@ typedef struct {
     int32_t tile_sort;      /* 1: gather tile arcs in source order, 0: row order, <0: keep setting */
     int32_t hot_sources;    /* > 0: gather sources past this relabelled id non-temporally;
                                0: keep setting; < 0: all gathers cached */
+    int32_t blocked;        /* 1: propagation-blocked iterations (unweighted graphs, FP32
+                               iterates): sources are scattered by destination bin
+                               through LDS, then summed per bin in LDS, with no
+                               random gathers; 0: pull SpMV; < 0: keep setting */
 } egx_bin_opts;
 
 typedef struct {
