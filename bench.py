@@ -13,7 +13,8 @@ synchronisation on both sides.
 Prints one JSON line.  ``roofline`` uses the algorithmic bytes of one
 iteration, 4m + 16n (int32 column index per arc; row pointer, out-degree,
 x read and x' write per node), over the mean duration of the SpMV launch
-measured with hipEvents on the graph's stream.  ``cpu_baseline`` times the
+measured with hipEvents on the graph's stream (the scatter + sum launch pair
+when the tuner picks the propagation-blocked iteration).  ``cpu_baseline`` times the
 OpenMP push of pagerank.cpp:21-122 (oracle/egx_oracle.c) on this host, on
 the same graph, for a few iterations.
 """
@@ -37,10 +38,12 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel_name_part):
-    """HBM bytes per SpMV launch from the committed rocprofv3 --pmc summary
-    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_rmat24.json")
+def read_traffic(blocked):
+    """HBM bytes per SpMV launch (per scatter + sum pair when blocked) from
+    the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy-kernel calibration)."""
+    name = "pmc_traffic_rmat24_blocked.json" if blocked else "pmc_traffic_rmat24.json"
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, None
     with open(path) as fh:
@@ -166,7 +169,8 @@ def main():
     reduce_mean = st_t["reduce_seconds"] / st_t["spmv_launches"]
     bytes_iter = 4 * m + 16 * n
     achieved = bytes_iter / spmv_mean
-    traffic, traffic_src = read_traffic("k_pr_spmv")
+    blocked = bool(info["blocked"])
+    traffic, traffic_src = read_traffic(blocked)
 
     # tol = 1e-6 solve (reference semantics) and the FP64 variant, beside
     pr, it_tol, _ = g.pagerank(0.85, 100, 1e-6)
@@ -222,7 +226,7 @@ def main():
             "iterate_dtype": "f32",
             "reductions": "f64 (dangling mass, L1 residual, teleport, row sums)",
             "bins": {k: info[k] for k in ("thread_row_max", "wave_row_max", "hub_segment", "tile_items", "tile_sort",
-                                                  "hot_sources")},
+                                                  "hot_sources", "blocked")},
             "autotune": tuned,
             "hub_rows": info["rows_hub"],
             "tiles": info["tiles"],
@@ -241,7 +245,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
             "traffic": traffic,
-            "kernel": "k_pr_spmv",
+            "kernel": "k_pb_scatter + k_pb_sum" if blocked else "k_pr_spmv",
             "algorithmic_bytes_per_launch": bytes_iter,
             "traffic_source": traffic_src,
         },

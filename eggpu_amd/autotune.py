@@ -10,7 +10,10 @@ The knobs are those of egx_graph_set_bins:
 * tile_items     -- rows + arcs per merge-path tile (LDS-staged);
 * tile_sort      -- gather a tile's arcs in source order (1) or row order (0);
 * hot_sources    -- sources past this relabelled id are gathered with
-  non-temporal loads, keeping the hot prefix resident (0: off).
+  non-temporal loads, keeping the hot prefix resident (0: off);
+* blocked        -- propagation-blocked iterations instead of the pull
+  SpMV (unweighted graphs, FP32 iterates): tried once, after the pull knobs,
+  on the best pull setting's relabelling.
 
 ``tune`` runs a coordinate descent over candidate values, timing the SpMV and
 reduce launches of a few fixed iterations with hipEvents on the graph's own
@@ -29,7 +32,8 @@ import time
 
 from . import native
 
-DEFAULT = dict(thread_row_max=12, wave_row_max=2048, hub_segment=8192, tile_items=5888, tile_sort=0, hot_sources=0)
+DEFAULT = dict(thread_row_max=12, wave_row_max=2048, hub_segment=8192, tile_items=5888, tile_sort=0, hot_sources=0,
+               blocked=0)
 
 SPACE = {
     "thread_row_max": [4, 8, 12, 16, 24, 48],
@@ -80,7 +84,8 @@ def tune(g, precision=native.PRECISION_FP32, iters=5, passes=2, space=None, star
     Returns (best cfg, seconds per iteration, trials).  Leaves ``g`` binned
     with the best setting."""
     space = space or SPACE
-    cur = dict(start or DEFAULT)
+    cur = dict(DEFAULT, **(start or {}))
+    cur["blocked"] = 0
     trials = []
     seen = {}
 
@@ -108,6 +113,12 @@ def tune(g, precision=native.PRECISION_FP32, iters=5, passes=2, space=None, star
                     best, cur, improved = c, cand, True
         if not improved:
             break
+    info = g.info()
+    if not info["weighted"] and precision == native.PRECISION_FP32 and info["m"] > 0:
+        cand = dict(cur, blocked=1)
+        c = cost(cand)
+        if c < best * 0.99:
+            best, cur = c, cand
     set_bins(g, cur)
     return cur, best, trials
 

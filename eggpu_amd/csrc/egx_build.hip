@@ -318,7 +318,9 @@ int build_tile_sort(Graph& g, const BinCfg& cfg) {
 // the value slots and by (chunk, sub-bin) for the scatter's source list.
 // Both keep CSC order (row, then source) inside each (sub-bin, chunk) run,
 // so run k of chunk c fills value slots [val0, val0 + len) in order.
-int build_blocked(Graph& g, const BinCfg& cfg) {
+int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
+    // the blocked layout depends only on the iteration CSC
+    if (cfg.blocked && g.pb_ready && !layout_changed) return EGX_SUCC;
     g.pb_ready = false;
     for (DevBuf* b : {&g.pb_src, &g.pb_dst, &g.pb_vals, &g.pb_vgrp, &g.pb_scatter, &g.pb_sum, &g.pb_hrows,
                       &g.pb_hub_part})
@@ -485,8 +487,9 @@ int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
     if (cfg.reorder != 0) cfg.reorder = 1;
     if (cfg.tile_sort != 0) cfg.tile_sort = 1;
     if (cfg.blocked != 0) cfg.blocked = 1;
-    if (!g.layout_built || g.layout_hub != cfg.wave_row_max || g.layout_reorder != cfg.reorder)
-        EGX_TRY(build_layout(g, cfg));
+    const bool layout_changed =
+        !g.layout_built || g.layout_hub != cfg.wave_row_max || g.layout_reorder != cfg.reorder;
+    if (layout_changed) EGX_TRY(build_layout(g, cfg));
     g.bins = cfg;
     hipStream_t s = g.stream;
     const int32_t n = g.n;
@@ -545,7 +548,7 @@ int rebuild_bins(Graph& g, const BinCfg& cfg_in) {
         else g.rows_thread += r;
     }
     EGX_TRY(build_tile_sort(g, cfg));
-    const int rc = build_blocked(g, cfg);
+    const int rc = build_blocked(g, cfg, layout_changed);
     if (rc != EGX_SUCC) g.bins.blocked = 0;  // the pull layout stays usable
     return rc;
 }

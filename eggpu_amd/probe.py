@@ -174,25 +174,35 @@ def cmd_traffic(a):
 
     cf, _, _ = pick(fetch, "k_probe_copy")
     cw, _, _ = pick(write, "k_probe_copy")
-    kf, nf, kname = pick(fetch, a.kernel)
-    kw, nw, _ = pick(write, a.kernel)
-    if None in (cf, cw, kf, kw):
-        raise SystemExit(f"missing counters: copy fetch {cf} write {cw}; {a.kernel} fetch {kf} write {kw}")
+    if None in (cf, cw):
+        raise SystemExit(f"missing copy counters: fetch {cf} write {cw}")
     f_scale = a.bytes / cf  # bytes per counter unit, from the copy's known read
     w_scale = a.bytes / cw
+    # --kernel a,b: the launches of one iteration, summed (e.g. a scatter + sum pair)
+    per = []
+    for part in [k for k in a.kernel.split(",") if k]:
+        kf, nf, kname = pick(fetch, part)
+        kw, nw, _ = pick(write, part)
+        if None in (kf, kw):
+            raise SystemExit(f"missing counters for {part}: fetch {kf} write {kw}")
+        per.append({"kernel": kname, "dispatches": {"fetch": nf, "write": nw}, "FETCH_SIZE_mean": kf,
+                    "WRITE_SIZE_mean": kw, "hbm_read_bytes": kf * f_scale, "hbm_write_bytes": kw * w_scale})
+    rd = sum(p["hbm_read_bytes"] for p in per)
+    wr = sum(p["hbm_write_bytes"] for p in per)
     out = {
-        "kernel": kname,
-        "dispatches": {"fetch": nf, "write": nw},
+        "kernel": " + ".join(p["kernel"] for p in per),
+        "dispatches": per[0]["dispatches"],
         "copy_bytes": a.bytes,
         "copy_FETCH_SIZE": cf,
         "copy_WRITE_SIZE": cw,
         "fetch_scale_bytes_per_unit": f_scale,
         "write_scale_bytes_per_unit": w_scale,
-        "kernel_FETCH_SIZE_mean": kf,
-        "kernel_WRITE_SIZE_mean": kw,
-        "hbm_read_bytes_per_launch": kf * f_scale,
-        "hbm_write_bytes_per_launch": kw * w_scale,
-        "hbm_bytes_per_launch": kf * f_scale + kw * w_scale,
+        "kernel_FETCH_SIZE_mean": sum(p["FETCH_SIZE_mean"] for p in per),
+        "kernel_WRITE_SIZE_mean": sum(p["WRITE_SIZE_mean"] for p in per),
+        "hbm_read_bytes_per_launch": rd,
+        "hbm_write_bytes_per_launch": wr,
+        "hbm_bytes_per_launch": rd + wr,
+        "per_kernel": per if len(per) > 1 else None,
         "fetch_db": a.fetch,
         "write_db": a.write,
     }
@@ -229,7 +239,7 @@ def main(argv=None):
     p.add_argument("--fetch", required=True)
     p.add_argument("--write", required=True)
     p.add_argument("--bytes", type=int, default=1 << 30)
-    p.add_argument("--kernel", default="k_pr_spmv")
+    p.add_argument("--kernel", default="k_pr_spmv", help="kernel name part, or several joined by commas")
     p.add_argument("--out")
     a = ap.parse_args(argv)
     {"copy": cmd_copy, "gather": cmd_gather, "calib": cmd_calib, "stats": cmd_stats,

@@ -8,14 +8,15 @@ from eggpu_amd import autotune
 
 class FakeGraph:
     """Cost is smallest at thread_row_max=16, wave_row_max=1024,
-    hub_segment=4096, tile_items=4416."""
+    hub_segment=4096, tile_items=4416, and with the blocked iteration."""
 
     best = dict(thread_row_max=16, wave_row_max=1024, hub_segment=4096, tile_items=4416, tile_sort=1,
-                hot_sources=1 << 19)
+                hot_sources=1 << 19, blocked=1)
 
-    def __init__(self):
+    def __init__(self, weighted=0):
         self.cfg = dict(autotune.DEFAULT)
         self.calls = 0
+        self.weighted = weighted
 
     def set_bins(self, **cfg):
         assert cfg["tile_items"] + cfg["wave_row_max"] <= 7936
@@ -27,14 +28,14 @@ class FakeGraph:
         self.calls += 1
         c = 1e-3
         for k, v in self.best.items():
-            if k in ("tile_sort", "hot_sources"):
+            if k in ("tile_sort", "hot_sources", "blocked"):
                 c += 1e-4 * (self.cfg[k] != v)
             else:
                 c += 1e-4 * abs(np.log2(self.cfg[k] / v))
         return None, {"spmv_seconds": c * iters, "reduce_seconds": 0.0, "spmv_launches": iters}
 
     def info(self):
-        return {"n": 1 << 20, "m": 16 << 20, "weighted": 0, "max_in_degree": 50000}
+        return {"n": 1 << 20, "m": 16 << 20, "weighted": self.weighted, "max_in_degree": 50000}
 
 
 def test_coordinate_descent_finds_the_optimum():
@@ -44,6 +45,12 @@ def test_coordinate_descent_finds_the_optimum():
     assert g.cfg == cfg
     assert all(t[0]["tile_items"] + t[0]["wave_row_max"] <= 7936 for t in trials)
     assert sec == pytest.approx(1e-3)
+    # the blocked iteration is tried once, last, and never on weighted graphs
+    assert [t[0]["blocked"] for t in trials].count(1) == 1 and trials[-1][0]["blocked"] == 1
+    gw = FakeGraph(weighted=1)
+    cfg_w, _, trials_w = autotune.tune(gw, passes=3)
+    assert cfg_w["blocked"] == 0 and all(t[0]["blocked"] == 0 for t in trials_w)
+    assert cfg_w == dict(FakeGraph.best, blocked=0)
 
 
 def test_cache_round_trip_and_signature(tmp_path):
