@@ -245,7 +245,7 @@ size_t Graph::device_bytes() const {
                             &p_outw, &p_deg,  &p_nval, &tiles,  &t_col,  &t_pos,  &t_val,  &hsegs,
                             &hrows,  &hub_part, &x[0],  &x[1],
                             &s[0],   &s[1],   &state,  &partials, &red_part, &out,    &tele,
-                            &pb_src, &pb_dst, &pb_vals, &pb_vgrp, &pb_scatter, &pb_sum, &pb_hrows,
+                            &pb_src, &pb_dst, &pb_vals, &pb_vgrp, &pb_mask, &pb_scatter, &pb_sum, &pb_hrows,
                             &pb_hub_part};
     size_t total = 0;
     for (const DevBuf* b : bufs) total += b->bytes;

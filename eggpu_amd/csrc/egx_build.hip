@@ -314,17 +314,20 @@ int build_tile_sort(Graph& g, const BinCfg& cfg) {
 }
 
 // Propagation-blocked layout of the iteration CSC (it_row, it_col), built on
-// the host by two stable counting sorts of the arcs: by (sub-bin, chunk) for
-// the value slots and by (chunk, sub-bin) for the scatter's source list.
-// Both keep CSC order (row, then source) inside each (sub-bin, chunk) run,
-// so run k of chunk c fills value slots [val0, val0 + len) in order.
+// the host.  Arcs are bucketed by (chunk c, sub-bin b) into runs, in CSC
+// order (row, then source) inside each run, each run padded to a multiple
+// of kPbVec source slots.  Each kPbVec-slot group of a run is cut into
+// stretches of equal rows (bit j of its mask marks a stretch starting at
+// slot j); every stretch has one value slot.  Value slots follow
+// (sub-bin, chunk) order, each run's stretches in group order, padded to a
+// multiple of kPbVec (padding rows 0, values 0: never written).
 int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     // the blocked layout depends only on the iteration CSC
     if (cfg.blocked && g.pb_ready && !layout_changed) return EGX_SUCC;
     g.pb_ready = false;
-    for (DevBuf* b : {&g.pb_src, &g.pb_dst, &g.pb_vals, &g.pb_vgrp, &g.pb_scatter, &g.pb_sum, &g.pb_hrows,
-                      &g.pb_hub_part})
-        b->release();
+    for (DevBuf* d : {&g.pb_src, &g.pb_dst, &g.pb_vals, &g.pb_vgrp, &g.pb_mask, &g.pb_scatter, &g.pb_sum,
+                      &g.pb_hrows, &g.pb_hub_part})
+        d->release();
     g.n_pb_scatter = g.n_pb_sum = g.n_pb_hrows = 0;
     if (!cfg.blocked || g.n <= 0 || g.m <= 0) return EGX_SUCC;
     if (g.weighted) {
@@ -344,72 +347,109 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     EGX_HIP_TRY(hipMemcpy(row.data(), g.it_row, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpy(col.data(), g.it_col, (size_t)m * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
 
-    // run lengths, bin-major (sub-bin b, chunk c) at b*S + c, then padded
+    // run lengths in source slots, chunk-major (chunk c, sub-bin b) at c*B + b
     std::vector<int32_t> cnt((size_t)(S * B), 0);
     for (int32_t r = 0; r < n; ++r) {
-        const int64_t kb = (int64_t)(r / kPbRows) * S;
-        for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e) ++cnt[(size_t)(kb + col[(size_t)e] / kPbChunk)];
+        const int64_t b = r / kPbRows;
+        for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e)
+            ++cnt[(size_t)((col[(size_t)e] / kPbChunk) * B + b)];
     }
-    for (int32_t& k : cnt) k = (k + kPbVec - 1) / kPbVec * kPbVec;
-    // value-slot starts (bin-major) and source-list starts (chunk-major)
-    std::vector<int32_t> voff((size_t)(S * B)), soff((size_t)(S * B));
-    int64_t acc = 0;
+    std::vector<int32_t> soff((size_t)(S * B));
+    int64_t slots = 0;
     for (int64_t k = 0; k < S * B; ++k) {
-        voff[(size_t)k] = (int32_t)acc;
-        acc += cnt[(size_t)k];
+        soff[(size_t)k] = (int32_t)slots;
+        slots += (cnt[(size_t)k] + kPbVec - 1) / kPbVec * kPbVec;
     }
-    const int64_t slots = acc;
     if (slots > INT32_MAX) {
         set_last_error("propagation-blocked layout: too many slots");
         return EGX_UNSUPPORTED_GRAPH;
     }
-    acc = 0;
-    for (int64_t c = 0; c < S; ++c)
-        for (int64_t b = 0; b < B; ++b) {
-            soff[(size_t)(c * B + b)] = (int32_t)acc;
-            acc += cnt[(size_t)(b * S + c)];
-        }
-    // slots, in CSC order within each run; padding slots keep source
-    // kPbChunk (the zero LDS entry) and row 0
-    std::vector<uint16_t> src16((size_t)slots, (uint16_t)kPbChunk), dst16((size_t)slots, 0);
+    const int64_t groups = slots / kPbVec;
+    // source slots in CSC order within each run; the row of every slot
+    // (padding: -1) for the stretch masks
+    std::vector<uint16_t> src16((size_t)slots, (uint16_t)kPbChunk);
+    std::vector<int32_t> srow((size_t)slots, -1);
     {
-        std::vector<int32_t> vcur(voff), scur(soff);
+        std::vector<int32_t> cur(soff);
         for (int32_t r = 0; r < n; ++r) {
             const int64_t b = r / kPbRows;
-            const uint16_t rl = (uint16_t)(r - b * kPbRows);
             for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e) {
                 const int32_t u = col[(size_t)e];
                 const int64_t c = u / kPbChunk;
-                dst16[(size_t)vcur[(size_t)(b * S + c)]++] = rl;
-                src16[(size_t)scur[(size_t)(c * B + b)]++] = (uint16_t)(u - c * kPbChunk);
+                const int32_t p = cur[(size_t)(c * B + b)]++;
+                src16[(size_t)p] = (uint16_t)(u - c * kPbChunk);
+                srow[(size_t)p] = r;
             }
         }
     }
-    // scatter work: the value-slot group of every source-slot group, and
-    // contiguous group ranges of at most kPbScatterItem slots per chunk
-    const int64_t groups = slots / kPbVec;
-    std::vector<int32_t> vgrp((size_t)std::max<int64_t>(groups, 1));
-    for (int64_t c = 0; c < S; ++c)
-        for (int64_t b = 0; b < B; ++b) {
-            const int32_t len = cnt[(size_t)(b * S + c)];
-            const int32_t sg = soff[(size_t)(c * B + b)] / kPbVec, vg = voff[(size_t)(b * S + c)] / kPbVec;
-            for (int32_t o = 0; o < len / kPbVec; ++o) vgrp[(size_t)(sg + o)] = vg + o;
+    row.clear();
+    row.shrink_to_fit();
+    col.clear();
+    col.shrink_to_fit();
+    // stretch masks and value slots per run; padding slots join the stretch
+    // before them (they add the zero LDS entry)
+    std::vector<uint8_t> mask((size_t)std::max<int64_t>(groups, 1), 0);
+    std::vector<int32_t> outs((size_t)(S * B), 0);  // value slots per run, unpadded
+    for (int64_t k = 0; k < S * B; ++k) {
+        const int64_t g0 = soff[(size_t)k] / kPbVec;
+        const int64_t g1 = (k + 1 < S * B ? soff[(size_t)k + 1] : slots) / kPbVec;
+        int32_t o = 0;
+        for (int64_t gi = g0; gi < g1; ++gi) {
+            uint8_t mk = 1;
+            for (int j = 1; j < kPbVec; ++j) {
+                const int32_t rj = srow[(size_t)(gi * kPbVec + j)];
+                if (rj >= 0 && rj != srow[(size_t)(gi * kPbVec + j - 1)]) mk |= (uint8_t)(1u << j);
+            }
+            mask[(size_t)gi] = mk;
+            o += __builtin_popcount(mk);
         }
+        outs[(size_t)k] = o;
+    }
+    // value-slot starts, bin-major (sub-bin b, chunk c), each run padded
+    std::vector<int32_t> voff((size_t)(S * B));  // indexed like cnt: c*B + b
+    int64_t nvals = 0;
+    for (int64_t b = 0; b < B; ++b)
+        for (int64_t c = 0; c < S; ++c) {
+            voff[(size_t)(c * B + b)] = (int32_t)nvals;
+            nvals += (outs[(size_t)(c * B + b)] + kPbVec - 1) / kPbVec * kPbVec;
+        }
+    if (nvals > INT32_MAX) {
+        set_last_error("propagation-blocked layout: too many value slots");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    // first value slot of every group, and the row of every value slot
+    std::vector<int32_t> vgrp((size_t)std::max<int64_t>(groups, 1));
+    std::vector<uint16_t> dst16((size_t)std::max<int64_t>(nvals, 1), 0);
+    for (int64_t k = 0; k < S * B; ++k) {
+        const int64_t b = k % B;
+        const int64_t g0 = soff[(size_t)k] / kPbVec;
+        const int64_t g1 = (k + 1 < S * B ? soff[(size_t)k + 1] : slots) / kPbVec;
+        int32_t o = voff[(size_t)k];
+        for (int64_t gi = g0; gi < g1; ++gi) {
+            vgrp[(size_t)gi] = o;
+            const uint8_t mk = mask[(size_t)gi];
+            for (int j = 0; j < kPbVec; ++j)
+                if (mk & (1u << j)) dst16[(size_t)o++] = (uint16_t)(srow[(size_t)(gi * kPbVec + j)] - b * kPbRows);
+        }
+    }
+    srow.clear();
+    srow.shrink_to_fit();
+    // scatter work: contiguous group ranges of at most kPbScatterItem slots per chunk
     std::vector<PbScatter> scat;
     for (int64_t c = 0; c < S; ++c) {
         const int32_t c0 = soff[(size_t)(c * B)] / kPbVec;
         const int32_t c1 = (c + 1 < S) ? soff[(size_t)((c + 1) * B)] / kPbVec : (int32_t)groups;
-        for (int32_t g0 = c0; g0 < c1; g0 += kPbScatterItem / kPbVec)
-            scat.push_back(PbScatter{(int32_t)c, g0, std::min(c1, g0 + kPbScatterItem / kPbVec), 0});
+        for (int32_t q0 = c0; q0 < c1; q0 += kPbScatterItem / kPbVec)
+            scat.push_back(PbScatter{(int32_t)c, q0, std::min(c1, q0 + kPbScatterItem / kPbVec), 0});
     }
-    // sum work: a sub-bin per item, or pieces of at most kPbItem slots whose
-    // row sums go through hub_part (rows `pieces` apart) to the reduce launch
+    // sum work: a sub-bin per item, or pieces of at most kPbItem value slots
+    // whose row sums go through hub_part (rows `pieces` apart) to the reduce
     std::vector<PbSum> sums;
     std::vector<HubRow> hrows;
     int64_t parts = 0;
     for (int64_t b = 0; b < B; ++b) {
-        const int32_t e0 = voff[(size_t)(b * S)];
-        const int32_t e1 = (b + 1 < B) ? voff[(size_t)((b + 1) * S)] : (int32_t)slots;
+        const int32_t e0 = voff[(size_t)b];  // chunk 0 of sub-bin b
+        const int32_t e1 = (b + 1 < B) ? voff[(size_t)(b + 1)] : (int32_t)nvals;
         const int32_t r0 = (int32_t)(b * kPbRows);
         const int32_t nr = std::min<int32_t>(kPbRows, n - r0);
         if (e1 - e0 <= kPbItem) {
@@ -430,15 +470,17 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
         for (int32_t i = 0; i < nr; ++i) hrows.push_back(HubRow{r0 + i, (int32_t)(parts + (int64_t)i * np), np, 0});
         parts += (int64_t)nr * np;
     }
-    // every slot range the kernels will touch, checked before any upload
+    // every range the kernels will touch, checked before any upload
     bool ok = true;
-    for (int64_t k = 0; k < groups; ++k) ok = ok && vgrp[(size_t)k] >= 0 && vgrp[(size_t)k] < groups;
+    for (int64_t k = 0; k < groups; ++k)
+        ok = ok && vgrp[(size_t)k] >= 0 && (int64_t)vgrp[(size_t)k] + __builtin_popcount(mask[(size_t)k]) <= nvals &&
+             (mask[(size_t)k] & 1u);
     for (const PbScatter& it : scat)
         ok = ok && it.chunk >= 0 && (int64_t)it.chunk * kPbChunk < n && it.g0 >= 0 && it.g0 < it.g1 &&
              it.g1 <= groups;
     for (const PbSum& it : sums) {
         ok = ok && it.sub_bin >= 0 && (int64_t)it.sub_bin * kPbRows < n && 0 <= it.e0 && it.e0 <= it.e1 &&
-             (int64_t)it.e1 <= slots && it.e0 % kPbVec == 0 && it.e1 % kPbVec == 0;
+             (int64_t)it.e1 <= nvals && it.e0 % kPbVec == 0 && it.e1 % kPbVec == 0;
         if (ok && it.piece >= 0) {
             // the last row's slot of this piece: part0 + (nr - 1) * pieces + piece
             const int64_t nr = std::min<int64_t>(kPbRows, (int64_t)n - (int64_t)it.sub_bin * kPbRows);
@@ -446,21 +488,26 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
         }
     }
     for (const HubRow& h : hrows) ok = ok && h.row >= 0 && h.row < n && h.first >= 0 && (int64_t)h.first + h.nseg <= parts;
+    for (int64_t q = 0; q < nvals; ++q) ok = ok && dst16[(size_t)q] < kPbRows;
     if (!ok) {
         set_last_error("propagation-blocked layout failed its range check");
         return EGX_UNKNOW_ERROR;
     }
     EGX_TRY(g.pb_src.alloc((size_t)slots * 2));
-    EGX_TRY(g.pb_dst.alloc((size_t)slots * 2));
-    EGX_TRY(g.pb_vals.alloc((size_t)slots * 4));
+    EGX_TRY(g.pb_mask.alloc(mask.size()));
     EGX_TRY(g.pb_vgrp.alloc(vgrp.size() * 4));
+    EGX_TRY(g.pb_dst.alloc(dst16.size() * 2));
+    EGX_TRY(g.pb_vals.alloc(dst16.size() * 4));
     EGX_TRY(g.pb_scatter.alloc(std::max<size_t>(scat.size(), 1) * sizeof(PbScatter)));
     EGX_TRY(g.pb_sum.alloc(std::max<size_t>(sums.size(), 1) * sizeof(PbSum)));
     EGX_TRY(g.pb_hrows.alloc(std::max<size_t>(hrows.size(), 1) * sizeof(HubRow)));
     EGX_TRY(g.pb_hub_part.alloc((size_t)std::max<int64_t>(parts, 1) * 8));
     EGX_HIP_TRY(hipMemcpy(g.pb_src.p, src16.data(), (size_t)slots * 2, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpy(g.pb_dst.p, dst16.data(), (size_t)slots * 2, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(g.pb_mask.p, mask.data(), mask.size(), hipMemcpyHostToDevice), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpy(g.pb_vgrp.p, vgrp.data(), vgrp.size() * 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(g.pb_dst.p, dst16.data(), dst16.size() * 2, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
+    // padding value slots are never written: they stay 0
+    EGX_HIP_TRY(hipMemset(g.pb_vals.p, 0, dst16.size() * 4), EGX_DEVICE_ERR);
     if (!scat.empty())
         EGX_HIP_TRY(hipMemcpy(g.pb_scatter.p, scat.data(), scat.size() * sizeof(PbScatter), hipMemcpyHostToDevice),
                     EGX_DEVICE_ERR);

@@ -91,15 +91,17 @@ struct BinCfg {
 // of kPbChunk labels and destination rows into sub-bins of kPbRows rows.
 // Every arc has a slot in a value array grouped by (sub-bin, chunk) and, in
 // the same order inside each such run, (row, source).  Runs are padded to a
-// multiple of kPbVec slots (source kPbChunk, a zero LDS entry; row 0, value
-// 0), so every run starts 16-byte aligned in both index arrays.  A scatter
-// launch stages one chunk of s = x/deg in LDS and writes a contiguous range
-// of that chunk's source slots, kPbVec at a time, each group to the value
-// slot its run gives it (pb_vgrp); a sum launch gives each sub-bin (or piece
-// of a large one, at most kPbItem slots) to one wavefront, which adds its
-// slots into FP64 LDS row accumulators in slot order.  Sub-bins cut into
+// multiple of kPbVec source slots (source kPbChunk, a zero LDS entry).  A
+// scatter launch stages one chunk of s = x/deg in LDS and takes a
+// contiguous range of that chunk's source slots, kPbVec at a time: each
+// group's stretches of equal rows (pb_mask) are summed, one term per
+// stretch, written from the group's first value slot (pb_vgrp) on.  Value
+// slots are grouped by (sub-bin, chunk) and padded per run to a multiple of
+// kPbVec (row 0, value 0).  A sum launch gives each sub-bin (or piece of a
+// large one, at most kPbItem value slots) to one wavefront, which adds its
+// terms into FP64 LDS row accumulators in slot order.  Sub-bins cut into
 // several pieces hand per-piece row sums to the reduce launch as hub rows.
-constexpr int32_t kPbChunk = 16384;
+constexpr int32_t kPbChunk = 32768;
 constexpr int32_t kPbRows = 2048;
 constexpr int32_t kPbVec = 8;
 constexpr int32_t kPbScatterItem = 262144;
@@ -137,12 +139,13 @@ struct Graph {
     // fl32(alpha * w) (eigenvector centrality); false: unit weights
     bool adjacency_weights = false;
     // propagation-blocked layout (bins.blocked): pb_src uint16 chunk-local
-    // source per slot in (chunk, sub-bin, row, source) order; pb_dst uint16
-    // sub-bin-local row per slot in (sub-bin, chunk, row, source) order;
-    // pb_vals the per-iteration terms in the second order; pb_vgrp int32 per
-    // kPbVec-slot group of pb_src: the value-slot group it is written to
+    // source per slot in (chunk, sub-bin, row, source) order; per kPbVec-slot
+    // group of it, pb_mask uint8 (bit j: a stretch of one row starts at slot
+    // j) and pb_vgrp int32 (its first value slot); per value slot, in
+    // (sub-bin, chunk) order, pb_dst uint16 (sub-bin-local row) and pb_vals
+    // (the per-iteration FP32 term)
     bool pb_ready = false;
-    DevBuf pb_src, pb_dst, pb_vals, pb_vgrp, pb_scatter, pb_sum, pb_hrows, pb_hub_part;
+    DevBuf pb_src, pb_dst, pb_vals, pb_vgrp, pb_mask, pb_scatter, pb_sum, pb_hrows, pb_hub_part;
     int64_t n_pb_scatter = 0, n_pb_sum = 0, n_pb_hrows = 0;
     double alpha_vals = 0.0;   // alpha baked into it_val
     double alpha_orig = 0.0;   // alpha baked into in_val

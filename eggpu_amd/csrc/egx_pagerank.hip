@@ -392,17 +392,19 @@ struct PbArgs {
     const uint16_t* dst;
     float* vals;
     const int32_t* vgrp;
+    const uint8_t* mask;
     const PbScatter* scatter;
     const PbSum* sums;
 };
 
-constexpr int kPbThreads = 512;
+constexpr int kPbThreads = 1024;  // one scatter workgroup per CU: its chunk takes 128 KB of LDS
 
 // One scatter item: s of its chunk into LDS (one coalesced read; entry
 // kPbChunk is the zero that padding slots read), then the item's groups of
-// kPbVec source slots, one per lane: a 16-byte index load, kPbVec LDS reads
-// and two 16-byte stores to the value-slot group vgrp gives, kPbGroups
-// groups in flight per lane.
+// kPbVec source slots, one per lane: a 16-byte index load, kPbVec LDS reads,
+// each stretch of equal rows summed in FP64 and stored as one FP32 term from
+// the group's first value slot on; kPbGroups groups in flight per lane.
+// Consecutive lanes' terms are adjacent, so the stores fill whole lines.
 constexpr int kPbGroups = 4;
 typedef uint32_t pb_u32x4 __attribute__((ext_vector_type(4)));
 typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
@@ -419,16 +421,17 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
     if (threadIdx.x == 0) sc[kPbChunk] = 0.0f;
     __syncthreads();
     const pb_u32x4* src = reinterpret_cast<const pb_u32x4*>(pb.src);
-    pb_f32x4* out = reinterpret_cast<pb_f32x4*>(pb.vals);
     for (int32_t g0 = it.g0 + (int32_t)threadIdx.x; g0 < it.g1; g0 += kPbGroups * kPbThreads) {
         pb_u32x4 u[kPbGroups];
         int32_t vg[kPbGroups];
+        uint32_t mk[kPbGroups];
 #pragma unroll
         for (int q = 0; q < kPbGroups; ++q) {
             const int32_t gi = g0 + q * kPbThreads;
             const bool ok = gi < it.g1;
             u[q] = ok ? __builtin_nontemporal_load(&src[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
             vg[q] = ok ? __builtin_nontemporal_load(&pb.vgrp[gi]) : -1;
+            mk[q] = ok ? (uint32_t)__builtin_nontemporal_load(&pb.mask[gi]) : 0u;
         }
 #pragma unroll
         for (int q = 0; q < kPbGroups; ++q) {
@@ -440,8 +443,19 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
                     v[2 * h] = sc[w[h] & 0xffffu];
                     v[2 * h + 1] = sc[w[h] >> 16];
                 }
-                out[2 * (size_t)vg[q]] = pb_f32x4{v[0], v[1], v[2], v[3]};
-                out[2 * (size_t)vg[q] + 1] = pb_f32x4{v[4], v[5], v[6], v[7]};
+                float* o = pb.vals + vg[q];
+                double run = (double)v[0];
+                int32_t k = 0;
+#pragma unroll
+                for (int j = 1; j < 8; ++j) {
+                    if ((mk[q] >> j) & 1u) {
+                        o[k++] = (float)run;
+                        run = (double)v[j];
+                    } else {
+                        run += (double)v[j];
+                    }
+                }
+                o[k] = (float)run;
             }
         }
     }
@@ -851,6 +865,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
         pb.dst = g.pb_dst.as<uint16_t>();
         pb.vals = g.pb_vals.as<float>();
         pb.vgrp = g.pb_vgrp.as<int32_t>();
+        pb.mask = g.pb_mask.as<uint8_t>();
         pb.scatter = g.pb_scatter.as<PbScatter>();
         pb.sums = g.pb_sum.as<PbSum>();
     }

@@ -4,10 +4,12 @@ value slots, and every sub-bin sums its slots in LDS.
 
 Parity bar as for the pull SpMV: FP32 iterates against the FP64 oracle at
 max-abs 1e-6 and L1 1e-5 per vector with identical iteration counts.
-Against the pull path on the same graph the row sums differ only in FP32
-summation order.  Sub-bins too large for one wavefront are cut into pieces
-summed by the reduce launch; a graph with a 100,000-arc row exercises that
-path.  Two solves must agree bit for bit.
+Against the pull path on the same graph the results differ only by the
+FP32 rounding of each stretch sum (the scatter stores one FP32 term per
+stretch of equal rows in an 8-slot group; the pull kernel sums every row in
+FP64): held to max-abs 1e-7.  Sub-bins too large for one wavefront are cut
+into pieces summed by the reduce launch; a graph with a 100,000-arc row
+exercises that path.  Two solves must agree bit for bit.
 """
 import numpy as np
 import pytest
@@ -117,10 +119,11 @@ def test_blocked_refuses_weighted_graphs(gpu_ready):
     g.close()
 
 
-def test_blocked_matches_pull_bitwise_at_scale_22(gpu_ready):
+def test_blocked_matches_pull_at_scale_22(gpu_ready):
     # R-MAT 22 built on the device (too large for the oracle in seconds):
-    # the blocked iteration must reproduce the pull SpMV bit for bit, at
-    # fixed K and at tol 1e-6, and sum to 1
+    # the blocked iteration must agree with the pull SpMV to FP32 stretch
+    # rounding, at fixed K and at tol 1e-6 with the same iteration count,
+    # repeat bit for bit, and sum to 1
     g = native.DeviceGraph.rmat(22, 16, seed=5)
     _say(f"rmat22: n={g.info()['n']} m={g.info()['m']}")
     a, ia, _ = g.pagerank(0.85, 10, 0.0)
@@ -128,8 +131,11 @@ def test_blocked_matches_pull_bitwise_at_scale_22(gpu_ready):
     g.set_bins(blocked=1)
     assert g.info()["blocked"] == 1
     b, ib, _ = g.pagerank(0.85, 10, 0.0)
+    c, ic, _ = g.pagerank(0.85, 10, 0.0)
     u, iu, _ = g.pagerank(0.85, 100, 1e-6)
-    assert ia == ib and np.array_equal(a.view(np.uint64), b.view(np.uint64))
-    assert it_t == iu and np.array_equal(t.view(np.uint64), u.view(np.uint64))
+    assert ia == ib == ic and np.array_equal(b.view(np.uint64), c.view(np.uint64))
+    _assert_close(b, a, "rmat22 blocked vs pull K=10", tol=1e-9, l1=1e-7)
+    assert it_t == iu
+    _assert_close(u, t, "rmat22 blocked vs pull tol 1e-6", tol=1e-9, l1=1e-7)
     assert abs(b.sum() - 1.0) < 1e-9
     g.close()
