@@ -105,7 +105,7 @@ constexpr int32_t kPbChunk = 32768;
 constexpr int32_t kPbRows = 2048;
 constexpr int32_t kPbVec = 8;
 constexpr int32_t kPbScatterItem = 262144;
-constexpr int32_t kPbItem = 262144;
+constexpr int32_t kPbItem = 131072;
 
 struct PbScatter {
     int32_t chunk;
