@@ -402,16 +402,64 @@ constexpr int kPbThreads = 1024;  // one scatter workgroup per CU: its chunk tak
 // One scatter item: s of its chunk into LDS (one coalesced read; entry
 // kPbChunk is the zero that padding slots read), then the item's groups of
 // kPbVec source slots, one per lane: a 16-byte index load, kPbVec LDS reads,
-// each stretch of equal rows summed in FP64 and stored as one FP32 term from
-// the group's first value slot on; kPbGroups groups in flight per lane.
-// Consecutive lanes' terms are adjacent, so the stores fill whole lines.
+// each stretch of equal rows summed in FP64 and emitted as one FP32 term;
+// kPbGroups groups in flight per lane.  The terms of a wavefront's 64 groups
+// usually form one contiguous range (groups of one run): they are then
+// compacted in a per-wave LDS staging area (offsets from a wave prefix sum)
+// and stored by consecutive lanes to consecutive slots.  Otherwise each lane
+// stores its terms from its group's first value slot on.
 constexpr int kPbGroups = 4;
+constexpr int kPbStage = 504;  // staged terms per wavefront (64 groups emit at most 512)
 typedef uint32_t pb_u32x4 __attribute__((ext_vector_type(4)));
 typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int32_t wave_excl_scan(int32_t x, int32_t& total) {
+    const int l = lane_id();
+    int32_t incl = x;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const int32_t y = __shfl_up(incl, d, kWave);
+        if (l >= d) incl += y;
+    }
+    total = __shfl(incl, kWave - 1, kWave);
+    return incl - x;
+}
+
+// The stretch terms of one group (mask bit j: a stretch starts at slot j),
+// written to o[0], o[1], ...
+template <typename P>
+__device__ __forceinline__ void pb_emit(const float* sc, pb_u32x4 u, uint32_t mk, P* o) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+    float v[8];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        v[2 * h] = sc[w[h] & 0xffffu];
+        v[2 * h + 1] = sc[w[h] >> 16];
+    }
+    double run = (double)v[0];
+    int32_t k = 0;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+        if ((mk >> j) & 1u) {
+            o[k++] = (float)run;
+            run = (double)v[j];
+        } else {
+            run += (double)v[j];
+        }
+    }
+    o[k] = (float)run;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <typename T>
 __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs pb) {
     __shared__ float sc[kPbChunk + 4];
+    __shared__ float stage[kPbThreads / kWave][kPbStage];
     if (a.state->done) return;
     bind(a);
     const PbScatter it = pb.scatter[blockIdx.x];
@@ -421,13 +469,19 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
     if (threadIdx.x == 0) sc[kPbChunk] = 0.0f;
     __syncthreads();
     const pb_u32x4* src = reinterpret_cast<const pb_u32x4*>(pb.src);
-    for (int32_t g0 = it.g0 + (int32_t)threadIdx.x; g0 < it.g1; g0 += kPbGroups * kPbThreads) {
+    float* stg = stage[wave_id()];
+    const int l = lane_id();
+    // every lane of a wavefront runs the loop the same number of times (the
+    // bound depends on the wavefront's first thread only), so the shuffles
+    // below see all 64 lanes
+    const int32_t wave0 = it.g0 + (int32_t)(threadIdx.x - l);
+    for (int32_t g0 = wave0; g0 < it.g1; g0 += kPbGroups * kPbThreads) {
         pb_u32x4 u[kPbGroups];
         int32_t vg[kPbGroups];
         uint32_t mk[kPbGroups];
 #pragma unroll
         for (int q = 0; q < kPbGroups; ++q) {
-            const int32_t gi = g0 + q * kPbThreads;
+            const int32_t gi = g0 + q * kPbThreads + l;
             const bool ok = gi < it.g1;
             u[q] = ok ? __builtin_nontemporal_load(&src[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
             vg[q] = ok ? __builtin_nontemporal_load(&pb.vgrp[gi]) : -1;
@@ -435,27 +489,18 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
         }
 #pragma unroll
         for (int q = 0; q < kPbGroups; ++q) {
-            if (vg[q] >= 0) {
-                const uint32_t w[4] = {u[q].x, u[q].y, u[q].z, u[q].w};
-                float v[8];
-#pragma unroll
-                for (int h = 0; h < 4; ++h) {
-                    v[2 * h] = sc[w[h] & 0xffffu];
-                    v[2 * h + 1] = sc[w[h] >> 16];
-                }
-                float* o = pb.vals + vg[q];
-                double run = (double)v[0];
-                int32_t k = 0;
-#pragma unroll
-                for (int j = 1; j < 8; ++j) {
-                    if ((mk[q] >> j) & 1u) {
-                        o[k++] = (float)run;
-                        run = (double)v[j];
-                    } else {
-                        run += (double)v[j];
-                    }
-                }
-                o[k] = (float)run;
+            const bool ok = vg[q] >= 0;
+            int32_t total = 0;
+            const int32_t off = wave_excl_scan(__builtin_popcount(mk[q]), total);
+            const int32_t first = __shfl(vg[q] - off, 0, kWave);
+            const bool contig = total > 0 && total <= kPbStage && __all(!ok || vg[q] - off == first);
+            if (contig) {
+                if (ok) pb_emit(sc, u[q], mk[q], stg + off);
+                wave_lds_sync();
+                for (int32_t j = l; j < total; j += kWave) pb.vals[first + j] = stg[j];
+                wave_lds_sync();
+            } else if (ok) {
+                pb_emit(sc, u[q], mk[q], pb.vals + vg[q]);
             }
         }
     }
