@@ -15,8 +15,11 @@
 //   * out-weights are summed per row, sequentially in CSR order, in FP64, so
 //     fl32(alpha * (w / outW)) (:344) rounds exactly as the reference does.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <thread>
+#include <vector>
 
 #include "egx_graph.h"
 
@@ -313,6 +316,21 @@ int build_tile_sort(Graph& g, const BinCfg& cfg) {
     return EGX_SUCC;
 }
 
+// f(lo, hi) over [0, count) split into contiguous ranges, one per host
+// thread (at most 16).
+template <typename F>
+void host_parallel_for(int64_t count, F f) {
+    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
+                                                              (count + 4095) / 4096}));
+    if (nt == 1) {
+        f((int64_t)0, count);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int64_t t = 0; t < nt; ++t) th.emplace_back(f, count * t / nt, count * (t + 1) / nt);
+    for (std::thread& x : th) x.join();
+}
+
 // Propagation-blocked layout of the iteration CSC (it_row, it_col), built on
 // the host.  Arcs are bucketed by (chunk c, sub-bin b) into runs, in CSC
 // order (row, then source) inside each run, each run padded to a multiple
@@ -348,12 +366,16 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     EGX_HIP_TRY(hipMemcpy(col.data(), g.it_col, (size_t)m * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
 
     // run lengths in source slots, chunk-major (chunk c, sub-bin b) at c*B + b
+    // (threads own whole sub-bins, so no two touch one counter or cursor)
     std::vector<int32_t> cnt((size_t)(S * B), 0);
-    for (int32_t r = 0; r < n; ++r) {
-        const int64_t b = r / kPbRows;
-        for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e)
-            ++cnt[(size_t)((col[(size_t)e] / kPbChunk) * B + b)];
-    }
+    host_parallel_for(B, [&](int64_t b0, int64_t b1) {
+        const int32_t r1 = (int32_t)std::min<int64_t>(n, b1 * kPbRows);
+        for (int32_t r = (int32_t)(b0 * kPbRows); r < r1; ++r) {
+            const int64_t b = r / kPbRows;
+            for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e)
+                ++cnt[(size_t)((col[(size_t)e] / kPbChunk) * B + b)];
+        }
+    });
     std::vector<int32_t> soff((size_t)(S * B));
     int64_t slots = 0;
     for (int64_t k = 0; k < S * B; ++k) {
@@ -371,16 +393,19 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     std::vector<int32_t> srow((size_t)slots, -1);
     {
         std::vector<int32_t> cur(soff);
-        for (int32_t r = 0; r < n; ++r) {
-            const int64_t b = r / kPbRows;
-            for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e) {
-                const int32_t u = col[(size_t)e];
-                const int64_t c = u / kPbChunk;
-                const int32_t p = cur[(size_t)(c * B + b)]++;
-                src16[(size_t)p] = (uint16_t)(u - c * kPbChunk);
-                srow[(size_t)p] = r;
+        host_parallel_for(B, [&](int64_t b0, int64_t b1) {
+            const int32_t r1 = (int32_t)std::min<int64_t>(n, b1 * kPbRows);
+            for (int32_t r = (int32_t)(b0 * kPbRows); r < r1; ++r) {
+                const int64_t b = r / kPbRows;
+                for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e) {
+                    const int32_t u = col[(size_t)e];
+                    const int64_t c = u / kPbChunk;
+                    const int32_t p = cur[(size_t)(c * B + b)]++;
+                    src16[(size_t)p] = (uint16_t)(u - c * kPbChunk);
+                    srow[(size_t)p] = r;
+                }
             }
-        }
+        });
     }
     row.clear();
     row.shrink_to_fit();
@@ -390,21 +415,23 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     // before them (they add the zero LDS entry)
     std::vector<uint8_t> mask((size_t)std::max<int64_t>(groups, 1), 0);
     std::vector<int32_t> outs((size_t)(S * B), 0);  // value slots per run, unpadded
-    for (int64_t k = 0; k < S * B; ++k) {
-        const int64_t g0 = soff[(size_t)k] / kPbVec;
-        const int64_t g1 = (k + 1 < S * B ? soff[(size_t)k + 1] : slots) / kPbVec;
-        int32_t o = 0;
-        for (int64_t gi = g0; gi < g1; ++gi) {
-            uint8_t mk = 1;
-            for (int j = 1; j < kPbVec; ++j) {
-                const int32_t rj = srow[(size_t)(gi * kPbVec + j)];
-                if (rj >= 0 && rj != srow[(size_t)(gi * kPbVec + j - 1)]) mk |= (uint8_t)(1u << j);
+    host_parallel_for(S * B, [&](int64_t k0, int64_t k1) {
+        for (int64_t k = k0; k < k1; ++k) {
+            const int64_t g0 = soff[(size_t)k] / kPbVec;
+            const int64_t g1 = (k + 1 < S * B ? soff[(size_t)k + 1] : slots) / kPbVec;
+            int32_t o = 0;
+            for (int64_t gi = g0; gi < g1; ++gi) {
+                uint8_t mk = 1;
+                for (int j = 1; j < kPbVec; ++j) {
+                    const int32_t rj = srow[(size_t)(gi * kPbVec + j)];
+                    if (rj >= 0 && rj != srow[(size_t)(gi * kPbVec + j - 1)]) mk |= (uint8_t)(1u << j);
+                }
+                mask[(size_t)gi] = mk;
+                o += __builtin_popcount(mk);
             }
-            mask[(size_t)gi] = mk;
-            o += __builtin_popcount(mk);
+            outs[(size_t)k] = o;
         }
-        outs[(size_t)k] = o;
-    }
+    });
     // value-slot starts, bin-major (sub-bin b, chunk c), each run padded
     std::vector<int32_t> voff((size_t)(S * B));  // indexed like cnt: c*B + b
     int64_t nvals = 0;
@@ -420,18 +447,21 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     // first value slot of every group, and the row of every value slot
     std::vector<int32_t> vgrp((size_t)std::max<int64_t>(groups, 1));
     std::vector<uint16_t> dst16((size_t)std::max<int64_t>(nvals, 1), 0);
-    for (int64_t k = 0; k < S * B; ++k) {
-        const int64_t b = k % B;
-        const int64_t g0 = soff[(size_t)k] / kPbVec;
-        const int64_t g1 = (k + 1 < S * B ? soff[(size_t)k + 1] : slots) / kPbVec;
-        int32_t o = voff[(size_t)k];
-        for (int64_t gi = g0; gi < g1; ++gi) {
-            vgrp[(size_t)gi] = o;
-            const uint8_t mk = mask[(size_t)gi];
-            for (int j = 0; j < kPbVec; ++j)
-                if (mk & (1u << j)) dst16[(size_t)o++] = (uint16_t)(srow[(size_t)(gi * kPbVec + j)] - b * kPbRows);
+    host_parallel_for(S * B, [&](int64_t k0, int64_t k1) {
+        for (int64_t k = k0; k < k1; ++k) {
+            const int64_t b = k % B;
+            const int64_t g0 = soff[(size_t)k] / kPbVec;
+            const int64_t g1 = (k + 1 < S * B ? soff[(size_t)k + 1] : slots) / kPbVec;
+            int32_t o = voff[(size_t)k];
+            for (int64_t gi = g0; gi < g1; ++gi) {
+                vgrp[(size_t)gi] = o;
+                const uint8_t mk = mask[(size_t)gi];
+                for (int j = 0; j < kPbVec; ++j)
+                    if (mk & (1u << j))
+                        dst16[(size_t)o++] = (uint16_t)(srow[(size_t)(gi * kPbVec + j)] - b * kPbRows);
+            }
         }
-    }
+    });
     srow.clear();
     srow.shrink_to_fit();
     // scatter work: contiguous group ranges of at most kPbScatterItem slots per chunk
@@ -471,10 +501,20 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
         parts += (int64_t)nr * np;
     }
     // every range the kernels will touch, checked before any upload
-    bool ok = true;
-    for (int64_t k = 0; k < groups; ++k)
-        ok = ok && vgrp[(size_t)k] >= 0 && (int64_t)vgrp[(size_t)k] + __builtin_popcount(mask[(size_t)k]) <= nvals &&
-             (mask[(size_t)k] & 1u);
+    std::atomic<bool> ok_slots{true};
+    host_parallel_for(groups, [&](int64_t a0, int64_t a1) {
+        bool good = true;
+        for (int64_t k = a0; k < a1; ++k)
+            good = good && vgrp[(size_t)k] >= 0 &&
+                   (int64_t)vgrp[(size_t)k] + __builtin_popcount(mask[(size_t)k]) <= nvals && (mask[(size_t)k] & 1u);
+        if (!good) ok_slots = false;
+    });
+    host_parallel_for(nvals, [&](int64_t a0, int64_t a1) {
+        bool good = true;
+        for (int64_t q = a0; q < a1; ++q) good = good && dst16[(size_t)q] < kPbRows;
+        if (!good) ok_slots = false;
+    });
+    bool ok = ok_slots;
     for (const PbScatter& it : scat)
         ok = ok && it.chunk >= 0 && (int64_t)it.chunk * kPbChunk < n && it.g0 >= 0 && it.g0 < it.g1 &&
              it.g1 <= groups;
@@ -488,7 +528,6 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
         }
     }
     for (const HubRow& h : hrows) ok = ok && h.row >= 0 && h.row < n && h.first >= 0 && (int64_t)h.first + h.nseg <= parts;
-    for (int64_t q = 0; q < nvals; ++q) ok = ok && dst16[(size_t)q] < kPbRows;
     if (!ok) {
         set_last_error("propagation-blocked layout failed its range check");
         return EGX_UNKNOW_ERROR;
