@@ -139,3 +139,39 @@ def test_blocked_matches_pull_at_scale_22(gpu_ready):
     _assert_close(u, t, "rmat22 blocked vs pull tol 1e-6", tol=1e-9, l1=1e-7)
     assert abs(b.sum() - 1.0) < 1e-9
     g.close()
+
+
+def test_blocked_graph_in_multi_graph_and_batch_paths(gpu_ready):
+    # the multi-graph launch and the SpMM batch sessions keep their own
+    # kernels on a graph whose plain solves run blocked; all agree with the
+    # oracle, and the plain solve still runs blocked afterwards
+    V1, E1 = O.rmat_csr(12, 16, seed=31)
+    V2, E2 = O.er_csr(3000, 30000, seed=32)
+    g1 = native.DeviceGraph.from_csr(V1, E1)
+    g2 = native.DeviceGraph.from_csr(V2, E2)
+    g1.set_bins(blocked=1)
+    outs, iters, _ = native.multi_pagerank([g1, g2], 0.85, 100, 1e-6)
+    for (V, E), out, it in zip(((V1, E1), (V2, E2)), outs, iters):
+        ref, it_ref, _ = O.pagerank_csr(V, E, None, 0.85, 100, 1e-6)
+        _assert_close(out, ref, "multi-graph beside a blocked layout")
+        assert it == it_ref
+    n = len(V1) - 1
+    src, dst = O.csr_rows(V1, E1)
+    sets = [[], [3, 17, 900]]
+    cols, its, _ = g1.ppr(sets, alpha=0.85, max_iter=100, tol=1e-6)
+    for j, S in enumerate(sets):
+        if S:
+            p = np.zeros(n)
+            p[S] = 1.0 / len(S)
+            ref, it_ref = O.ppr_rows(n, src, dst, None, p, 0.85, 100, 1e-6)[:2]
+        else:
+            ref, it_ref, _ = O.pagerank_csr(V1, E1, None, 0.85, 100, 1e-6)
+        _assert_close(cols[j], ref, f"batch column {j} beside a blocked layout")
+        assert its[j] == it_ref
+    assert g1.info()["blocked"] == 1
+    pr, it, _ = g1.pagerank(0.85, 100, 1e-6)
+    ref, it_ref, _ = O.pagerank_csr(V1, E1, None, 0.85, 100, 1e-6)
+    _assert_close(pr, ref, "blocked after the batch")
+    assert it == it_ref
+    g1.close()
+    g2.close()
