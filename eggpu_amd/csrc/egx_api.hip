@@ -415,6 +415,7 @@ int egx_graph_download_csr(const egx_graph* gh, int32_t* V, int32_t* E, double* 
     return EGX_SUCC;
 }
 
+extern "C++" {  // helpers, not part of the C ABI
 namespace {
 egx::SolveParams solve_params(const egx_solve_opts* opts, int32_t flags) {
     egx::SolveParams p;
@@ -450,6 +451,7 @@ void copy_stats(const egx::SolveStats& st, egx_solve_stats* stats) {
     stats->spmv_grid = st.spmv_grid;
 }
 }  // namespace
+}  // extern "C++"
 
 int egx_graph_pagerank_ex(egx_graph* gh, const egx_solve_opts* opts, double* PR, egx_solve_stats* stats,
                           int32_t flags) {

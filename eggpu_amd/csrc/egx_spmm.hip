@@ -40,7 +40,6 @@ constexpr int kCols = 64;            // columns per batch (one per lane)
 constexpr int kRowThreads = 256;     // 4 wavefronts
 constexpr int kRowWaves = kRowThreads / kWave;
 constexpr int kSpmmHubSeg = 2048;    // arcs per hub segment (one workgroup)
-constexpr int kUnroll = 8;
 constexpr int kMinWidth = 8;         // narrowest X row (columns)
 constexpr size_t kArenaBytes = (size_t)1 << 20;  // pinned staging for queued batch updates
 
