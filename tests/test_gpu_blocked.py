@@ -175,3 +175,47 @@ def test_blocked_graph_in_multi_graph_and_batch_paths(gpu_ready):
     assert it == it_ref
     g1.close()
     g2.close()
+
+
+def _edge_shape(name):
+    rng = np.random.default_rng(17)
+    if name == "chunk_plus_one":
+        # 32,769 nodes: the last chunk holds one source
+        n = 32769
+        src = rng.integers(0, n, 200000)
+        dst = rng.integers(0, n, 200000)
+    elif name == "sparse_bins":
+        # arcs only into every 5th sub-bin's rows and from a few chunks
+        n = 70000
+        src = rng.integers(0, 2000, 150000) * 31 % n
+        dst = (rng.integers(0, n // 2048, 150000) // 5 * 5) * 2048 + rng.integers(0, 2048, 150000)
+        dst = np.minimum(dst, n - 1)
+    elif name == "one_row":
+        # every arc into row 5 (one sub-bin split into many pieces)
+        n = 300000
+        src = np.arange(n)
+        dst = np.full(n, 5)
+    else:  # self_loops: only i -> i
+        n = 5000
+        src = np.arange(n)
+        dst = np.arange(n)
+    keep = src != dst if name != "self_loops" else np.ones(len(src), dtype=bool)
+    if name == "self_loops":
+        V = np.arange(n + 1, dtype=np.int32)
+        return V, np.arange(n, dtype=np.int32)
+    return O.simple_csr_from_coo(n, src[keep], dst[keep])
+
+
+@pytest.mark.parametrize("name", ["chunk_plus_one", "sparse_bins", "one_row", "self_loops"])
+def test_blocked_edge_shapes(gpu_ready, name):
+    V, E = _edge_shape(name)
+    _say(f"{name}: n={len(V) - 1} m={len(E)}")
+    g = native.DeviceGraph.from_csr(V, E)
+    g.set_bins(blocked=1)
+    assert g.info()["blocked"] == 1
+    for tol, K in ((1e-6, 100), (0.0, 15)):
+        pr, it, _ = g.pagerank(0.85, K, tol)
+        ref, it_ref, _ = O.pagerank_csr(V, E, None, 0.85, K, tol)
+        _assert_close(pr, ref, f"{name} tol={tol}")
+        assert it == it_ref
+    g.close()
