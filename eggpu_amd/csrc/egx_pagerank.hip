@@ -413,16 +413,20 @@ constexpr int kPbStage = 504;  // staged terms per wavefront (64 groups emit at 
 typedef uint32_t pb_u32x4 __attribute__((ext_vector_type(4)));
 typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ int32_t wave_excl_scan(int32_t x, int32_t& total) {
-    const int l = lane_id();
-    int32_t incl = x;
+// Exclusive prefix sum over the wavefront of x in [0, 15], from one ballot
+// per bit (no LDS traffic).
+__device__ __forceinline__ int32_t wave_excl_scan_small(uint32_t x, int32_t& total) {
+    int32_t off = 0, tot = 0;
 #pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const int32_t y = __shfl_up(incl, d, kWave);
-        if (l >= d) incl += y;
+    for (int b = 0; b < 4; ++b) {
+        const uint64_t bits = __ballot((x >> b) & 1u);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bits >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)bits, 0u));
+        off += (int32_t)below << b;
+        tot += __popcll(bits) << b;
     }
-    total = __shfl(incl, kWave - 1, kWave);
-    return incl - x;
+    total = tot;
+    return off;
 }
 
 // The stretch terms of one group (mask bit j: a stretch starts at slot j),
@@ -491,8 +495,8 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
         for (int q = 0; q < kPbGroups; ++q) {
             const bool ok = vg[q] >= 0;
             int32_t total = 0;
-            const int32_t off = wave_excl_scan(__builtin_popcount(mk[q]), total);
-            const int32_t first = __shfl(vg[q] - off, 0, kWave);
+            const int32_t off = wave_excl_scan_small((uint32_t)__builtin_popcount(mk[q]), total);
+            const int32_t first = __builtin_amdgcn_readfirstlane(vg[q] - off);  // lane 0: every lane runs here
             const bool contig = total > 0 && total <= kPbStage && __all(!ok || vg[q] - off == first);
             if (contig) {
                 if (ok) pb_emit(sc, u[q], mk[q], stg + off);
