@@ -1,8 +1,9 @@
 /*
  * egx_oracle.c -- CPU oracle for the PageRank path.  TEST INFRASTRUCTURE ONLY.
  *
- * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load this library, and only as the checker or as the timed CPU baseline.
+ * Only tests/, the repository's smoke check and bench.py's cpu_baseline leg
+ * may load this library, and only to check results or as the timed CPU
+ * baseline.
  * Nothing in the eggpu_amd package links it or calls it.
  *
  * Every function restates a reference algorithm in plain C, in the

@@ -1,8 +1,8 @@
 """CPU oracle for the PageRank path -- TEST INFRASTRUCTURE ONLY.
 
-Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
-leg may import this module, and only as the checker (or as the timed CPU
-baseline).  The ``eggpu_amd`` package never imports it; its HIP path fails
+Only ``tests/``, the repository's smoke check and ``bench.py``'s
+cpu_baseline leg may import this module, and only to check results (or as
+the timed CPU baseline).  The ``eggpu_amd`` package never imports it; its HIP path fails
 loudly when the native library is missing instead of falling back here.
 
 Python-side helpers around ``egx_oracle.c``:

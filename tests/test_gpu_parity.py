@@ -1,7 +1,7 @@
 """GPU parity tests: the HIP path (through libegx.so's C ABI) against the
 pinned CPU oracle and the golden fixtures.
 
-Bars (north star): integer graph construction bit-exact; PageRank within
+Bars: integer graph construction bit-exact; PageRank within
 max-abs 1e-6 per node and L1 1e-5 of the FP64 oracle with the identical
 iteration count; FP64 iterates within 1e-12.
 """
