@@ -15,10 +15,8 @@
 //   * out-weights are summed per row, sequentially in CSR order, in FP64, so
 //     fl32(alpha * (w / outW)) (:344) rounds exactly as the reference does.
 #include <algorithm>
-#include <atomic>
 #include <chrono>
 #include <cmath>
-#include <thread>
 #include <vector>
 
 #include "egx_graph.h"
@@ -316,29 +314,154 @@ int build_tile_sort(Graph& g, const BinCfg& cfg) {
     return EGX_SUCC;
 }
 
-// f(lo, hi) over [0, count) split into contiguous ranges, one per host
-// thread (at most 16).
-template <typename F>
-void host_parallel_for(int64_t count, F f) {
-    const int64_t nt = std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
-                                                              (count + 4095) / 4096}));
-    if (nt == 1) {
-        f((int64_t)0, count);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (int64_t t = 0; t < nt; ++t) th.emplace_back(f, count * t / nt, count * (t + 1) / nt);
-    for (std::thread& x : th) x.join();
+namespace {
+
+// Run key of every arc, (chunk of its source) * B + (sub-bin of its row),
+// with the arc's CSC position as the value: a stable sort by key keeps CSC
+// order (row, then position) inside every run.
+__global__ void k_pb_keys(const int32_t* __restrict__ col, const int32_t* __restrict__ rowof, int64_t m,
+                          int64_t B, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m) return;
+    keys[e] = (uint32_t)((int64_t)(col[e] / kPbChunk) * B + rowof[e] / kPbRows);
+    vals[e] = (uint32_t)e;
 }
 
+// Source slots of every run (its arcs rounded up to kPbVec) from the lower
+// bounds of the sorted keys; entry `runs` is 0, so the exclusive scan ends
+// with the total.
+__global__ void k_pb_run_slots(const int32_t* __restrict__ lb, int64_t runs, uint32_t* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > runs) return;
+    out[k] = k < runs ? (uint32_t)((lb[k + 1] - lb[k] + kPbVec - 1) / kPbVec * kPbVec) : 0u;
+}
+
+__global__ void k_fill_u16(uint16_t* __restrict__ p, int64_t n, uint16_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Sorted arc i of run k to source slot soff[k] + (i - lb[k]): its source
+// label inside the chunk and its row.  Anything out of range raises *bad.
+__global__ void k_pb_place(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                           const int32_t* __restrict__ lb, const uint32_t* __restrict__ soff,
+                           const int32_t* __restrict__ col, const int32_t* __restrict__ rowof, int64_t m,
+                           int64_t runs, int64_t B, uint16_t* __restrict__ src16, int32_t* __restrict__ srow,
+                           int32_t* __restrict__ bad) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t k = keys[i];
+    const uint32_t e = vals[i];
+    if ((int64_t)k >= runs || (int64_t)e >= m || i < lb[k]) {
+        *bad = 1;
+        return;
+    }
+    const int64_t p = (int64_t)soff[k] + (i - lb[k]);
+    const int64_t u = (int64_t)col[e] - (int64_t)(k / B) * kPbChunk;
+    if (p >= (int64_t)soff[k + 1] || u < 0 || u >= kPbChunk) {
+        *bad = 1;
+        return;
+    }
+    src16[p] = (uint16_t)u;
+    srow[p] = rowof[e];
+}
+
+// Stretch mask of every kPbVec-slot group (bit j: slot j starts a stretch of
+// equal rows; padding slots, row -1, join the stretch before them) and its
+// value-slot count; entry `groups` of pc is 0 for the scan total.
+static_assert(kPbVec == 8, "a group is two int4 loads of rows");
+__global__ void k_pb_masks(const int32_t* __restrict__ srow, int64_t groups, uint8_t* __restrict__ mask,
+                           uint32_t* __restrict__ pc) {
+    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi > groups) return;
+    if (gi == groups) {
+        pc[gi] = 0;
+        return;
+    }
+    const int4 lo = reinterpret_cast<const int4*>(srow)[2 * gi];
+    const int4 hi = reinterpret_cast<const int4*>(srow)[2 * gi + 1];
+    const int32_t r[kPbVec] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    uint32_t mk = 1;
+#pragma unroll
+    for (int j = 1; j < kPbVec; ++j)
+        if (r[j] >= 0 && r[j] != r[j - 1]) mk |= 1u << j;
+    mask[gi] = (uint8_t)mk;
+    pc[gi] = (uint32_t)__popc(mk);
+}
+
+// Value slots of every run, rounded up to kPbVec, stored bin-major at
+// b * S + c; entry `runs` is 0 for the scan total.  gs is the exclusive scan
+// of the groups' value-slot counts.
+__global__ void k_pb_run_values(const uint32_t* __restrict__ soff, const uint32_t* __restrict__ gs, int64_t runs,
+                                int64_t S, int64_t B, uint32_t* __restrict__ pout) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > runs) return;
+    if (k == runs) {
+        pout[runs] = 0;
+        return;
+    }
+    const uint32_t o = gs[soff[k + 1] / kPbVec] - gs[soff[k] / kPbVec];
+    pout[(k % B) * S + k / B] = (o + kPbVec - 1) / kPbVec * kPbVec;
+}
+
+// First value slot of every group and the sub-bin row of each of its value
+// slots.  The group's run is the last run whose first slot is at or before
+// the group's (empty runs share the next run's first slot).
+__global__ void k_pb_groups(const uint32_t* __restrict__ soff, const uint32_t* __restrict__ gs,
+                            const uint32_t* __restrict__ voff, const uint8_t* __restrict__ mask,
+                            const int32_t* __restrict__ srow, int64_t groups, int64_t runs, int64_t S, int64_t B,
+                            int32_t* __restrict__ vgrp, uint16_t* __restrict__ dst16, int32_t* __restrict__ bad) {
+    const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= groups) return;
+    const int64_t s0 = gi * kPbVec;
+    // first k in [0, runs] with soff[k] > s0; soff[runs] is the slot total
+    int64_t lo = 0, hi = runs;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)soff[mid] <= s0) lo = mid + 1;
+        else hi = mid;
+    }
+    const int64_t k = lo - 1;
+    if (k < 0) {
+        *bad = 1;
+        return;
+    }
+    const int64_t v = (k % B) * S + k / B;
+    const int64_t o = (int64_t)voff[v] + gs[gi] - gs[soff[k] / kPbVec];
+    const uint32_t mk = mask[gi];
+    if (!(mk & 1u) || o + __popc(mk) > (int64_t)voff[v + 1]) {
+        *bad = 1;
+        return;
+    }
+    vgrp[gi] = (int32_t)o;
+    const int64_t r0 = (k % B) * kPbRows;
+    int64_t q = o;
+    for (int j = 0; j < kPbVec; ++j)
+        if (mk & (1u << j)) {
+            const int64_t d = (int64_t)srow[s0 + j] - r0;
+            if (d < 0 || d >= kPbRows) {
+                *bad = 1;
+                return;
+            }
+            dst16[q++] = (uint16_t)d;
+        }
+}
+
+}  // namespace
+
 // Propagation-blocked layout of the iteration CSC (it_row, it_col), built on
-// the host.  Arcs are bucketed by (chunk c, sub-bin b) into runs, in CSC
+// the device.  Arcs are bucketed by (chunk c, sub-bin b) into runs, in CSC
 // order (row, then source) inside each run, each run padded to a multiple
 // of kPbVec source slots.  Each kPbVec-slot group of a run is cut into
 // stretches of equal rows (bit j of its mask marks a stretch starting at
 // slot j); every stretch has one value slot.  Value slots follow
 // (sub-bin, chunk) order, each run's stretches in group order, padded to a
 // multiple of kPbVec (padding rows 0, values 0: never written).
+//
+// The runs come from a stable radix sort of the arcs by run key, the slot
+// offsets from scans of rounded run sizes; only the run offsets (two arrays
+// of S*B + 1 entries) come back to the host, which cuts them into scatter
+// and sum work items.
 int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     // the blocked layout depends only on the iteration CSC
     if (cfg.blocked && g.pb_ready && !layout_changed) return EGX_SUCC;
@@ -356,119 +479,108 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     const int64_t m = g.m;
     const int64_t S = (n + kPbChunk - 1) / kPbChunk;
     const int64_t B = (n + kPbRows - 1) / kPbRows;
-    if (S * B > ((int64_t)1 << 28)) {
+    const int64_t runs = S * B;
+    if (runs > ((int64_t)1 << 28) || m > INT32_MAX) {
         set_last_error("graph too large for the propagation-blocked layout");
         return EGX_UNSUPPORTED_GRAPH;
     }
-    std::vector<int32_t> row((size_t)n + 1), col((size_t)m);
-    EGX_HIP_TRY(hipStreamSynchronize(g.stream), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpy(row.data(), g.it_row, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpy(col.data(), g.it_col, (size_t)m * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    hipStream_t s = g.stream;
+    const unsigned tb = 256;
 
-    // run lengths in source slots, chunk-major (chunk c, sub-bin b) at c*B + b
-    // (threads own whole sub-bins, so no two touch one counter or cursor)
-    std::vector<int32_t> cnt((size_t)(S * B), 0);
-    host_parallel_for(B, [&](int64_t b0, int64_t b1) {
-        const int32_t r1 = (int32_t)std::min<int64_t>(n, b1 * kPbRows);
-        for (int32_t r = (int32_t)(b0 * kPbRows); r < r1; ++r) {
-            const int64_t b = r / kPbRows;
-            for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e)
-                ++cnt[(size_t)((col[(size_t)e] / kPbChunk) * B + b)];
-        }
-    });
-    std::vector<int32_t> soff((size_t)(S * B));
-    int64_t slots = 0;
-    for (int64_t k = 0; k < S * B; ++k) {
-        soff[(size_t)k] = (int32_t)slots;
-        slots += (cnt[(size_t)k] + kPbVec - 1) / kPbVec * kPbVec;
+    // arcs sorted by run; lower bounds of every run key; source-slot offsets
+    DevBuf rowof, keys, vals, lb, soff, bad;
+    {
+        DevBuf keys_alt, vals_alt;
+        EGX_TRY(rowof.alloc((size_t)m * 4));
+        EGX_TRY(keys.alloc((size_t)m * 4));
+        EGX_TRY(vals.alloc((size_t)m * 4));
+        EGX_TRY(keys_alt.alloc((size_t)m * 4));
+        EGX_TRY(vals_alt.alloc((size_t)m * 4));
+        k_expand_src<<<nblocks(m, tb), tb, 0, s>>>(g.it_row, n, m, rowof.as<int32_t>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        k_pb_keys<<<nblocks(m, tb), tb, 0, s>>>(g.it_col, rowof.as<int32_t>(), m, B, keys.as<uint32_t>(),
+                                                vals.as<uint32_t>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        EGX_TRY(radix_sort_u32(keys.as<uint32_t>(), keys_alt.as<uint32_t>(), vals.as<uint32_t>(),
+                               vals_alt.as<uint32_t>(), m, bits_for((uint64_t)(runs - 1)), s));
     }
+    EGX_TRY(lb.alloc((size_t)(runs + 1) * 4));
+    EGX_TRY(lower_bound_iota_u32(keys.as<uint32_t>(), m, lb.as<int32_t>(), runs + 1, s));
+    EGX_TRY(soff.alloc((size_t)(runs + 1) * 4));
+    k_pb_run_slots<<<nblocks(runs + 1, tb), tb, 0, s>>>(lb.as<int32_t>(), runs, soff.as<uint32_t>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_TRY(exclusive_scan_u32(soff.as<uint32_t>(), soff.as<uint32_t>(), runs + 1, s));
+    std::vector<uint32_t> soff_h((size_t)runs + 1);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(soff_h.data(), soff.p, soff_h.size() * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    const int64_t slots = soff_h[(size_t)runs];
     if (slots > INT32_MAX) {
         set_last_error("propagation-blocked layout: too many slots");
         return EGX_UNSUPPORTED_GRAPH;
     }
     const int64_t groups = slots / kPbVec;
-    // source slots in CSC order within each run; the row of every slot
-    // (padding: -1) for the stretch masks
-    std::vector<uint16_t> src16((size_t)slots, (uint16_t)kPbChunk);
-    std::vector<int32_t> srow((size_t)slots, -1);
-    {
-        std::vector<int32_t> cur(soff);
-        host_parallel_for(B, [&](int64_t b0, int64_t b1) {
-            const int32_t r1 = (int32_t)std::min<int64_t>(n, b1 * kPbRows);
-            for (int32_t r = (int32_t)(b0 * kPbRows); r < r1; ++r) {
-                const int64_t b = r / kPbRows;
-                for (int32_t e = row[(size_t)r]; e < row[(size_t)r + 1]; ++e) {
-                    const int32_t u = col[(size_t)e];
-                    const int64_t c = u / kPbChunk;
-                    const int32_t p = cur[(size_t)(c * B + b)]++;
-                    src16[(size_t)p] = (uint16_t)(u - c * kPbChunk);
-                    srow[(size_t)p] = r;
-                }
-            }
-        });
-    }
-    row.clear();
-    row.shrink_to_fit();
-    col.clear();
-    col.shrink_to_fit();
-    // stretch masks and value slots per run; padding slots join the stretch
-    // before them (they add the zero LDS entry)
-    std::vector<uint8_t> mask((size_t)std::max<int64_t>(groups, 1), 0);
-    std::vector<int32_t> outs((size_t)(S * B), 0);  // value slots per run, unpadded
-    host_parallel_for(S * B, [&](int64_t k0, int64_t k1) {
-        for (int64_t k = k0; k < k1; ++k) {
-            const int64_t g0 = soff[(size_t)k] / kPbVec;
-            const int64_t g1 = (k + 1 < S * B ? soff[(size_t)k + 1] : slots) / kPbVec;
-            int32_t o = 0;
-            for (int64_t gi = g0; gi < g1; ++gi) {
-                uint8_t mk = 1;
-                for (int j = 1; j < kPbVec; ++j) {
-                    const int32_t rj = srow[(size_t)(gi * kPbVec + j)];
-                    if (rj >= 0 && rj != srow[(size_t)(gi * kPbVec + j - 1)]) mk |= (uint8_t)(1u << j);
-                }
-                mask[(size_t)gi] = mk;
-                o += __builtin_popcount(mk);
-            }
-            outs[(size_t)k] = o;
-        }
-    });
-    // value-slot starts, bin-major (sub-bin b, chunk c), each run padded
-    std::vector<int32_t> voff((size_t)(S * B));  // indexed like cnt: c*B + b
-    int64_t nvals = 0;
-    for (int64_t b = 0; b < B; ++b)
-        for (int64_t c = 0; c < S; ++c) {
-            voff[(size_t)(c * B + b)] = (int32_t)nvals;
-            nvals += (outs[(size_t)(c * B + b)] + kPbVec - 1) / kPbVec * kPbVec;
-        }
+
+    // source slots (padding: the zero LDS entry kPbChunk) and their rows
+    // (padding: -1)
+    DevBuf srow;
+    EGX_TRY(bad.alloc(4));
+    EGX_TRY(g.pb_src.alloc((size_t)slots * 2));
+    EGX_TRY(srow.alloc((size_t)slots * 4));
+    EGX_HIP_TRY(hipMemsetAsync(bad.p, 0, 4, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemsetAsync(srow.p, 0xFF, (size_t)slots * 4, s), EGX_DEVICE_ERR);
+    k_fill_u16<<<nblocks(slots, tb), tb, 0, s>>>(g.pb_src.as<uint16_t>(), slots, (uint16_t)kPbChunk);
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    k_pb_place<<<nblocks(m, tb), tb, 0, s>>>(keys.as<uint32_t>(), vals.as<uint32_t>(), lb.as<int32_t>(),
+                                             soff.as<uint32_t>(), g.it_col, rowof.as<int32_t>(), m, runs, B,
+                                             g.pb_src.as<uint16_t>(), srow.as<int32_t>(), bad.as<int32_t>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    keys.release();
+    vals.release();
+    rowof.release();
+    lb.release();
+
+    // stretch masks; value slots per group, per run (bin-major), per group
+    DevBuf gs, voff;
+    EGX_TRY(g.pb_mask.alloc((size_t)groups));
+    EGX_TRY(gs.alloc((size_t)(groups + 1) * 4));
+    k_pb_masks<<<nblocks(groups + 1, tb), tb, 0, s>>>(srow.as<int32_t>(), groups, g.pb_mask.as<uint8_t>(),
+                                                      gs.as<uint32_t>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_TRY(exclusive_scan_u32(gs.as<uint32_t>(), gs.as<uint32_t>(), groups + 1, s));
+    EGX_TRY(voff.alloc((size_t)(runs + 1) * 4));
+    k_pb_run_values<<<nblocks(runs + 1, tb), tb, 0, s>>>(soff.as<uint32_t>(), gs.as<uint32_t>(), runs, S, B,
+                                                         voff.as<uint32_t>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    EGX_TRY(exclusive_scan_u32(voff.as<uint32_t>(), voff.as<uint32_t>(), runs + 1, s));
+    std::vector<uint32_t> voff_h((size_t)runs + 1);  // bin-major: sub-bin b, chunk c at b*S + c
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(voff_h.data(), voff.p, voff_h.size() * 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+    const int64_t nvals = voff_h[(size_t)runs];
     if (nvals > INT32_MAX) {
         set_last_error("propagation-blocked layout: too many value slots");
         return EGX_UNSUPPORTED_GRAPH;
     }
-    // first value slot of every group, and the row of every value slot
-    std::vector<int32_t> vgrp((size_t)std::max<int64_t>(groups, 1));
-    std::vector<uint16_t> dst16((size_t)std::max<int64_t>(nvals, 1), 0);
-    host_parallel_for(S * B, [&](int64_t k0, int64_t k1) {
-        for (int64_t k = k0; k < k1; ++k) {
-            const int64_t b = k % B;
-            const int64_t g0 = soff[(size_t)k] / kPbVec;
-            const int64_t g1 = (k + 1 < S * B ? soff[(size_t)k + 1] : slots) / kPbVec;
-            int32_t o = voff[(size_t)k];
-            for (int64_t gi = g0; gi < g1; ++gi) {
-                vgrp[(size_t)gi] = o;
-                const uint8_t mk = mask[(size_t)gi];
-                for (int j = 0; j < kPbVec; ++j)
-                    if (mk & (1u << j))
-                        dst16[(size_t)o++] = (uint16_t)(srow[(size_t)(gi * kPbVec + j)] - b * kPbRows);
-            }
-        }
-    });
-    srow.clear();
-    srow.shrink_to_fit();
+    EGX_TRY(g.pb_vgrp.alloc((size_t)groups * 4));
+    EGX_TRY(g.pb_dst.alloc((size_t)nvals * 2));
+    EGX_TRY(g.pb_vals.alloc((size_t)nvals * 4));
+    // padding value slots are never written: rows 0, values 0
+    EGX_HIP_TRY(hipMemsetAsync(g.pb_dst.p, 0, (size_t)nvals * 2, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemsetAsync(g.pb_vals.p, 0, (size_t)nvals * 4, s), EGX_DEVICE_ERR);
+    k_pb_groups<<<nblocks(groups, tb), tb, 0, s>>>(soff.as<uint32_t>(), gs.as<uint32_t>(), voff.as<uint32_t>(),
+                                                   g.pb_mask.as<uint8_t>(), srow.as<int32_t>(), groups, runs, S, B,
+                                                   g.pb_vgrp.as<int32_t>(), g.pb_dst.as<uint16_t>(),
+                                                   bad.as<int32_t>());
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    int32_t bad_h = 0;
+    EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpy(&bad_h, bad.p, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
+
     // scatter work: contiguous group ranges of at most kPbScatterItem slots per chunk
     std::vector<PbScatter> scat;
     for (int64_t c = 0; c < S; ++c) {
-        const int32_t c0 = soff[(size_t)(c * B)] / kPbVec;
-        const int32_t c1 = (c + 1 < S) ? soff[(size_t)((c + 1) * B)] / kPbVec : (int32_t)groups;
+        const int32_t c0 = (int32_t)(soff_h[(size_t)(c * B)] / kPbVec);
+        const int32_t c1 = (int32_t)(soff_h[(size_t)((c + 1) * B)] / kPbVec);
         for (int32_t q0 = c0; q0 < c1; q0 += kPbScatterItem / kPbVec)
             scat.push_back(PbScatter{(int32_t)c, q0, std::min(c1, q0 + kPbScatterItem / kPbVec), 0});
     }
@@ -478,8 +590,8 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     std::vector<HubRow> hrows;
     int64_t parts = 0;
     for (int64_t b = 0; b < B; ++b) {
-        const int32_t e0 = voff[(size_t)b];  // chunk 0 of sub-bin b
-        const int32_t e1 = (b + 1 < B) ? voff[(size_t)(b + 1)] : (int32_t)nvals;
+        const int32_t e0 = (int32_t)voff_h[(size_t)(b * S)];  // chunk 0 of sub-bin b
+        const int32_t e1 = (int32_t)voff_h[(size_t)((b + 1) * S)];
         const int32_t r0 = (int32_t)(b * kPbRows);
         const int32_t nr = std::min<int32_t>(kPbRows, n - r0);
         if (e1 - e0 <= kPbItem) {
@@ -500,21 +612,9 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
         for (int32_t i = 0; i < nr; ++i) hrows.push_back(HubRow{r0 + i, (int32_t)(parts + (int64_t)i * np), np, 0});
         parts += (int64_t)nr * np;
     }
-    // every range the kernels will touch, checked before any upload
-    std::atomic<bool> ok_slots{true};
-    host_parallel_for(groups, [&](int64_t a0, int64_t a1) {
-        bool good = true;
-        for (int64_t k = a0; k < a1; ++k)
-            good = good && vgrp[(size_t)k] >= 0 &&
-                   (int64_t)vgrp[(size_t)k] + __builtin_popcount(mask[(size_t)k]) <= nvals && (mask[(size_t)k] & 1u);
-        if (!good) ok_slots = false;
-    });
-    host_parallel_for(nvals, [&](int64_t a0, int64_t a1) {
-        bool good = true;
-        for (int64_t q = a0; q < a1; ++q) good = good && dst16[(size_t)q] < kPbRows;
-        if (!good) ok_slots = false;
-    });
-    bool ok = ok_slots;
+    // every range the kernels will touch, checked before any item upload
+    // (the groups' value slots and rows were checked on the device)
+    bool ok = bad_h == 0;
     for (const PbScatter& it : scat)
         ok = ok && it.chunk >= 0 && (int64_t)it.chunk * kPbChunk < n && it.g0 >= 0 && it.g0 < it.g1 &&
              it.g1 <= groups;
@@ -532,21 +632,10 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
         set_last_error("propagation-blocked layout failed its range check");
         return EGX_UNKNOW_ERROR;
     }
-    EGX_TRY(g.pb_src.alloc((size_t)slots * 2));
-    EGX_TRY(g.pb_mask.alloc(mask.size()));
-    EGX_TRY(g.pb_vgrp.alloc(vgrp.size() * 4));
-    EGX_TRY(g.pb_dst.alloc(dst16.size() * 2));
-    EGX_TRY(g.pb_vals.alloc(dst16.size() * 4));
     EGX_TRY(g.pb_scatter.alloc(std::max<size_t>(scat.size(), 1) * sizeof(PbScatter)));
     EGX_TRY(g.pb_sum.alloc(std::max<size_t>(sums.size(), 1) * sizeof(PbSum)));
     EGX_TRY(g.pb_hrows.alloc(std::max<size_t>(hrows.size(), 1) * sizeof(HubRow)));
     EGX_TRY(g.pb_hub_part.alloc((size_t)std::max<int64_t>(parts, 1) * 8));
-    EGX_HIP_TRY(hipMemcpy(g.pb_src.p, src16.data(), (size_t)slots * 2, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpy(g.pb_mask.p, mask.data(), mask.size(), hipMemcpyHostToDevice), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpy(g.pb_vgrp.p, vgrp.data(), vgrp.size() * 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpy(g.pb_dst.p, dst16.data(), dst16.size() * 2, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
-    // padding value slots are never written: they stay 0
-    EGX_HIP_TRY(hipMemset(g.pb_vals.p, 0, dst16.size() * 4), EGX_DEVICE_ERR);
     if (!scat.empty())
         EGX_HIP_TRY(hipMemcpy(g.pb_scatter.p, scat.data(), scat.size() * sizeof(PbScatter), hipMemcpyHostToDevice),
                     EGX_DEVICE_ERR);
