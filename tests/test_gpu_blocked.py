@@ -126,6 +126,10 @@ def test_blocked_matches_pull_at_scale_22(gpu_ready):
     # repeat bit for bit, and sum to 1
     g = native.DeviceGraph.rmat(22, 16, seed=5)
     _say(f"rmat22: n={g.info()['n']} m={g.info()['m']}")
+    # 65M arcs: blocked by default; the pull layout on request
+    assert g.info()["blocked"] == 1
+    g.set_bins(blocked=0)
+    assert g.info()["blocked"] == 0
     a, ia, _ = g.pagerank(0.85, 10, 0.0)
     t, it_t, _ = g.pagerank(0.85, 100, 1e-6)
     g.set_bins(blocked=1)
@@ -139,6 +143,25 @@ def test_blocked_matches_pull_at_scale_22(gpu_ready):
     _assert_close(u, t, "rmat22 blocked vs pull tol 1e-6", tol=1e-9, l1=1e-7)
     assert abs(b.sum() - 1.0) < 1e-9
     g.close()
+
+
+def test_blocked_default_policy(gpu_ready):
+    # unweighted graphs of at least 2^23 arcs start blocked; smaller or
+    # weighted ones start on the pull layout; set_bins overrides either way
+    small = native.DeviceGraph.rmat(18, 16, seed=2)
+    assert small.info()["m"] < 1 << 23 and small.info()["blocked"] == 0
+    small.close()
+    big = native.DeviceGraph.rmat(20, 16, seed=2)
+    assert big.info()["m"] >= 1 << 23 and big.info()["blocked"] == 1
+    a, ia, _ = big.pagerank(0.85, 100, 1e-6)
+    big.set_bins(blocked=0)
+    b, ib, _ = big.pagerank(0.85, 100, 1e-6)
+    assert ia == ib
+    _assert_close(a, b, "rmat20 default (blocked) vs pull", tol=1e-9, l1=1e-7)
+    big.close()
+    w = native.DeviceGraph.rmat(20, 16, seed=2, weighted=True, wseed=3)
+    assert w.info()["blocked"] == 0
+    w.close()
 
 
 def test_blocked_graph_in_multi_graph_and_batch_paths(gpu_ready):
@@ -195,6 +218,11 @@ def _edge_shape(name):
         n = 300000
         src = np.arange(n)
         dst = np.full(n, 5)
+    elif name == "huge_n_few_arcs":
+        # 3M nodes and 500 arcs: almost every (chunk, sub-bin) run is empty
+        n = 3000000
+        src = rng.integers(0, n, 500)
+        dst = rng.integers(0, n, 500)
     else:  # self_loops: only i -> i
         n = 5000
         src = np.arange(n)
@@ -206,7 +234,7 @@ def _edge_shape(name):
     return O.simple_csr_from_coo(n, src[keep], dst[keep])
 
 
-@pytest.mark.parametrize("name", ["chunk_plus_one", "sparse_bins", "one_row", "self_loops"])
+@pytest.mark.parametrize("name", ["chunk_plus_one", "sparse_bins", "one_row", "self_loops", "huge_n_few_arcs"])
 def test_blocked_edge_shapes(gpu_ready, name):
     V, E = _edge_shape(name)
     _say(f"{name}: n={len(V) - 1} m={len(E)}")

@@ -382,11 +382,15 @@ def test_scale22_csc_invariants(gpu_ready):
     assert np.array_equal(dang, (np.diff(V) == 0).astype(np.int32))
 
 
-def test_scale24_solve_properties(gpu_ready):
+@pytest.mark.parametrize("blocked", [1, 0])
+def test_scale24_solve_properties(gpu_ready, blocked):
     """C2 size: 1 iteration at tol=1e-6 (err < tol*n for n > 2e6), ranks sum
-    to 1, and FP32 iterates stay within 1e-9 of FP64 ones after 20 iterations."""
-    _say("scale 24 build")
+    to 1, and FP32 iterates stay within 1e-9 of FP64 ones after 20 iterations,
+    on the blocked iteration (the default at this size) and on the pull SpMV."""
+    _say(f"scale 24 build, blocked={blocked}")
     g = native.DeviceGraph.rmat(24, 16, seed=1)
+    g.set_bins(blocked=blocked)
+    assert g.info()["blocked"] == blocked
     pr, it, _ = g.pagerank(0.85, 100, 1e-6)
     assert it == 1 and abs(pr.sum() - 1.0) < 1e-9
     p32, _, _ = g.pagerank(0.85, 20, 0.0)
@@ -396,9 +400,12 @@ def test_scale24_solve_properties(gpu_ready):
     assert err.max() < 1e-9 and err.sum() < 1e-5
 
 
-def test_scale20_fixed_k_vs_openmp_port(gpu_ready):
+@pytest.mark.parametrize("blocked", [1, 0])
+def test_scale20_fixed_k_vs_openmp_port(gpu_ready, blocked):
     V, E = O.rmat_csr(20, 16, seed=1)
     g = native.DeviceGraph.from_csr(V, E)
+    g.set_bins(blocked=blocked)
+    assert g.info()["blocked"] == blocked
     pr, it, _ = g.pagerank(0.85, 20, 0.0)
     omp, it_omp = O.pagerank_omp(V, E, None, 0.85, 20, 0.0, threads=16)
     assert it == it_omp == 20
