@@ -12,8 +12,8 @@ The knobs are those of egx_graph_set_bins:
 * hot_sources    -- sources past this relabelled id are gathered with
   non-temporal loads, keeping the hot prefix resident (0: off);
 * blocked        -- propagation-blocked iterations instead of the pull
-  SpMV (unweighted graphs, FP32 iterates): tried once, after the pull knobs,
-  on the best pull setting's relabelling.
+  SpMV (FP32 iterates): tried once, after the pull knobs, on the best pull
+  setting's relabelling.
 
 ``tune`` runs a coordinate descent over candidate values, timing the SpMV and
 reduce launches of a few fixed iterations with hipEvents on the graph's own
@@ -114,7 +114,7 @@ def tune(g, precision=native.PRECISION_FP32, iters=5, passes=2, space=None, star
         if not improved:
             break
     info = g.info()
-    if not info["weighted"] and precision == native.PRECISION_FP32 and info["m"] > 0:
+    if precision == native.PRECISION_FP32 and info["m"] > 0:
         cand = dict(cur, blocked=1)
         c = cost(cand)
         if c < best * 0.99:

@@ -41,12 +41,18 @@ def batched_ppr(scale=22, columns=64, K=20, seed=1, plain_sample=64):
     wall = time.perf_counter() - t0
     assert (it == K).all()
     sample = sets[: max(1, min(plain_sample, columns))]
-    # one vector per call on the plain SpMV (egx_graph_ppr_single)
-    g.ppr_single(sample[0], max_iter=2, tol=0.0, download=False)  # warm
-    t0 = time.perf_counter()
-    for S in sample:
-        g.ppr_single(S, max_iter=K, tol=0.0, download=False)
-    plain_wall = time.perf_counter() - t0
+    # one vector per call (egx_graph_ppr_single) on the graph's default
+    # iteration (propagation-blocked at this size), then on the pull SpMV
+    plain_blocked = g.info()["blocked"]
+    plain_walls = []
+    for blocked in (plain_blocked, 0):
+        g.set_bins(blocked=blocked)
+        g.ppr_single(sample[0], max_iter=2, tol=0.0, download=False)  # warm
+        t0 = time.perf_counter()
+        for S in sample:
+            g.ppr_single(S, max_iter=K, tol=0.0, download=False)
+        plain_walls.append(time.perf_counter() - t0)
+    plain_wall, pull_wall = plain_walls
     # and through a resident single-column batch session (the SpMM engine at width 8)
     with g.batch() as b:
         b.assign(0, sample[0], max_iter=2, tol=0.0)
@@ -70,8 +76,11 @@ def batched_ppr(scale=22, columns=64, K=20, seed=1, plain_sample=64):
         "batch_kernel_ms_per_iter": round(ks / K * 1e3, 4),
         "batch_algorithmic_GBps": round(bytes_iter * K / ks / 1e9, 2),
         "plain_solves_per_s": round(plain_rate, 3),
+        "plain_blocked": int(plain_blocked),
         "plain_calls_timed": len(sample),
         "speedup_vs_plain": round(batch_rate / plain_rate, 2),
+        "plain_pull_solves_per_s": round(len(sample) / pull_wall, 3),
+        "speedup_vs_plain_pull": round(batch_rate * pull_wall / len(sample), 2),
         "single_column_session_solves_per_s": round(len(sample) / session_wall, 3),
     }
     g.close()

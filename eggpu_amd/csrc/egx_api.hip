@@ -246,7 +246,7 @@ size_t Graph::device_bytes() const {
                             &hrows,  &hub_part, &x[0],  &x[1],
                             &s[0],   &s[1],   &state,  &partials, &red_part, &out,    &tele,
                             &pb_src, &pb_dst, &pb_vals, &pb_vgrp, &pb_mask, &pb_scatter, &pb_sum, &pb_hrows,
-                            &pb_hub_part};
+                            &pb_hub_part, &pb_w, &pb_arc};
     size_t total = 0;
     for (const DevBuf* b : bufs) total += b->bytes;
     return total;

@@ -347,7 +347,7 @@ __global__ void k_pb_place(const uint32_t* __restrict__ keys, const uint32_t* __
                            const int32_t* __restrict__ lb, const uint32_t* __restrict__ soff,
                            const int32_t* __restrict__ col, const int32_t* __restrict__ rowof, int64_t m,
                            int64_t runs, int64_t B, uint16_t* __restrict__ src16, int32_t* __restrict__ srow,
-                           int32_t* __restrict__ bad) {
+                           int32_t* __restrict__ arc, int32_t* __restrict__ bad) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const uint32_t k = keys[i];
@@ -364,6 +364,16 @@ __global__ void k_pb_place(const uint32_t* __restrict__ keys, const uint32_t* __
     }
     src16[p] = (uint16_t)u;
     srow[p] = rowof[e];
+    if (arc) arc[p] = (int32_t)e;
+}
+
+// FP32 arc value of every source slot of a weighted graph (padding: 0).
+__global__ void k_pb_gather_w(const int32_t* __restrict__ arc, const float* __restrict__ val, int64_t slots,
+                              int64_t m, float* __restrict__ w) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= slots) return;
+    const int32_t e = arc[p];
+    w[p] = (e >= 0 && (int64_t)e < m) ? val[e] : 0.0f;
 }
 
 // Stretch mask of every kPbVec-slot group (bit j: slot j starts a stretch of
@@ -487,14 +497,10 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     if (cfg.blocked && g.pb_ready && !layout_changed) return EGX_SUCC;
     g.pb_ready = false;
     for (DevBuf* d : {&g.pb_src, &g.pb_dst, &g.pb_vals, &g.pb_vgrp, &g.pb_mask, &g.pb_scatter, &g.pb_sum,
-                      &g.pb_hrows, &g.pb_hub_part})
+                      &g.pb_hrows, &g.pb_hub_part, &g.pb_w, &g.pb_arc})
         d->release();
     g.n_pb_scatter = g.n_pb_sum = g.n_pb_hrows = 0;
     if (!cfg.blocked || g.n <= 0 || g.m <= 0) return EGX_SUCC;
-    if (g.weighted) {
-        set_last_error("propagation-blocked iterations take unweighted graphs");
-        return EGX_UNSUPPORTED_GRAPH;
-    }
     const int32_t n = g.n;
     const int64_t m = g.m;
     const int64_t S = (n + kPbChunk - 1) / kPbChunk;
@@ -548,12 +554,23 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     EGX_TRY(srow.alloc((size_t)slots * 4));
     EGX_HIP_TRY(hipMemsetAsync(bad.p, 0, 4, s), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemsetAsync(srow.p, 0xFF, (size_t)slots * 4, s), EGX_DEVICE_ERR);
+    if (g.weighted) {
+        EGX_TRY(g.pb_arc.alloc((size_t)slots * 4));
+        EGX_HIP_TRY(hipMemsetAsync(g.pb_arc.p, 0xFF, (size_t)slots * 4, s), EGX_DEVICE_ERR);
+    }
     k_fill_u16<<<nblocks(slots, tb), tb, 0, s>>>(g.pb_src.as<uint16_t>(), slots, (uint16_t)kPbChunk);
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     k_pb_place<<<nblocks(m, tb), tb, 0, s>>>(keys.as<uint32_t>(), vals.as<uint32_t>(), lb.as<int32_t>(),
                                              soff.as<uint32_t>(), g.it_col, rowof.as<int32_t>(), m, runs, B,
-                                             g.pb_src.as<uint16_t>(), srow.as<int32_t>(), bad.as<int32_t>());
+                                             g.pb_src.as<uint16_t>(), srow.as<int32_t>(),
+                                             g.weighted ? g.pb_arc.as<int32_t>() : nullptr, bad.as<int32_t>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    if (g.weighted) {
+        EGX_TRY(g.pb_w.alloc((size_t)slots * 4));
+        k_pb_gather_w<<<nblocks(slots, tb), tb, 0, s>>>(g.pb_arc.as<int32_t>(), g.it_val, slots, m,
+                                                        g.pb_w.as<float>());
+        EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    }
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     keys.release();
     vals.release();
@@ -852,9 +869,9 @@ int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const 
     vals.release();
     g.layout_built = false;
     EGX_TRY(rebuild_bins(g, g.bins));
-    // large unweighted PageRank graphs iterate propagation-blocked; one the
-    // layout cannot hold keeps the pull layout alone
-    if (!g.weighted && !g.adjacency && m >= kPbDefaultArcs && !g.bins.blocked) {
+    // large PageRank graphs iterate propagation-blocked; one the layout
+    // cannot hold keeps the pull layout alone
+    if (!g.adjacency && m >= kPbDefaultArcs && !g.bins.blocked) {
         BinCfg cfg = g.bins;
         cfg.blocked = 1;
         if (build_blocked(g, cfg, true) == EGX_SUCC) {
@@ -982,6 +999,12 @@ int refresh_alpha(Graph& g, double alpha) {
             k_tile_vals<<<(unsigned)g.n_tiles, 256, 0, g.stream>>>(g.it_row, g.tiles.as<int32_t>(),
                                                                    g.t_pos.as<uint16_t>(), g.it_val,
                                                                    g.t_val.as<float>());
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+        }
+        if (g.pb_ready && g.pb_w.p) {
+            const int64_t slots = (int64_t)(g.pb_w.bytes / 4);
+            k_pb_gather_w<<<nblocks(slots, 256), 256, 0, g.stream>>>(g.pb_arc.as<int32_t>(), g.it_val, slots, g.m,
+                                                                     g.pb_w.as<float>());
             EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
         }
     }

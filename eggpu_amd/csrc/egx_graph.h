@@ -106,8 +106,8 @@ constexpr int32_t kPbRows = 2048;
 constexpr int32_t kPbVec = 8;
 constexpr int32_t kPbScatterItem = 262144;
 constexpr int32_t kPbItem = 131072;
-// unweighted PageRank graphs with at least this many arcs start with
-// bins.blocked = 1 (measured faster than the pull SpMV from R-MAT 18 up)
+// PageRank graphs with at least this many arcs start with bins.blocked = 1
+// (measured faster than the pull SpMV from R-MAT 18 up, weighted or not)
 constexpr int64_t kPbDefaultArcs = (int64_t)1 << 23;
 
 struct PbScatter {
@@ -146,9 +146,11 @@ struct Graph {
     // group of it, pb_mask uint8 (bit j: a stretch of one row starts at slot
     // j) and pb_vgrp int32 (its first value slot); per value slot, in
     // (sub-bin, chunk) order, pb_dst uint16 (sub-bin-local row) and pb_vals
-    // (the per-iteration FP32 term)
+    // (the per-iteration FP32 term); weighted graphs add pb_w, the FP32 arc
+    // value of every source slot (0 for padding), and pb_arc, its arc index
+    // (-1 for padding) for refreshing pb_w after an alpha change
     bool pb_ready = false;
-    DevBuf pb_src, pb_dst, pb_vals, pb_vgrp, pb_mask, pb_scatter, pb_sum, pb_hrows, pb_hub_part;
+    DevBuf pb_src, pb_dst, pb_vals, pb_vgrp, pb_mask, pb_scatter, pb_sum, pb_hrows, pb_hub_part, pb_w, pb_arc;
     int64_t n_pb_scatter = 0, n_pb_sum = 0, n_pb_hrows = 0;
     double alpha_vals = 0.0;   // alpha baked into it_val
     double alpha_orig = 0.0;   // alpha baked into in_val

@@ -395,6 +395,7 @@ struct PbArgs {
     const uint8_t* mask;
     const PbScatter* scatter;
     const PbSum* sums;
+    const float* w;  // weighted graphs: FP32 arc value per source slot
 };
 
 constexpr int kPbThreads = 1024;  // one scatter workgroup per CU: its chunk takes 128 KB of LDS
@@ -430,15 +431,21 @@ __device__ __forceinline__ int32_t wave_excl_scan_small(uint32_t x, int32_t& tot
 }
 
 // The stretch terms of one group (mask bit j: a stretch starts at slot j),
-// written to o[0], o[1], ...
-template <typename P>
-__device__ __forceinline__ void pb_emit(const float* sc, pb_u32x4 u, uint32_t mk, P* o) {
+// written to o[0], o[1], ...  Weighted (W): slot j's term is the FP32
+// product of its arc value and the staged x, as in the pull kernel.
+template <bool W, typename P>
+__device__ __forceinline__ void pb_emit(const float* sc, pb_u32x4 u, pb_f32x4 wa, pb_f32x4 wb, uint32_t mk, P* o) {
     const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+    const float wt[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
     float v[8];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
         v[2 * h] = sc[w[h] & 0xffffu];
         v[2 * h + 1] = sc[w[h] >> 16];
+    }
+    if (W) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = wt[j] * v[j];
     }
     double run = (double)v[0];
     int32_t k = 0;
@@ -460,7 +467,7 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T>
+template <typename T, bool W>
 __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs pb) {
     __shared__ float sc[kPbChunk + 4];
     __shared__ float stage[kPbThreads / kWave][kPbStage];
@@ -469,10 +476,13 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
     const PbScatter it = pb.scatter[blockIdx.x];
     const int32_t base = it.chunk * kPbChunk;
     const int32_t cnt = min(kPbChunk, a.n - base);
-    for (int32_t i = threadIdx.x; i < cnt; i += kPbThreads) sc[i] = (float)a.s_in[base + i];
+    // staged sources: x (weighted) or s = x/deg
+    const T* sv = W ? a.x_in : a.s_in;
+    for (int32_t i = threadIdx.x; i < cnt; i += kPbThreads) sc[i] = (float)sv[base + i];
     if (threadIdx.x == 0) sc[kPbChunk] = 0.0f;
     __syncthreads();
     const pb_u32x4* src = reinterpret_cast<const pb_u32x4*>(pb.src);
+    const pb_f32x4* wts = reinterpret_cast<const pb_f32x4*>(pb.w);
     float* stg = stage[wave_id()];
     const int l = lane_id();
     // every lane of a wavefront runs the loop the same number of times (the
@@ -481,6 +491,7 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
     const int32_t wave0 = it.g0 + (int32_t)(threadIdx.x - l);
     for (int32_t g0 = wave0; g0 < it.g1; g0 += kPbGroups * kPbThreads) {
         pb_u32x4 u[kPbGroups];
+        pb_f32x4 wa[kPbGroups], wb[kPbGroups];
         int32_t vg[kPbGroups];
         uint32_t mk[kPbGroups];
 #pragma unroll
@@ -488,6 +499,8 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
             const int32_t gi = g0 + q * kPbThreads + l;
             const bool ok = gi < it.g1;
             u[q] = ok ? __builtin_nontemporal_load(&src[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
+            wa[q] = (W && ok) ? __builtin_nontemporal_load(&wts[2 * gi]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+            wb[q] = (W && ok) ? __builtin_nontemporal_load(&wts[2 * gi + 1]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
             vg[q] = ok ? __builtin_nontemporal_load(&pb.vgrp[gi]) : -1;
             mk[q] = ok ? (uint32_t)__builtin_nontemporal_load(&pb.mask[gi]) : 0u;
         }
@@ -499,12 +512,12 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
             const int32_t first = __builtin_amdgcn_readfirstlane(vg[q] - off);  // lane 0: every lane runs here
             const bool contig = total > 0 && total <= kPbStage && __all(!ok || vg[q] - off == first);
             if (contig) {
-                if (ok) pb_emit(sc, u[q], mk[q], stg + off);
+                if (ok) pb_emit<W>(sc, u[q], wa[q], wb[q], mk[q], stg + off);
                 wave_lds_sync();
                 for (int32_t j = l; j < total; j += kWave) pb.vals[first + j] = stg[j];
                 wave_lds_sync();
             } else if (ok) {
-                pb_emit(sc, u[q], mk[q], pb.vals + vg[q]);
+                pb_emit<W>(sc, u[q], wa[q], wb[q], mk[q], pb.vals + vg[q]);
             }
         }
     }
@@ -517,7 +530,7 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
 // its accumulators, and a lane adds its kPbVec slots in slot order (each
 // stretch of one row summed in registers first), so the order of the adds
 // is fixed.  Padding slots add 0 to row 0.
-template <typename T>
+template <typename T, bool W>
 __global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
     __shared__ double acc[kPbRows];
     if (a.state->done) return;
@@ -569,7 +582,7 @@ __global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
     double err = 0.0, sum = 0.0, dang = 0.0;
     if (it.piece < 0) {
         const double jump = jump_of(a);
-        for (int32_t i = l; i < nr; i += kWave) finalize_row<T, false>(a, r0 + i, acc[i], jump, err, sum, dang);
+        for (int32_t i = l; i < nr; i += kWave) finalize_row<T, W>(a, r0 + i, acc[i], jump, err, sum, dang);
         err = wave_sum(err);
         sum = wave_sum(sum);
         dang = wave_sum(dang);
@@ -895,10 +908,11 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
     hipStream_t s = g.stream;
     IterArgs<T> a;
     EGX_TRY(prepare_graph<T>(g, p, s, a));
-    // propagation-blocked iterations: FP32 iterates on an unweighted graph
-    // with the blocked layout; split sub-bins go through the hub-row reduce
-    constexpr bool kBlockable = std::is_same<T, float>::value && !W;
-    const bool blocked = kBlockable && g.pb_ready;
+    // propagation-blocked iterations: FP32 iterates on a graph with the
+    // blocked layout (weighted: with its per-slot arc values); split
+    // sub-bins go through the hub-row reduce
+    constexpr bool kBlockable = std::is_same<T, float>::value;
+    const bool blocked = kBlockable && g.pb_ready && (!W || g.pb_w.p != nullptr);
     PbArgs pb{};
     if (blocked) {
         EGX_TRY(ensure_workspace(g, (uint32_t)g.n_pb_sum));
@@ -917,14 +931,15 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
         pb.mask = g.pb_mask.as<uint8_t>();
         pb.scatter = g.pb_scatter.as<PbScatter>();
         pb.sums = g.pb_sum.as<PbSum>();
+        pb.w = W ? g.pb_w.as<float>() : nullptr;
     }
     const uint32_t grid = a.grid;
     // the SpMV of one iteration: pull kernel, or scatter + sum
     auto launch_spmv = [&]() {
         if constexpr (kBlockable) {
             if (blocked) {
-                k_pb_scatter<T><<<(uint32_t)g.n_pb_scatter, kPbThreads, 0, s>>>(a, pb);
-                k_pb_sum<T><<<(uint32_t)g.n_pb_sum, kWave, 0, s>>>(a, pb);
+                k_pb_scatter<T, W><<<(uint32_t)g.n_pb_scatter, kPbThreads, 0, s>>>(a, pb);
+                k_pb_sum<T, W><<<(uint32_t)g.n_pb_sum, kWave, 0, s>>>(a, pb);
                 return;
             }
         }

@@ -110,12 +110,57 @@ def test_blocked_single_vector_ppr_and_fp64(gpu_ready):
     g.close()
 
 
-def test_blocked_refuses_weighted_graphs(gpu_ready):
-    V, E = O.rmat_csr(10, 16, seed=2)
-    W = O.edge_weights(V, E, 1)
+WEIGHTED = {
+    "rmat14w": lambda: O.rmat_csr(14, 16, seed=41),
+    "in_star_w": lambda: _in_star_graph(43),
+    "er_w": lambda: O.er_csr(50000, 400000, seed=44),
+}
+
+
+@pytest.mark.parametrize("case", sorted(WEIGHTED))
+def test_blocked_weighted_matches_oracle_and_pull(gpu_ready, case):
+    # weighted graphs: each source slot carries its FP32 arc value
+    # fl32(alpha * w / outW); terms are FP32 products as in the pull kernel
+    V, E = WEIGHTED[case]()
+    W = O.edge_weights(V, E, 7)
+    _say(f"{case}: n={len(V) - 1} m={len(E)} weighted")
     g = native.DeviceGraph.from_csr(V, E, W)
-    with pytest.raises(Exception):
+    for tol, K in ((1e-6, 100), (0.0, 30)):
+        g.set_bins(blocked=0)
+        pr0, it0, _ = g.pagerank(0.85, K, tol)
         g.set_bins(blocked=1)
+        assert g.info()["blocked"] == 1
+        pr, it, _ = g.pagerank(0.85, K, tol)
+        ref, it_ref, _ = O.pagerank_csr(V, E, W, 0.85, K, tol)
+        _assert_close(pr, ref, f"{case} blocked weighted vs oracle tol={tol}")
+        assert it == it_ref == it0
+        _assert_close(pr, pr0, f"{case} blocked weighted vs pull", tol=1e-7, l1=1e-6)
+    # another alpha refreshes the per-slot values; then back again
+    for alpha in (0.9, 0.85):
+        pr, it, _ = g.pagerank(alpha, 200, 1e-6)
+        ref, it_ref, _ = O.pagerank_csr(V, E, W, alpha, 200, 1e-6)
+        _assert_close(pr, ref, f"{case} blocked weighted alpha={alpha}")
+        assert it == it_ref
+    g.close()
+
+
+def test_blocked_weighted_ppr_and_determinism(gpu_ready):
+    V, E = O.rmat_csr(15, 16, seed=45)
+    W = O.edge_weights(V, E, 8)
+    n = len(V) - 1
+    src, dst = O.csr_rows(V, E)
+    g = native.DeviceGraph.from_csr(V, E, W)
+    g.set_bins(blocked=1)
+    a, ia, _ = g.pagerank(0.85, 25, 0.0)
+    b, ib, _ = g.pagerank(0.85, 25, 0.0)
+    assert ia == ib and np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    seeds = [1, 99, 2048]
+    p = np.zeros(n)
+    p[seeds] = 1.0 / len(seeds)
+    out, it, _ = g.ppr_single(seeds, alpha=0.85, max_iter=200, tol=1e-6)
+    ref, it_ref = O.ppr_rows(n, src, dst, W, p, 0.85, 200, 1e-6)[:2]
+    _assert_close(out, ref, "blocked weighted ppr_single")
+    assert it == it_ref
     g.close()
 
 
@@ -146,8 +191,8 @@ def test_blocked_matches_pull_at_scale_22(gpu_ready):
 
 
 def test_blocked_default_policy(gpu_ready):
-    # unweighted graphs of at least 2^23 arcs start blocked; smaller or
-    # weighted ones start on the pull layout; set_bins overrides either way
+    # graphs of at least 2^23 arcs start blocked, weighted or not; smaller
+    # ones start on the pull layout; set_bins overrides either way
     small = native.DeviceGraph.rmat(18, 16, seed=2)
     assert small.info()["m"] < 1 << 23 and small.info()["blocked"] == 0
     small.close()
@@ -160,7 +205,12 @@ def test_blocked_default_policy(gpu_ready):
     _assert_close(a, b, "rmat20 default (blocked) vs pull", tol=1e-9, l1=1e-7)
     big.close()
     w = native.DeviceGraph.rmat(20, 16, seed=2, weighted=True, wseed=3)
-    assert w.info()["blocked"] == 0
+    assert w.info()["blocked"] == 1
+    a, ia, _ = w.pagerank(0.85, 100, 1e-6)
+    w.set_bins(blocked=0)
+    b, ib, _ = w.pagerank(0.85, 100, 1e-6)
+    assert ia == ib
+    _assert_close(a, b, "weighted rmat20 default (blocked) vs pull", tol=1e-9, l1=1e-7)
     w.close()
 
 
