@@ -51,12 +51,12 @@ def read_traffic(blocked):
     return d.get("hbm_bytes_per_launch"), os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(g, n, m, iters, threads):
+def cpu_baseline(g, n, m, iters, threads, weighted=False):
     from oracle import oracle as O
 
-    V, E, _ = g.download_csr(with_weights=False)
+    V, E, W = g.download_csr(with_weights=weighted)
     t0 = time.perf_counter()
-    _, it = O.pagerank_omp(V, E, None, 0.85, iters, 0.0, threads=threads)
+    _, it = O.pagerank_omp(V, E, W if weighted else None, 0.85, iters, 0.0, threads=threads)
     dt = time.perf_counter() - t0
     return {
         "value": round(m * it / dt / 1e9, 4),
@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default bins")
+    ap.add_argument("--weighted", action="store_true",
+                    help="U(0.1,1) arc weights (a secondary measurement; the headline graph is unweighted)")
     ap.add_argument("--bins", help='fixed bins instead of the tuner, JSON of egx_graph_set_bins fields')
     ap.add_argument("--sections", default="batched_ppr,multi_graph,serving,katz,eigenvector,ingest",
                     help="secondary workloads reported beside the headline line ('' for none)")
@@ -129,7 +131,8 @@ def main():
 
     log(f"building R-MAT scale {args.scale} on the device")
     t0 = time.perf_counter()
-    g = native.DeviceGraph.rmat(args.scale, 16, a=0.57, b=0.19, c=0.19, seed=1, keep_csr=not args.no_cpu)
+    g = native.DeviceGraph.rmat(args.scale, 16, a=0.57, b=0.19, c=0.19, seed=1, keep_csr=not args.no_cpu,
+                                weighted=args.weighted, wseed=2)
     build_wall = time.perf_counter() - t0
     info = g.info()
     n, m = int(info["n"]), int(info["m"])
@@ -167,10 +170,11 @@ def main():
     _, st_t = g.pagerank_stats(0.85, K, 0.0, time_each_launch=True)
     spmv_mean = st_t["spmv_seconds"] / st_t["spmv_launches"]
     reduce_mean = st_t["reduce_seconds"] / st_t["spmv_launches"]
-    bytes_iter = 4 * m + 16 * n
+    # weighted graphs also read a 4-byte arc value per arc
+    bytes_iter = (8 if args.weighted else 4) * m + 16 * n
     achieved = bytes_iter / spmv_mean
     blocked = bool(info["blocked"])
-    traffic, traffic_src = read_traffic(blocked)
+    traffic, traffic_src = (None, None) if args.weighted else read_traffic(blocked)
 
     # tol = 1e-6 solve (reference semantics) and the FP64 variant, beside
     pr, it_tol, _ = g.pagerank(0.85, 100, 1e-6)
@@ -180,7 +184,7 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {args.cpu_iters} iterations, {threads} threads")
-        cpu = cpu_baseline(g, n, m, args.cpu_iters, threads)
+        cpu = cpu_baseline(g, n, m, args.cpu_iters, threads, weighted=args.weighted)
 
     # the CPU-runnable configuration (R-MAT 14) on the GPU, with its CPU legs
     from eggpu_amd import benchmarks
@@ -219,7 +223,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded R-MAT generated on the device)",
         "config": {
-            "workload": f"pagerank R-MAT scale {args.scale} plain method, fixed K={K} (tol=0)",
+            "workload": f"pagerank R-MAT scale {args.scale}{' weighted U(0.1,1)' if args.weighted else ''} plain "
+                        f"method, fixed K={K} (tol=0)",
             "n": n,
             "m": m,
             "alpha": 0.85,
