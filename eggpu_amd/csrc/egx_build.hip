@@ -465,8 +465,9 @@ __global__ void k_pb_groups(const uint32_t* __restrict__ soff, const uint32_t* _
 // CUs and the heavy sub-bins of skewed graphs do not trail.  Items never
 // shrink below the minimums: a scatter item stages its chunk's 128 KB of
 // sources, and a sub-bin cut into pieces hands its row sums to the reduce
-// launch.  Measured on R-MAT 18 to 24 (ms per iteration, before -> after):
-// 0.125 -> 0.047, 0.163 -> 0.083, 0.243 -> 0.184, 0.609 -> 0.611.
+// launch.  Measured on R-MAT 18, 20, 22 and 24 (ms per iteration, fixed
+// 256K-slot scatter and 128K-slot sum items -> these sizes): 0.125 -> 0.047,
+// 0.163 -> 0.083, 0.243 -> 0.184, 0.609 -> 0.611.
 constexpr int64_t kPbScatterItems = 1024;
 constexpr int64_t kPbScatterMin = 32768;
 constexpr int64_t kPbSumItems = 512;
