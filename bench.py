@@ -38,11 +38,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def read_traffic(blocked):
+def read_traffic(blocked, weighted=False):
     """HBM bytes per SpMV launch (per scatter + sum pair when blocked) from
     the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by
-    the copy-kernel calibration)."""
-    name = "pmc_traffic_rmat24_blocked.json" if blocked else "pmc_traffic_rmat24.json"
+    the copy-kernel calibration); none is kept for the weighted pull SpMV."""
+    if weighted and not blocked:
+        return None, None
+    name = ("pmc_traffic_rmat24_weighted_blocked.json" if weighted else "pmc_traffic_rmat24_blocked.json") \
+        if blocked else "pmc_traffic_rmat24.json"
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, None
@@ -174,7 +177,7 @@ def main():
     bytes_iter = (8 if args.weighted else 4) * m + 16 * n
     achieved = bytes_iter / spmv_mean
     blocked = bool(info["blocked"])
-    traffic, traffic_src = (None, None) if args.weighted else read_traffic(blocked)
+    traffic, traffic_src = read_traffic(blocked, args.weighted) if args.scale == 24 else (None, None)
 
     # tol = 1e-6 solve (reference semantics) and the FP64 variant, beside
     pr, it_tol, _ = g.pagerank(0.85, 100, 1e-6)

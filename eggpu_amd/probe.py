@@ -57,7 +57,7 @@ def cmd_gather(a):
 
 def cmd_calib(a):
     _copy(a.bytes, 1)
-    g = native.DeviceGraph.rmat(a.scale, 16, seed=1)
+    g = native.DeviceGraph.rmat(a.scale, 16, seed=1, weighted=a.weighted, wseed=2)
     if a.bins:
         g.set_bins(**json.loads(a.bins))
     inf = g.info()
@@ -227,6 +227,7 @@ def main(argv=None):
     p.add_argument("--iters", type=int, default=3)
     p.add_argument("--bytes", type=int, default=1 << 30)
     p.add_argument("--bins", help='JSON of egx_graph_set_bins fields, e.g. {"tile_sort": 1}')
+    p.add_argument("--weighted", action="store_true", help="U(0.1,1) arc weights, as bench.py --weighted")
     p = sub.add_parser("stats")
     p.add_argument("db")
     p.add_argument("--out")
