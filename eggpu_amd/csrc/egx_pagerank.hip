@@ -467,57 +467,94 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The source slots of kPbGroups groups per lane, loaded together; the
+// weighted arrays only when W.
+// Weighted graphs carry 32 more bytes per group, so they keep half as
+// many groups per batch (the double-buffered batches then fit in registers).
+template <bool W>
+struct PbScatterBatch {
+    static constexpr int kG = W ? kPbGroups / 2 : kPbGroups;
+    pb_u32x4 u[kG];
+    pb_f32x4 wa[W ? kG : 1], wb[W ? kG : 1];
+    int32_t vg[kG];
+    uint32_t mk[kG];
+};
+
+template <bool W>
+__device__ __forceinline__ void pb_scatter_load(const PbArgs& pb, int32_t g0, int32_t g1, int l,
+                                                PbScatterBatch<W>& b) {
+    const pb_u32x4* src = reinterpret_cast<const pb_u32x4*>(pb.src);
+    const pb_f32x4* wts = reinterpret_cast<const pb_f32x4*>(pb.w);
+#pragma unroll
+    for (int q = 0; q < PbScatterBatch<W>::kG; ++q) {
+        const int32_t gi = g0 + q * kPbThreads + l;
+        const bool ok = gi < g1;
+        b.u[q] = ok ? __builtin_nontemporal_load(&src[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
+        if (W) {
+            b.wa[q] = ok ? __builtin_nontemporal_load(&wts[2 * gi]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+            b.wb[q] = ok ? __builtin_nontemporal_load(&wts[2 * gi + 1]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+        b.vg[q] = ok ? __builtin_nontemporal_load(&pb.vgrp[gi]) : -1;
+        b.mk[q] = ok ? (uint32_t)__builtin_nontemporal_load(&pb.mask[gi]) : 0u;
+    }
+}
+
 template <typename T, bool W>
 __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs pb) {
-    __shared__ float sc[kPbChunk + 4];
+    __shared__ __align__(16) float sc[kPbChunk + 4];
     __shared__ float stage[kPbThreads / kWave][kPbStage];
     if (a.state->done) return;
     bind(a);
     const PbScatter it = pb.scatter[blockIdx.x];
-    const int32_t base = it.chunk * kPbChunk;
-    const int32_t cnt = min(kPbChunk, a.n - base);
-    // staged sources: x (weighted) or s = x/deg
-    const T* sv = W ? a.x_in : a.s_in;
-    for (int32_t i = threadIdx.x; i < cnt; i += kPbThreads) sc[i] = (float)sv[base + i];
-    if (threadIdx.x == 0) sc[kPbChunk] = 0.0f;
-    __syncthreads();
-    const pb_u32x4* src = reinterpret_cast<const pb_u32x4*>(pb.src);
-    const pb_f32x4* wts = reinterpret_cast<const pb_f32x4*>(pb.w);
-    float* stg = stage[wave_id()];
     const int l = lane_id();
     // every lane of a wavefront runs the loop the same number of times (the
     // bound depends on the wavefront's first thread only), so the shuffles
     // below see all 64 lanes
     const int32_t wave0 = it.g0 + (int32_t)(threadIdx.x - l);
-    for (int32_t g0 = wave0; g0 < it.g1; g0 += kPbGroups * kPbThreads) {
-        pb_u32x4 u[kPbGroups];
-        pb_f32x4 wa[kPbGroups], wb[kPbGroups];
-        int32_t vg[kPbGroups];
-        uint32_t mk[kPbGroups];
+    constexpr int32_t kStep = PbScatterBatch<W>::kG * kPbThreads;
+    // the first batch of slots is in flight while the chunk is staged
+    PbScatterBatch<W> nxt;
+    pb_scatter_load<W>(pb, wave0, it.g1, l, nxt);
+    const int32_t base = it.chunk * kPbChunk;
+    const int32_t cnt = min(kPbChunk, a.n - base);
+    // staged sources: x (weighted) or s = x/deg; 16-byte loads for the
+    // whole quads (base is a multiple of kPbChunk, so they are aligned)
+    const T* sv = W ? a.x_in : a.s_in;
+    if constexpr (sizeof(T) == 4) {
+        const int32_t nq = cnt / 4;
+        const pb_f32x4* sv4 = reinterpret_cast<const pb_f32x4*>(sv + base);
+        pb_f32x4* sc4 = reinterpret_cast<pb_f32x4*>(sc);
+        for (int32_t i = threadIdx.x; i < nq; i += kPbThreads) sc4[i] = sv4[i];
+        for (int32_t i = 4 * nq + threadIdx.x; i < cnt; i += kPbThreads) sc[i] = (float)sv[base + i];
+    } else {
+        for (int32_t i = threadIdx.x; i < cnt; i += kPbThreads) sc[i] = (float)sv[base + i];
+    }
+    if (threadIdx.x == 0) sc[kPbChunk] = 0.0f;
+    __syncthreads();
+    float* stg = stage[wave_id()];
+    for (int32_t g0 = wave0; g0 < it.g1; g0 += kStep) {
+        // software pipelined: the next batch's loads are issued before this
+        // batch's LDS reads and term stores
+        const PbScatterBatch<W> cur = nxt;
+        if (g0 + kStep < it.g1) pb_scatter_load<W>(pb, g0 + kStep, it.g1, l, nxt);
 #pragma unroll
-        for (int q = 0; q < kPbGroups; ++q) {
-            const int32_t gi = g0 + q * kPbThreads + l;
-            const bool ok = gi < it.g1;
-            u[q] = ok ? __builtin_nontemporal_load(&src[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
-            wa[q] = (W && ok) ? __builtin_nontemporal_load(&wts[2 * gi]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-            wb[q] = (W && ok) ? __builtin_nontemporal_load(&wts[2 * gi + 1]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-            vg[q] = ok ? __builtin_nontemporal_load(&pb.vgrp[gi]) : -1;
-            mk[q] = ok ? (uint32_t)__builtin_nontemporal_load(&pb.mask[gi]) : 0u;
-        }
-#pragma unroll
-        for (int q = 0; q < kPbGroups; ++q) {
-            const bool ok = vg[q] >= 0;
+        for (int q = 0; q < PbScatterBatch<W>::kG; ++q) {
+            const int32_t vgq = cur.vg[q];
+            const uint32_t mkq = cur.mk[q];
+            const pb_f32x4 waq = W ? cur.wa[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+            const pb_f32x4 wbq = W ? cur.wb[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+            const bool ok = vgq >= 0;
             int32_t total = 0;
-            const int32_t off = wave_excl_scan_small((uint32_t)__builtin_popcount(mk[q]), total);
-            const int32_t first = __builtin_amdgcn_readfirstlane(vg[q] - off);  // lane 0: every lane runs here
-            const bool contig = total > 0 && total <= kPbStage && __all(!ok || vg[q] - off == first);
+            const int32_t off = wave_excl_scan_small((uint32_t)__builtin_popcount(mkq), total);
+            const int32_t first = __builtin_amdgcn_readfirstlane(vgq - off);  // lane 0: every lane runs here
+            const bool contig = total > 0 && total <= kPbStage && __all(!ok || vgq - off == first);
             if (contig) {
-                if (ok) pb_emit<W>(sc, u[q], wa[q], wb[q], mk[q], stg + off);
+                if (ok) pb_emit<W>(sc, cur.u[q], waq, wbq, mkq, stg + off);
                 wave_lds_sync();
                 for (int32_t j = l; j < total; j += kWave) pb.vals[first + j] = stg[j];
                 wave_lds_sync();
             } else if (ok) {
-                pb_emit<W>(sc, u[q], wa[q], wb[q], mk[q], pb.vals + vg[q]);
+                pb_emit<W>(sc, cur.u[q], waq, wbq, mkq, pb.vals + vgq);
             }
         }
     }
@@ -530,6 +567,66 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
 // its accumulators, and a lane adds its kPbVec slots in slot order (each
 // stretch of one row summed in registers first), so the order of the adds
 // is fixed.  Padding slots add 0 to row 0.
+// The value slots of kPbGroups 8-slot groups per lane, loaded together.
+struct PbSumBatch {
+    pb_u32x4 d[kPbGroups];
+    pb_f32x4 va[kPbGroups], vb[kPbGroups];
+};
+
+__device__ __forceinline__ void pb_sum_load(const pb_u32x4* dst, const pb_f32x4* val, int32_t g0, int32_t nv, int l,
+                                            PbSumBatch& b) {
+#pragma unroll
+    for (int q = 0; q < kPbGroups; ++q) {
+        const int32_t gi = g0 + q * kWave + l;
+        const bool ok = gi < nv;
+        b.d[q] = ok ? __builtin_nontemporal_load(&dst[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
+        b.va[q] = ok ? __builtin_nontemporal_load(&val[2 * gi]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+        b.vb[q] = ok ? __builtin_nontemporal_load(&val[2 * gi + 1]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+}
+
+// Finalise rows r0 + i, i = lane, lane + 64, ... < nr, from their FP64 sums
+// acc[i]; each lane loads the inputs of 8 rows before it writes any of them,
+// so the loads of a batch are in flight together.  Same arithmetic and the
+// same per-lane order as finalize_row.
+template <typename T, bool W>
+__device__ __forceinline__ void finalize_rows_wave(const IterArgs<T>& a, int32_t r0, int32_t nr, const double* acc,
+                                                   double jump, double& err, double& sum, double& dang) {
+    const int l = lane_id();
+    if (a.eig) {
+        for (int32_t i = l; i < nr; i += kWave) finalize_row<T, W>(a, r0 + i, acc[i], jump, err, sum, dang);
+        return;
+    }
+    constexpr int kRowBatch = 8;
+    for (int32_t i0 = 0; i0 < nr; i0 += kRowBatch * kWave) {
+        T xo[kRowBatch];
+        float d[kRowBatch];
+        double jv[kRowBatch];
+#pragma unroll
+        for (int q = 0; q < kRowBatch; ++q) {
+            const int32_t i = i0 + q * kWave + l;
+            const int32_t v = r0 + (i < nr ? i : 0);
+            xo[q] = a.x_in[v];
+            d[q] = a.deg[v];
+            jv[q] = a.bias ? a.bias[v] : (a.tele ? jump * a.tele[v] : jump);
+        }
+#pragma unroll
+        for (int q = 0; q < kRowBatch; ++q) {
+            const int32_t i = i0 + q * kWave + l;
+            if (i < nr) {
+                const int32_t v = r0 + i;
+                const double xd = W ? (acc[i] + jv[q]) : (a.alpha * acc[i] + jv[q]);
+                const T xn = (T)xd;
+                err += fabs((double)xn - (double)xo[q]);
+                sum += (double)xn;
+                if (d[q] == 0.0f) dang += (double)xn;
+                a.x_out[v] = xn;
+                if (!W) a.s_out[v] = (d[q] > 0.0f) ? (T)(xn / (T)d[q]) : (T)0;
+            }
+        }
+    }
+}
+
 template <typename T, bool W>
 __global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
     __shared__ double acc[kPbRows];
@@ -542,24 +639,21 @@ __global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
     const pb_u32x4* dst = reinterpret_cast<const pb_u32x4*>(pb.dst + it.e0);
     const pb_f32x4* val = reinterpret_cast<const pb_f32x4*>(pb.vals + it.e0);
     const int32_t nv = (it.e1 - it.e0) / kPbVec;
+    // software pipelined: the next batch's loads are issued before this
+    // batch is added (the loop bound is wavefront-uniform)
+    PbSumBatch nxt;
+    pb_sum_load(dst, val, 0, nv, l, nxt);
     for (int32_t g0 = 0; g0 < nv; g0 += kPbGroups * kWave) {
-        pb_u32x4 d[kPbGroups];
-        pb_f32x4 va[kPbGroups], vb[kPbGroups];
-#pragma unroll
-        for (int q = 0; q < kPbGroups; ++q) {
-            const int32_t gi = g0 + q * kWave + l;
-            const bool ok = gi < nv;
-            d[q] = ok ? __builtin_nontemporal_load(&dst[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
-            va[q] = ok ? __builtin_nontemporal_load(&val[2 * gi]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-            vb[q] = ok ? __builtin_nontemporal_load(&val[2 * gi + 1]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-        }
+        const PbSumBatch cur = nxt;
+        if (g0 + kPbGroups * kWave < nv) pb_sum_load(dst, val, g0 + kPbGroups * kWave, nv, l, nxt);
 #pragma unroll
         for (int q = 0; q < kPbGroups; ++q) {
             if (g0 + q * kWave + l < nv) {
-                const uint32_t w[4] = {d[q].x, d[q].y, d[q].z, d[q].w};
+                const uint32_t w[4] = {cur.d[q].x, cur.d[q].y, cur.d[q].z, cur.d[q].w};
                 const uint32_t r[8] = {w[0] & 0xffffu, w[0] >> 16, w[1] & 0xffffu, w[1] >> 16,
                                        w[2] & 0xffffu, w[2] >> 16, w[3] & 0xffffu, w[3] >> 16};
-                const float v[8] = {va[q].x, va[q].y, va[q].z, va[q].w, vb[q].x, vb[q].y, vb[q].z, vb[q].w};
+                const float v[8] = {cur.va[q].x, cur.va[q].y, cur.va[q].z, cur.va[q].w,
+                                    cur.vb[q].x, cur.vb[q].y, cur.vb[q].z, cur.vb[q].w};
                 // slots of one row are consecutive inside a run: add each
                 // stretch in registers, one LDS add per stretch
                 double run = (double)v[0];
@@ -582,7 +676,7 @@ __global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
     double err = 0.0, sum = 0.0, dang = 0.0;
     if (it.piece < 0) {
         const double jump = jump_of(a);
-        for (int32_t i = l; i < nr; i += kWave) finalize_row<T, W>(a, r0 + i, acc[i], jump, err, sum, dang);
+        finalize_rows_wave<T, W>(a, r0, nr, acc, jump, err, sum, dang);
         err = wave_sum(err);
         sum = wave_sum(sum);
         dang = wave_sum(dang);
