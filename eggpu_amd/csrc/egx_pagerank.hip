@@ -400,15 +400,17 @@ struct PbArgs {
 
 constexpr int kPbThreads = 1024;  // one scatter workgroup per CU: its chunk takes 128 KB of LDS
 
-// One scatter item: s of its chunk into LDS (one coalesced read; entry
-// kPbChunk is the zero that padding slots read), then the item's groups of
-// kPbVec source slots, one per lane: a 16-byte index load, kPbVec LDS reads,
-// each stretch of equal rows summed in FP64 and emitted as one FP32 term;
-// kPbGroups groups in flight per lane.  The terms of a wavefront's 64 groups
-// usually form one contiguous range (groups of one run): they are then
-// compacted in a per-wave LDS staging area (offsets from a wave prefix sum)
-// and stored by consecutive lanes to consecutive slots.  Otherwise each lane
-// stores its terms from its group's first value slot on.
+// One scatter item: s of its chunk into LDS (16-byte loads; entry kPbChunk
+// is the zero that padding slots read), then the item's groups of kPbVec
+// source slots, one per lane: a 16-byte index load, kPbVec LDS reads, each
+// stretch of equal rows summed in FP64 and emitted as one FP32 term.  A lane
+// takes kPbGroups groups per batch, and works through two batches in turn:
+// one batch's loads are in flight while the other is processed.  The terms
+// of a wavefront's 64 groups usually form one contiguous range (groups of
+// one run): they are then compacted in a per-wave LDS staging area (offsets
+// from a wave prefix sum) and stored by consecutive lanes to consecutive
+// slots.  Otherwise each lane stores its terms from its group's first value
+// slot on.
 constexpr int kPbGroups = 4;
 constexpr int kPbStage = 504;  // staged terms per wavefront (64 groups emit at most 512)
 typedef uint32_t pb_u32x4 __attribute__((ext_vector_type(4)));
@@ -461,6 +463,38 @@ __device__ __forceinline__ void pb_emit(const float* sc, pb_u32x4 u, pb_f32x4 wa
     o[k] = (float)run;
 }
 
+// The same terms, written without branches for the LDS staging area: every
+// slot writes the running sum of its stretch to the stretch's entry o[k], so
+// a stretch's entry is written once per slot and, since a lane's LDS writes
+// land in program order, ends with the stretch's full sum (the value pb_emit
+// writes).
+template <bool W>
+__device__ __forceinline__ void pb_emit_stage(const float* sc, pb_u32x4 u, pb_f32x4 wa, pb_f32x4 wb, uint32_t mk,
+                                              float* o) {
+    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
+    const float wt[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
+    float v[8];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        v[2 * h] = sc[w[h] & 0xffffu];
+        v[2 * h + 1] = sc[w[h] >> 16];
+    }
+    if (W) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = wt[j] * v[j];
+    }
+    double run = (double)v[0];
+    o[0] = (float)run;
+    int32_t k = 0;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+        const uint32_t st = (mk >> j) & 1u;
+        k += (int32_t)st;
+        run = st ? (double)v[j] : run + (double)v[j];
+        o[k] = (float)run;
+    }
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -480,6 +514,12 @@ struct PbScatterBatch {
     uint32_t mk[kG];
 };
 
+// Every lane loads from an index inside the item (lanes past its end load
+// its last group; the batch is told which lanes are past the end), so the
+// loads are unconditional and nothing reads their results until the batch
+// is processed: with loads under per-lane branches, or a select right after
+// them, the compiler waits for every load in flight, the prefetched batch
+// included.
 template <bool W>
 __device__ __forceinline__ void pb_scatter_load(const PbArgs& pb, int32_t g0, int32_t g1, int l,
                                                 PbScatterBatch<W>& b) {
@@ -488,14 +528,43 @@ __device__ __forceinline__ void pb_scatter_load(const PbArgs& pb, int32_t g0, in
 #pragma unroll
     for (int q = 0; q < PbScatterBatch<W>::kG; ++q) {
         const int32_t gi = g0 + q * kPbThreads + l;
-        const bool ok = gi < g1;
-        b.u[q] = ok ? __builtin_nontemporal_load(&src[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
+        const int32_t gc = gi < g1 ? gi : g1 - 1;
+        b.u[q] = __builtin_nontemporal_load(&src[gc]);
+        b.vg[q] = __builtin_nontemporal_load(&pb.vgrp[gc]);
+        b.mk[q] = (uint32_t)__builtin_nontemporal_load(&pb.mask[gc]);
         if (W) {
-            b.wa[q] = ok ? __builtin_nontemporal_load(&wts[2 * gi]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-            b.wb[q] = ok ? __builtin_nontemporal_load(&wts[2 * gi + 1]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+            b.wa[q] = __builtin_nontemporal_load(&wts[2 * gc]);
+            b.wb[q] = __builtin_nontemporal_load(&wts[2 * gc + 1]);
         }
-        b.vg[q] = ok ? __builtin_nontemporal_load(&pb.vgrp[gi]) : -1;
-        b.mk[q] = ok ? (uint32_t)__builtin_nontemporal_load(&pb.mask[gi]) : 0u;
+    }
+}
+
+// The stretch terms of one batch: for each group, a wavefront prefix sum of
+// the lanes' term counts; when the wavefront's terms form one contiguous
+// range they are compacted in the wave's staging area and stored by
+// consecutive lanes, otherwise each lane stores its own.
+template <bool W>
+__device__ __forceinline__ void pb_scatter_batch(const PbScatterBatch<W>& cur, int32_t g0, int32_t g1,
+                                                 const float* sc, float* stg, float* vals, int l) {
+#pragma unroll
+    for (int q = 0; q < PbScatterBatch<W>::kG; ++q) {
+        const bool ok = g0 + q * kPbThreads + l < g1;
+        const int32_t vgq = cur.vg[q];
+        const uint32_t mkq = ok ? cur.mk[q] : 0u;
+        const pb_f32x4 waq = W ? cur.wa[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+        const pb_f32x4 wbq = W ? cur.wb[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+        int32_t total = 0;
+        const int32_t off = wave_excl_scan_small((uint32_t)__builtin_popcount(mkq), total);
+        const int32_t first = __builtin_amdgcn_readfirstlane(vgq - off);  // lane 0: every lane runs here
+        const bool contig = total > 0 && total <= kPbStage && __all(!ok || vgq - off == first);
+        if (contig) {
+            if (ok) pb_emit_stage<W>(sc, cur.u[q], waq, wbq, mkq, stg + off);
+            wave_lds_sync();
+            for (int32_t j = l; j < total; j += kWave) vals[first + j] = stg[j];
+            wave_lds_sync();
+        } else if (ok) {
+            pb_emit<W>(sc, cur.u[q], waq, wbq, mkq, vals + vgq);
+        }
     }
 }
 
@@ -510,7 +579,7 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
     // every lane of a wavefront runs the loop the same number of times (the
     // bound depends on the wavefront's first thread only), so the shuffles
     // below see all 64 lanes
-    const int32_t wave0 = it.g0 + (int32_t)(threadIdx.x - l);
+    const int32_t wave0 = __builtin_amdgcn_readfirstlane(it.g0 + (int32_t)(threadIdx.x - l));
     constexpr int32_t kStep = PbScatterBatch<W>::kG * kPbThreads;
     // the first batch of slots is in flight while the chunk is staged
     PbScatterBatch<W> nxt;
@@ -532,122 +601,146 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
     if (threadIdx.x == 0) sc[kPbChunk] = 0.0f;
     __syncthreads();
     float* stg = stage[wave_id()];
-    for (int32_t g0 = wave0; g0 < it.g1; g0 += kStep) {
-        // software pipelined: the next batch's loads are issued before this
-        // batch's LDS reads and term stores
-        const PbScatterBatch<W> cur = nxt;
-        if (g0 + kStep < it.g1) pb_scatter_load<W>(pb, g0 + kStep, it.g1, l, nxt);
-#pragma unroll
-        for (int q = 0; q < PbScatterBatch<W>::kG; ++q) {
-            const int32_t vgq = cur.vg[q];
-            const uint32_t mkq = cur.mk[q];
-            const pb_f32x4 waq = W ? cur.wa[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-            const pb_f32x4 wbq = W ? cur.wb[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-            const bool ok = vgq >= 0;
-            int32_t total = 0;
-            const int32_t off = wave_excl_scan_small((uint32_t)__builtin_popcount(mkq), total);
-            const int32_t first = __builtin_amdgcn_readfirstlane(vgq - off);  // lane 0: every lane runs here
-            const bool contig = total > 0 && total <= kPbStage && __all(!ok || vgq - off == first);
-            if (contig) {
-                if (ok) pb_emit<W>(sc, cur.u[q], waq, wbq, mkq, stg + off);
-                wave_lds_sync();
-                for (int32_t j = l; j < total; j += kWave) pb.vals[first + j] = stg[j];
-                wave_lds_sync();
-            } else if (ok) {
-                pb_emit<W>(sc, cur.u[q], waq, wbq, mkq, pb.vals + vgq);
-            }
-        }
+    // software pipelined over two batches in turn: the next batch's loads
+    // are issued before this batch's LDS reads and term stores, and neither
+    // batch is copied
+    PbScatterBatch<W> nx2;
+    for (int32_t g0 = wave0; g0 < it.g1; g0 += 2 * kStep) {
+        if (g0 + kStep < it.g1) pb_scatter_load<W>(pb, g0 + kStep, it.g1, l, nx2);
+        pb_scatter_batch<W>(nxt, g0, it.g1, sc, stg, pb.vals, l);
+        if (g0 + kStep >= it.g1) break;
+        if (g0 + 2 * kStep < it.g1) pb_scatter_load<W>(pb, g0 + 2 * kStep, it.g1, l, nxt);
+        pb_scatter_batch<W>(nx2, g0 + kStep, it.g1, sc, stg, pb.vals, l);
     }
 }
 
-// One sum item (one wavefront): its value slots added into FP64 LDS row
-// accumulators in slot order (the pull kernel also sums rows in FP64), then
-// the sub-bin's rows finalised (or, for a piece of a split sub-bin, its row
-// sums left in hub_part for the reduce launch).  Only this wavefront touches
-// its accumulators, and a lane adds its kPbVec slots in slot order (each
-// stretch of one row summed in registers first), so the order of the adds
-// is fixed.  Padding slots add 0 to row 0.
-// The value slots of kPbGroups 8-slot groups per lane, loaded together.
+// One sum item (one workgroup of kPbSumWaves wavefronts): its value slots
+// added into the sub-bin's LDS row accumulators, then the sub-bin's rows
+// finalised (or, for a piece of a split sub-bin, its row sums left in
+// hub_part for the reduce launch).  The accumulators are 64-bit unsigned
+// fixed point in units of 2^-62: every FP32 term is converted exactly from
+// its bits (truncated below 2^-62), and integer addition is associative, so
+// the wavefronts (each taking a quarter of the item's slots) add in any
+// order and the result is the same bits every run.  A lane adds the terms
+// of each stretch of one row inside its 8-slot group in registers, then
+// adds that with one LDS atomic.  Terms are non-negative and a row's sum is
+// at most the total rank mass (about 1), far below the 4.0 the format
+// holds; the truncation is below 2.2e-19 per term.  Padding slots add 0 to
+// row 0.
+constexpr int kPbSumWaves = 4;
+constexpr int kPbSumGroups = 2;  // value groups per lane per batch (70 VGPRs: 7 wavefronts per SIMD)
+constexpr int kPbSumThreads = kPbSumWaves * kWave;
+constexpr double kPbUnfix = 1.0 / 4611686018427387904.0;  // 2^-62
+
+// t * 2^62 as an integer, truncated, for FP32 t in [0, 4): t = m * 2^(e - 150)
+// with m the significand (implicit bit included for normal numbers), so
+// t * 2^62 = m * 2^(e - 88), one 64-bit shift.  Subnormals and zero give 0.
+__device__ __forceinline__ unsigned long long pb_fix32(float t) {
+    const uint32_t b = __float_as_uint(t);
+    const int32_t e = (int32_t)(b >> 23);
+    // subnormals and zero: significand 0, so every shift gives 0
+    const unsigned long long m = (unsigned long long)((b & 0x7fffffu) | (e != 0 ? 0x800000u : 0u));
+    const int32_t sh = e - 88;
+    const unsigned long long up = m << (uint32_t)min(max(sh, 0), 40);
+    const unsigned long long dn = m >> (uint32_t)min(max(-sh, 0), 63);
+    return sh >= 0 ? up : dn;
+}
+
+// The value slots of kPbSumGroups 8-slot groups per lane, loaded together.
 struct PbSumBatch {
-    pb_u32x4 d[kPbGroups];
-    pb_f32x4 va[kPbGroups], vb[kPbGroups];
+    pb_u32x4 d[kPbSumGroups];
+    pb_f32x4 va[kPbSumGroups], vb[kPbSumGroups];
 };
 
-__device__ __forceinline__ void pb_sum_load(const pb_u32x4* dst, const pb_f32x4* val, int32_t g0, int32_t nv, int l,
-                                            PbSumBatch& b) {
+// Value groups [base + g0, ...) of the whole value array (base: the
+// wavefront's first group).  Unconditional loads, read only when the batch
+// is processed (see pb_scatter_load): lanes past the end load group 0, which
+// every layout has, and the processing skips them.
+__device__ __forceinline__ void pb_sum_load(const pb_u32x4* dst, const pb_f32x4* val, int32_t base, int32_t g0,
+                                            int32_t nv, int l, PbSumBatch& b) {
 #pragma unroll
-    for (int q = 0; q < kPbGroups; ++q) {
+    for (int q = 0; q < kPbSumGroups; ++q) {
         const int32_t gi = g0 + q * kWave + l;
         const bool ok = gi < nv;
-        b.d[q] = ok ? __builtin_nontemporal_load(&dst[gi]) : pb_u32x4{0u, 0u, 0u, 0u};
-        b.va[q] = ok ? __builtin_nontemporal_load(&val[2 * gi]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-        b.vb[q] = ok ? __builtin_nontemporal_load(&val[2 * gi + 1]) : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+        const int32_t gc = ok ? base + gi : 0;
+        b.d[q] = __builtin_nontemporal_load(&dst[gc]);
+        b.va[q] = __builtin_nontemporal_load(&val[2 * gc]);
+        b.vb[q] = __builtin_nontemporal_load(&val[2 * gc + 1]);
     }
 }
 
-// Finalise rows r0 + i, i = lane, lane + 64, ... < nr, from their FP64 sums
-// acc[i]; each lane loads the inputs of 8 rows before it writes any of them,
-// so the loads of a batch are in flight together.  Same arithmetic and the
-// same per-lane order as finalize_row.
+// Finalise rows r0 + i, i = thread, thread + kPbSumThreads, ... < nr, from
+// their fixed-point sums acc[i]; each thread loads the inputs of all its
+// rows before it writes any of them.  The arithmetic is that of
+// finalize_row.
 template <typename T, bool W>
-__device__ __forceinline__ void finalize_rows_wave(const IterArgs<T>& a, int32_t r0, int32_t nr, const double* acc,
-                                                   double jump, double& err, double& sum, double& dang) {
-    const int l = lane_id();
+__device__ __forceinline__ void finalize_rows_fix(const IterArgs<T>& a, int32_t r0, int32_t nr,
+                                                  const unsigned long long* acc, double jump, double& err,
+                                                  double& sum, double& dang) {
+    const int t = threadIdx.x;
     if (a.eig) {
-        for (int32_t i = l; i < nr; i += kWave) finalize_row<T, W>(a, r0 + i, acc[i], jump, err, sum, dang);
+        for (int32_t i = t; i < nr; i += kPbSumThreads)
+            finalize_row<T, W>(a, r0 + i, (double)acc[i] * kPbUnfix, jump, err, sum, dang);
         return;
     }
-    constexpr int kRowBatch = 8;
-    for (int32_t i0 = 0; i0 < nr; i0 += kRowBatch * kWave) {
-        T xo[kRowBatch];
-        float d[kRowBatch];
-        double jv[kRowBatch];
+    constexpr int kRowBatch = kPbRows / kPbSumThreads;
+    T xo[kRowBatch];
+    float d[kRowBatch];
+    double jv[kRowBatch];
 #pragma unroll
-        for (int q = 0; q < kRowBatch; ++q) {
-            const int32_t i = i0 + q * kWave + l;
-            const int32_t v = r0 + (i < nr ? i : 0);
-            xo[q] = a.x_in[v];
-            d[q] = a.deg[v];
-            jv[q] = a.bias ? a.bias[v] : (a.tele ? jump * a.tele[v] : jump);
-        }
+    for (int q = 0; q < kRowBatch; ++q) {
+        const int32_t i = q * kPbSumThreads + t;
+        const int32_t v = r0 + (i < nr ? i : 0);
+        xo[q] = a.x_in[v];
+        d[q] = a.deg[v];
+        jv[q] = a.bias ? a.bias[v] : (a.tele ? jump * a.tele[v] : jump);
+    }
 #pragma unroll
-        for (int q = 0; q < kRowBatch; ++q) {
-            const int32_t i = i0 + q * kWave + l;
-            if (i < nr) {
-                const int32_t v = r0 + i;
-                const double xd = W ? (acc[i] + jv[q]) : (a.alpha * acc[i] + jv[q]);
-                const T xn = (T)xd;
-                err += fabs((double)xn - (double)xo[q]);
-                sum += (double)xn;
-                if (d[q] == 0.0f) dang += (double)xn;
-                a.x_out[v] = xn;
-                if (!W) a.s_out[v] = (d[q] > 0.0f) ? (T)(xn / (T)d[q]) : (T)0;
-            }
+    for (int q = 0; q < kRowBatch; ++q) {
+        const int32_t i = q * kPbSumThreads + t;
+        if (i < nr) {
+            const int32_t v = r0 + i;
+            const double rs = (double)acc[i] * kPbUnfix;
+            const double xd = W ? (rs + jv[q]) : (a.alpha * rs + jv[q]);
+            const T xn = (T)xd;
+            err += fabs((double)xn - (double)xo[q]);
+            sum += (double)xn;
+            if (d[q] == 0.0f) dang += (double)xn;
+            a.x_out[v] = xn;
+            if (!W) a.s_out[v] = (d[q] > 0.0f) ? (T)(xn / (T)d[q]) : (T)0;
         }
     }
 }
 
 template <typename T, bool W>
-__global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
-    __shared__ double acc[kPbRows];
+__global__ __launch_bounds__(kPbSumThreads) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
+    __shared__ unsigned long long acc[kPbRows];
+    __shared__ double sh_red[3 * kPbSumWaves];
     if (a.state->done) return;
     bind(a);
     const PbSum it = pb.sums[blockIdx.x];
-    const int l = threadIdx.x;
-    for (int32_t i = l; i < kPbRows; i += kWave) acc[i] = 0.0;
+    const int t = threadIdx.x;
+    const int l = lane_id();
+    for (int32_t i = t; i < kPbRows; i += kPbSumThreads) acc[i] = 0ull;
     __syncthreads();
-    const pb_u32x4* dst = reinterpret_cast<const pb_u32x4*>(pb.dst + it.e0);
-    const pb_f32x4* val = reinterpret_cast<const pb_f32x4*>(pb.vals + it.e0);
-    const int32_t nv = (it.e1 - it.e0) / kPbVec;
+    // wavefront w takes value groups [q0, q1) of the item
+    const int32_t ng = (it.e1 - it.e0) / kPbVec;
+    const int32_t wv = __builtin_amdgcn_readfirstlane(wave_id());
+    const int32_t q0 = (int32_t)(((int64_t)ng * wv) / kPbSumWaves);
+    const int32_t q1 = (int32_t)(((int64_t)ng * (wv + 1)) / kPbSumWaves);
+    const pb_u32x4* dst = reinterpret_cast<const pb_u32x4*>(pb.dst);
+    const pb_f32x4* val = reinterpret_cast<const pb_f32x4*>(pb.vals);
+    const int32_t base = it.e0 / kPbVec + q0;
+    const int32_t nv = q1 - q0;
     // software pipelined: the next batch's loads are issued before this
     // batch is added (the loop bound is wavefront-uniform)
     PbSumBatch nxt;
-    pb_sum_load(dst, val, 0, nv, l, nxt);
-    for (int32_t g0 = 0; g0 < nv; g0 += kPbGroups * kWave) {
+    pb_sum_load(dst, val, base, 0, nv, l, nxt);
+    for (int32_t g0 = 0; g0 < nv; g0 += kPbSumGroups * kWave) {
         const PbSumBatch cur = nxt;
-        if (g0 + kPbGroups * kWave < nv) pb_sum_load(dst, val, g0 + kPbGroups * kWave, nv, l, nxt);
+        if (g0 + kPbSumGroups * kWave < nv) pb_sum_load(dst, val, base, g0 + kPbSumGroups * kWave, nv, l, nxt);
 #pragma unroll
-        for (int q = 0; q < kPbGroups; ++q) {
+        for (int q = 0; q < kPbSumGroups; ++q) {
             if (g0 + q * kWave + l < nv) {
                 const uint32_t w[4] = {cur.d[q].x, cur.d[q].y, cur.d[q].z, cur.d[q].w};
                 const uint32_t r[8] = {w[0] & 0xffffu, w[0] >> 16, w[1] & 0xffffu, w[1] >> 16,
@@ -656,15 +749,13 @@ __global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
                                     cur.vb[q].x, cur.vb[q].y, cur.vb[q].z, cur.vb[q].w};
                 // slots of one row are consecutive inside a run: add each
                 // stretch in registers, one LDS add per stretch
-                double run = (double)v[0];
+                unsigned long long run = pb_fix32(v[0]);
 #pragma unroll
                 for (int j = 0; j < 7; ++j) {
-                    if (r[j + 1] == r[j]) {
-                        run += (double)v[j + 1];
-                    } else {
-                        atomicAdd(&acc[r[j]], run);
-                        run = (double)v[j + 1];
-                    }
+                    const unsigned long long f = pb_fix32(v[j + 1]);
+                    const bool same = r[j + 1] == r[j];
+                    if (!same) atomicAdd(&acc[r[j]], run);
+                    run = same ? run + f : f;
                 }
                 atomicAdd(&acc[r[7]], run);
             }
@@ -676,15 +767,13 @@ __global__ __launch_bounds__(kWave) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
     double err = 0.0, sum = 0.0, dang = 0.0;
     if (it.piece < 0) {
         const double jump = jump_of(a);
-        finalize_rows_wave<T, W>(a, r0, nr, acc, jump, err, sum, dang);
-        err = wave_sum(err);
-        sum = wave_sum(sum);
-        dang = wave_sum(dang);
+        finalize_rows_fix<T, W>(a, r0, nr, acc, jump, err, sum, dang);
+        block_sum3(err, sum, dang, sh_red);
     } else {
-        for (int32_t i = l; i < nr; i += kWave)
-            a.hub_part[(size_t)it.part0 + (size_t)i * it.pieces + it.piece] = acc[i];
+        for (int32_t i = t; i < nr; i += kPbSumThreads)
+            a.hub_part[(size_t)it.part0 + (size_t)i * it.pieces + it.piece] = (double)acc[i] * kPbUnfix;
     }
-    if (l == 0) {
+    if (t == 0) {
         a.partials[blockIdx.x] = err;
         a.partials[(size_t)a.grid + blockIdx.x] = sum;
         a.partials[2 * (size_t)a.grid + blockIdx.x] = dang;
@@ -1033,7 +1122,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
         if constexpr (kBlockable) {
             if (blocked) {
                 k_pb_scatter<T, W><<<(uint32_t)g.n_pb_scatter, kPbThreads, 0, s>>>(a, pb);
-                k_pb_sum<T, W><<<(uint32_t)g.n_pb_sum, kWave, 0, s>>>(a, pb);
+                k_pb_sum<T, W><<<(uint32_t)g.n_pb_sum, kPbSumThreads, 0, s>>>(a, pb);
                 return;
             }
         }
