@@ -16,7 +16,17 @@ x read and x' write per node), over the mean duration of the SpMV launch
 measured with hipEvents on the graph's stream (the scatter + sum launch pair
 when the tuner picks the propagation-blocked iteration).  ``cpu_baseline`` times the
 OpenMP push of pagerank.cpp:21-122 (oracle/egx_oracle.c) on this host, on
-the same graph, for a few iterations.
+the same graph, for the same K iterations; its ranks and the sequential
+FP64 restatement of pagerank.py:75-139 at tol=1e-6 check the GPU's ranks,
+and the max-abs and L1 errors go to ``config.parity``.
+
+    python bench.py --calib-copy BYTES ...
+        one device copy of BYTES before the workload: run under
+        rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE), the copy calibrates
+        the counters
+    python bench.py --pmc-traffic FETCH_DIR WRITE_DIR [--calib-copy BYTES]
+                    [--kernels k_pb_scatter,k_pb_sum]
+        no GPU work: HBM bytes per launch of the kernels from those runs
 """
 from __future__ import annotations
 
@@ -54,13 +64,116 @@ def read_traffic(blocked, weighted=False):
     return d.get("hbm_bytes_per_launch"), os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(g, n, m, iters, threads, weighted=False):
+def calib_copy(nbytes):
+    """One plain device-to-device copy of `nbytes` through the HIP runtime
+    (its blit kernel), so rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of
+    this command read a kernel of known traffic beside the iteration's."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(nbytes)) != 0:
+            raise SystemExit("hipMalloc failed for the calibration copy")
+    try:
+        rc = hip.hipMemset(b, 1, ctypes.c_size_t(nbytes)) or \
+            hip.hipMemcpy(a, b, ctypes.c_size_t(nbytes), 3) or hip.hipDeviceSynchronize()  # 3: device to device
+        if rc != 0:
+            raise SystemExit(f"calibration copy failed (hip error {rc})")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def _pmc_rows(path, counter):
+    """{kernel name: [(dispatch id, value)]} of one counter from a rocprofv3
+    --pmc output directory (*_counter_collection.csv files)."""
+    import csv
+    import glob
+
+    files = [path] if os.path.isfile(path) else glob.glob(os.path.join(path, "**", "*counter_collection.csv"),
+                                                          recursive=True)
+    acc = {}
+    for f in files:
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") != counter:
+                    continue
+                key = (r["Kernel_Name"], int(r["Dispatch_Id"]))
+                acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
+    out = {}
+    for (name, disp), v in sorted(acc.items(), key=lambda kv: kv[0][1]):
+        out.setdefault(name, []).append((disp, v))
+    return out
+
+
+def pmc_traffic(fetch_dir, write_dir, copy_bytes, kernels):
+    """HBM bytes per launch of the named kernels (summed: the launches of one
+    iteration) from a FETCH_SIZE run and a WRITE_SIZE run of this script
+    with --calib-copy, each counter scaled by what it read for the copy.
+    The copy is the run's first launch of the runtime's copy kernel (the
+    library's own copies come later).  Launches that found the stop flag
+    already set return at once; only launches with at least half of the
+    kernel's largest reading are averaged."""
+    fetch = _pmc_rows(fetch_dir, "FETCH_SIZE")
+    write = _pmc_rows(write_dir, "WRITE_SIZE")
+
+    def first(d, part):
+        for k, v in d.items():
+            if part in k and v:
+                return v[0][1]
+        return None
+
+    def mean_full(d, part):
+        for k, v in d.items():
+            if part in k and v:
+                top = max(x for _, x in v)
+                full = [x for _, x in v if x >= 0.5 * top]
+                return sum(full) / len(full), len(full), k
+        return None, 0, None
+
+    cf, cw = first(fetch, "copyBuffer"), first(write, "copyBuffer")
+    if None in (cf, cw):
+        raise SystemExit(f"no calibration copy in the counter files (fetch {cf}, write {cw})")
+    f_scale, w_scale = copy_bytes / cf, copy_bytes / cw
+    per = []
+    for part in kernels:
+        kf, nf, kname = mean_full(fetch, part)
+        kw, nw, _ = mean_full(write, part)
+        if None in (kf, kw):
+            raise SystemExit(f"no counters for {part}")
+        per.append({"kernel": kname, "dispatches": {"fetch": nf, "write": nw}, "FETCH_SIZE_mean": kf,
+                    "WRITE_SIZE_mean": kw, "hbm_read_bytes": kf * f_scale, "hbm_write_bytes": kw * w_scale})
+    rd = sum(p["hbm_read_bytes"] for p in per)
+    wr = sum(p["hbm_write_bytes"] for p in per)
+    return {"kernel": " + ".join(p["kernel"] for p in per), "copy_bytes": copy_bytes, "copy_FETCH_SIZE": cf,
+            "copy_WRITE_SIZE": cw, "fetch_scale_bytes_per_unit": f_scale, "write_scale_bytes_per_unit": w_scale,
+            "hbm_read_bytes_per_launch": rd, "hbm_write_bytes_per_launch": wr, "hbm_bytes_per_launch": rd + wr,
+            "per_kernel": per}
+
+
+def cpu_baseline(g, n, m, iters, threads, gpu_k, gpu_tol, weighted=False):
+    """The OpenMP push of pagerank.cpp:21-122 timed over `iters` fixed
+    iterations on this host; its result is also the check of the GPU's
+    K-iteration ranks (gpu_k), and the sequential FP64 restatement of
+    pagerank.py:75-139 at tol=1e-6 checks the GPU's tol=1e-6 solve
+    (gpu_tol = (ranks, iterations)).  Errors go to `parity`."""
     from oracle import oracle as O
 
     V, E, W = g.download_csr(with_weights=weighted)
+    Wc = W if weighted else None
     t0 = time.perf_counter()
-    _, it = O.pagerank_omp(V, E, W if weighted else None, 0.85, iters, 0.0, threads=threads)
+    ref_k, it = O.pagerank_omp(V, E, Wc, 0.85, iters, 0.0, threads=threads)
     dt = time.perf_counter() - t0
+    d = np.abs(gpu_k - ref_k)
+    parity = {"fixed_K": {"K": iters, "oracle": "orc_pagerank_omp (FP64)", "max_abs": float(d.max()),
+                          "l1": float(d.sum())}}
+    t1 = time.perf_counter()
+    ref_t, it_t, _ = O.pagerank_csr(V, E, Wc, 0.85, 100, 1e-6)
+    d = np.abs(gpu_tol[0] - ref_t)
+    parity["tol1e-6"] = {"oracle": "orc_pagerank_push (FP64, pagerank.py:75-139 order)", "iterations": gpu_tol[1],
+                         "oracle_iterations": it_t, "max_abs": float(d.max()), "l1": float(d.sum()),
+                         "oracle_seconds": round(time.perf_counter() - t1, 2)}
     return {
         "value": round(m * it / dt / 1e9, 4),
         "unit": "GTEPS",
@@ -69,6 +182,7 @@ def cpu_baseline(g, n, m, iters, threads, weighted=False):
         "sample": f"same R-MAT scale graph (n={n}, m={m}), {it} iterations at fixed K, OpenMP push with "
                   f"atomic scatter (pagerank.cpp:21-122 semantics), {dt:.2f} s",
         "host": {"nproc": os.cpu_count(), "cpu_model": _cpu_model()},
+        "parity": parity,
     }
 
 
@@ -111,7 +225,6 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=24)
-    ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default bins")
     ap.add_argument("--weighted", action="store_true",
@@ -119,7 +232,18 @@ def main():
     ap.add_argument("--bins", help='fixed bins instead of the tuner, JSON of egx_graph_set_bins fields')
     ap.add_argument("--sections", default="batched_ppr,multi_graph,serving,katz,eigenvector,ingest",
                     help="secondary workloads reported beside the headline line ('' for none)")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
+                    help="run one device copy of BYTES first (counter calibration for --pmc-traffic)")
+    ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="no GPU work: summarise two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of this "
+                         "script with --calib-copy into HBM bytes per launch of --kernels")
+    ap.add_argument("--kernels", default="k_pb_scatter,k_pb_sum")
     args = ap.parse_args()
+    if args.pmc_traffic:
+        out = pmc_traffic(args.pmc_traffic[0], args.pmc_traffic[1], args.calib_copy or (1 << 30),
+                          [k for k in args.kernels.split(",") if k])
+        print(json.dumps(out, indent=1), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}), flush=True)
@@ -132,6 +256,8 @@ def main():
     if native.device_count() < 1:
         raise SystemExit("no HIP device visible")
 
+    if args.calib_copy > 0:
+        calib_copy(args.calib_copy)
     log(f"building R-MAT scale {args.scale} on the device")
     t0 = time.perf_counter()
     g = native.DeviceGraph.rmat(args.scale, 16, a=0.57, b=0.19, c=0.19, seed=1, keep_csr=not args.no_cpu,
@@ -179,15 +305,18 @@ def main():
     blocked = bool(info["blocked"])
     traffic, traffic_src = read_traffic(blocked, args.weighted) if args.scale == 24 else (None, None)
 
-    # tol = 1e-6 solve (reference semantics) and the FP64 variant, beside
+    # tol = 1e-6 solve (reference semantics), the K-iteration ranks and the
+    # FP64 variant, beside
     pr, it_tol, _ = g.pagerank(0.85, 100, 1e-6)
+    pr_k, _, _ = g.pagerank(0.85, K, 0.0)
     _, st64 = g.pagerank_stats(0.85, K, 0.0, precision=native.PRECISION_FP64, time_each_launch=False)
 
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {args.cpu_iters} iterations, {threads} threads")
-        cpu = cpu_baseline(g, n, m, args.cpu_iters, threads, weighted=args.weighted)
+        log(f"cpu baseline: {K} iterations, {threads} threads; oracle checks of the GPU ranks")
+        cpu = cpu_baseline(g, n, m, K, threads, pr_k, (pr, it_tol), weighted=args.weighted)
+        log(f"parity: {json.dumps(cpu['parity'])}")
 
     # the CPU-runnable configuration (R-MAT 14) on the GPU, with its CPU legs
     from eggpu_amd import benchmarks
@@ -245,6 +374,7 @@ def main():
             "tol1e-6_iterations": it_tol,
             "fp64_ms_per_iter": round(st64["kernel_seconds"] / K * 1e3, 4),
             "fp64_gteps": round(m * K / st64["kernel_seconds"] / 1e9, 3),
+            "parity": cpu["parity"] if cpu else None,
         },
         "roofline": {
             "bound": "hbm",

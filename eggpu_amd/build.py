@@ -29,7 +29,6 @@ SOURCES = [
     "egx_api.hip",
     "egx_serve.hip",
     "egx_ingest.hip",
-    "egx_probe.hip",
 ]
 
 # -packed-fp32-ops: no packed FP32 instructions are generated.

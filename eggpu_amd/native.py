@@ -241,8 +241,6 @@ SIGNATURES = {
                                     _i32p, _f64p]),
     "egx_eigenvector_csr": (ctypes.c_int, [_i32p, _i32p, _f64p, ctypes.c_int32, ctypes.c_int32, _f64p,
                                            ctypes.POINTER(EigOpts), _f64p, _i32p, _i32p, _f64p]),
-    "egx_probe_copy": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, _f64p]),
-    "egx_probe_gather": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _f64p]),
 }
 
 _lib = None

@@ -58,14 +58,3 @@ def serving_pool_specs(count=32, seed=77, m_lo=1e4, m_hi=5e7, edge_factor=16):
         specs.append(GraphSpec("rmat", n, m, seed=seed * 1000003 + i, weighted=(i % 2 == 1),
                                wseed=seed * 7 + i))
     return specs
-
-
-def spec_csr(spec, oracle):
-    """(V, E, W or None) of a spec from the CPU oracle module."""
-    if spec.kind == "er":
-        V, E = oracle.er_csr(spec.n, spec.nedges, spec.seed)
-    else:
-        a, b, c = RMAT_ABC
-        V, E = oracle.rmat_n_csr(spec.n, spec.nedges, spec.seed, a, b, c)
-    W = oracle.edge_weights(V, E, spec.wseed) if spec.weighted else None
-    return V, E, W

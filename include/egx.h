@@ -303,17 +303,6 @@ int egx_graph_ppr(egx_graph *g, int32_t k, const int32_t *seed_offsets, const in
                   int32_t *iterations, double *kernel_seconds);
 
 /* ------------------------------------------------------------------------ */
-/* Measurement probes (no counterpart in the reference): calibration of the   */
-/* roofline figures on the device the library runs on.                       */
-/* ------------------------------------------------------------------------ */
-/* Plain streaming device copy of `bytes`; mean seconds per copy over reps. */
-int egx_probe_copy(int64_t bytes, int32_t reps, double *seconds_per_copy);
-
-/* n_gathers random 4-byte loads from a working set of working_set_floats
- * floats; mean seconds per pass over reps. */
-int egx_probe_gather(int64_t working_set_floats, int64_t n_gathers, int32_t reps, double *seconds_per_pass);
-
-/* ------------------------------------------------------------------------ */
 /* Katz centrality on the same SpMV engine.  No counterpart on the reference's */
 /* GPU path; it restates the CPU function katz_centrality.py:7-110 (the     */
 /* Python route of its @hybrid dispatch): A[u][v] = 1 for every arc u->v of */

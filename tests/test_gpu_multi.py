@@ -19,6 +19,17 @@ def _say(msg):
     print(f"[gpu-multi] {msg}", flush=True)
 
 
+def _spec_csr(spec):
+    """(V, E, W or None) of a graph spec, from the CPU oracle's generators."""
+    if spec.kind == "er":
+        V, E = O.er_csr(spec.n, spec.nedges, spec.seed)
+    else:
+        a, b, c = workloads.RMAT_ABC
+        V, E = O.rmat_n_csr(spec.n, spec.nedges, spec.seed, a, b, c)
+    W = O.edge_weights(V, E, spec.wseed) if spec.weighted else None
+    return V, E, W
+
+
 def _small_set(count, seed, weighted=False):
     specs = workloads.hetero_specs(count, seed=seed, n_lo=300, n_hi=60000)
     if weighted:
@@ -66,7 +77,7 @@ def test_multi_vs_oracle(gpu_ready):
     graphs = [s.build(native) for s in specs]
     outs, iters, _ = native.multi_pagerank(graphs, 0.9, 100, 1e-6)
     for i, s in enumerate(specs):
-        V, E, W = workloads.spec_csr(s, O)
+        V, E, W = _spec_csr(s)
         ref, it_ref, _ = O.pagerank_csr(V, E, W, 0.9, 100, 1e-6)
         err = np.abs(outs[i] - ref)
         _say(f"{s.kind} n={s.n} m={len(E)}: max-abs {err.max():.3e} iterations {iters[i]} / {it_ref}")
