@@ -14,6 +14,10 @@ int DevBuf::alloc(size_t nbytes) {
     if (e != hipSuccess) {
         p = nullptr;
         set_last_error(std::string("hipMalloc(") + std::to_string(nbytes) + "): " + hipGetErrorString(e));
+        // the failure is reported here; clear HIP's sticky last error so the
+        // next launch check does not report it again (a caller that falls
+        // back, such as the blocked-layout build, then goes on cleanly)
+        (void)hipGetLastError();
         return EGX_FAILED_TO_ALLOCATE_DEVICE_MEM;
     }
     bytes = nbytes;
