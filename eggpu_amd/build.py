@@ -29,6 +29,7 @@ SOURCES = [
     "egx_api.hip",
     "egx_serve.hip",
     "egx_ingest.hip",
+    "egx_noalloc.hip",
 ]
 
 # -packed-fp32-ops: no packed FP32 instructions are generated.

@@ -44,6 +44,9 @@ int new_graph(std::unique_ptr<egx_graph>& out) {
     EGX_HIP_TRY(hipStreamCreate(&g.stream), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipEventCreate(&g.ev0), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipEventCreate(&g.ev1), EGX_DEVICE_ERR);
+    // the pinned slot a solve stages its state through, made here so that no
+    // solve allocates it
+    EGX_HIP_TRY(hipHostMalloc(&g.host_state, sizeof(PrState)), EGX_FAILED_TO_ALLOCATE_HOST_MEM);
     out = std::move(h);
     return EGX_SUCC;
 }

@@ -249,7 +249,31 @@ struct SolveParams {
     double eig_floor = 0.0;
     double eig_shift = 0.0;
     const double* eig_x0 = nullptr;
+    // per-solve workspace from caller memory instead of the graph's own
+    // buffers (allocated on first use); with it the normalised result goes
+    // to out_device (n doubles, caller order, device memory) and nothing is
+    // copied to the host
+    const struct SolveBufs* bufs = nullptr;
+    double* out_device = nullptr;
 };
+
+// The per-solve workspace of one PageRank solve: two iterate buffers and two
+// contribution buffers of n entries of the iterate type, the device state,
+// per-block partial sums (3 per SpMV work item) and the reduce launch's
+// partial sums (3 per reduce workgroup, at most solve_reduce_parts()).
+struct SolveBufs {
+    void* x[2];
+    void* s[2];
+    PrState* state;
+    double* partials;
+    double* red_part;
+};
+// Bytes a solve of graph g in this precision needs for its SolveBufs when
+// they are carved from one region with carve_solve_bufs (256-byte aligned).
+int64_t solve_scratch_bytes(const Graph& g, int32_t precision);
+// Carves the SolveBufs of graph g out of base (at least
+// solve_scratch_bytes(g, precision) bytes, 256-byte aligned).
+void carve_solve_bufs(const Graph& g, int32_t precision, char* base, SolveBufs& b);
 struct SolveStats {
     double kernel_seconds = 0.0;  // first to last iteration launch
     double spmv_seconds = 0.0;    // sum of k_pr_spmv launch durations

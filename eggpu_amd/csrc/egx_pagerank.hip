@@ -870,6 +870,11 @@ __global__ void k_pr_output_multi(const IterArgs<T>* __restrict__ table, const u
     a.out[v] = (norm > 0.0) ? (double)x[r] * (1.0 / norm) : 1.0 / (double)a.n;
 }
 
+__global__ void k_fill_f64(double* __restrict__ out, int32_t n, double v) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+
 // x0 = init (1/n for PageRank, 1 for Katz; as T), s0 = x0/deg.
 template <typename T>
 __global__ void k_pr_init(const float* __restrict__ deg, int32_t n, double init, T* __restrict__ x,
@@ -991,7 +996,7 @@ int fill_args(Graph& g, const SolveParams& p, IterArgs<T>& a, bool* refreshed = 
     const double before = g.alpha_vals;
     EGX_TRY(refresh_alpha(g, p.alpha));
     if (refreshed) *refreshed = g.weighted && before != g.alpha_vals;
-    EGX_TRY(ensure_workspace(g, grid));
+    if (!p.bufs) EGX_TRY(ensure_workspace(g, grid));
     const int32_t n = g.n;
     a = IterArgs<T>{};
     a.row = g.it_row;
@@ -1007,13 +1012,23 @@ int fill_args(Graph& g, const SolveParams& p, IterArgs<T>& a, bool* refreshed = 
     a.hsegs = g.hsegs.as<HubSeg>();
     a.hrows = g.hrows.as<HubRow>();
     a.hub_part = g.hub_part.as<double>();
-    a.xb[0] = g.x[0].as<T>();
-    a.xb[1] = g.x[1].as<T>();
-    a.sb[0] = g.s[0].as<T>();
-    a.sb[1] = g.s[1].as<T>();
-    a.state = g.state.as<PrState>();
-    a.partials = g.partials.as<double>();
-    a.red_part = g.red_part.as<double>();
+    if (p.bufs) {
+        a.xb[0] = static_cast<T*>(p.bufs->x[0]);
+        a.xb[1] = static_cast<T*>(p.bufs->x[1]);
+        a.sb[0] = static_cast<T*>(p.bufs->s[0]);
+        a.sb[1] = static_cast<T*>(p.bufs->s[1]);
+        a.state = p.bufs->state;
+        a.partials = p.bufs->partials;
+        a.red_part = p.bufs->red_part;
+    } else {
+        a.xb[0] = g.x[0].as<T>();
+        a.xb[1] = g.x[1].as<T>();
+        a.sb[0] = g.s[0].as<T>();
+        a.sb[1] = g.s[1].as<T>();
+        a.state = g.state.as<PrState>();
+        a.partials = g.partials.as<double>();
+        a.red_part = g.red_part.as<double>();
+    }
     a.n_hrows = g.n_hrows;
     a.n = n;
     a.thread_row_max = g.bins.thread_row_max;
@@ -1027,7 +1042,7 @@ int fill_args(Graph& g, const SolveParams& p, IterArgs<T>& a, bool* refreshed = 
     a.eig_floor = p.eig_floor;
     a.eig_shift = p.eig_shift;
     a.perm = g.reordered ? g.perm.as<int32_t>() : nullptr;
-    a.out = g.out.as<double>();
+    a.out = p.out_device ? p.out_device : g.out.as<double>();
     a.n_dangling = g.n_dangling;
     a.alpha = p.alpha;
     // Katz and eigenvector: absolute test
@@ -1043,46 +1058,42 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
     const int32_t n = g.n;
     if (p.eig_delta) {
         k_eig_init<T><<<nblk(n, 256), 256, 0, s>>>(p.eig_x0, g.reordered ? g.perm.as<int32_t>() : nullptr, n,
-                                                   g.x[0].as<T>(), g.s[0].as<T>());
+                                                   a.xb[0], a.sb[0]);
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
         PrState* st0 = static_cast<PrState*>(g.host_state);
         *st0 = PrState{};
         st0->scale = 1.0;
         st0->scale_prev = 1.0;
-        EGX_HIP_TRY(hipMemcpyAsync(g.state.p, st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipMemcpyAsync(a.state, st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
         return EGX_SUCC;
     }
-    k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, p.bias ? 1.0 : 1.0 / (double)n, g.x[0].as<T>(),
-                                              g.s[0].as<T>());
+    k_pr_init<T><<<nblk(n, 256), 256, 0, s>>>(g.it_deg, n, p.bias ? 1.0 : 1.0 / (double)n, a.xb[0], a.sb[0]);
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     // staged through the pinned state buffer: the copy may run after return
     PrState* st0 = static_cast<PrState*>(g.host_state);
     *st0 = PrState{};
     st0->dang = (double)g.n_dangling * (double)(T)(1.0 / (double)n);
     st0->sum = 1.0;
-    EGX_HIP_TRY(hipMemcpyAsync(g.state.p, st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(a.state, st0, sizeof(PrState), hipMemcpyHostToDevice, s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
 
-// Writes the result of graph g (and copies it to host_out): PageRank's
-// normalised iterate, or Katz's unnormalised previous iterate.
+// Writes the result of graph g to a.out (and copies it to host_out):
+// PageRank's normalised iterate, Katz's unnormalised previous iterate, or
+// the eigenvector iterate.
 template <typename T>
-int emit_result(Graph& g, hipStream_t s, double* host_out, bool katz = false, bool eig = false) {
+int emit_result(const Graph& g, const IterArgs<T>& a, hipStream_t s, double* host_out, bool katz = false,
+                bool eig = false) {
+    const int32_t* perm = g.reordered ? g.perm.as<int32_t>() : nullptr;
     if (eig)
-        k_eig_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
-                                                        g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
-                                                        g.out.as<double>());
+        k_eig_output<T><<<nblk(g.n, 256), 256, 0, s>>>(a.xb[0], a.xb[1], a.state, perm, g.n, a.out);
     else if (katz)
-        k_katz_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
-                                                         g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
-                                                         g.out.as<double>());
+        k_katz_output<T><<<nblk(g.n, 256), 256, 0, s>>>(a.xb[0], a.xb[1], a.state, perm, g.n, a.out);
     else
-        k_pr_output<T><<<nblk(g.n, 256), 256, 0, s>>>(g.x[0].as<T>(), g.x[1].as<T>(), g.state.as<PrState>(),
-                                                       g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
-                                                       g.out.as<double>());
+        k_pr_output<T><<<nblk(g.n, 256), 256, 0, s>>>(a.xb[0], a.xb[1], a.state, perm, g.n, a.out);
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     if (host_out)
-        EGX_HIP_TRY(hipMemcpyAsync(host_out, g.out.p, (size_t)g.n * 8, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipMemcpyAsync(host_out, a.out, (size_t)g.n * 8, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
 
@@ -1098,8 +1109,10 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
     const bool blocked = kBlockable && g.pb_ready && (!W || g.pb_w.p != nullptr);
     PbArgs pb{};
     if (blocked) {
-        EGX_TRY(ensure_workspace(g, (uint32_t)g.n_pb_sum));
-        a.partials = g.partials.as<double>();
+        if (!p.bufs) {
+            EGX_TRY(ensure_workspace(g, (uint32_t)g.n_pb_sum));
+            a.partials = g.partials.as<double>();
+        }
         a.hrows = g.pb_hrows.as<HubRow>();
         a.hub_part = g.pb_hub_part.as<double>();
         a.n_hrows = g.n_pb_hrows;
@@ -1173,13 +1186,13 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
             }
         }
         if (launched >= max_launch) break;
-        EGX_HIP_TRY(hipMemcpyAsync(hs, g.state.p, sizeof(PrState), hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipMemcpyAsync(hs, a.state, sizeof(PrState), hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
         EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
         if (hs->done) break;
         if (p.check_every <= 0) chunk = std::min(chunk * 2, 64);
     }
     EGX_HIP_TRY(hipEventRecord(g.ev1, s), EGX_DEVICE_ERR);
-    EGX_HIP_TRY(hipMemcpyAsync(hs, g.state.p, sizeof(PrState), hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(hipMemcpyAsync(hs, a.state, sizeof(PrState), hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipEventSynchronize(g.ev1), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     const int32_t iters = hs->iter;
@@ -1201,7 +1214,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
             st->spmv_launches = iters;
         }
     }
-    EGX_TRY(emit_result<T>(g, s, host_out, p.bias != nullptr, p.eig_delta != 0));
+    EGX_TRY(emit_result<T>(g, a, s, host_out, p.bias != nullptr, p.eig_delta != 0));
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
@@ -1416,7 +1429,39 @@ __global__ void k_tele_seeds(const int32_t* __restrict__ seeds, int32_t k, const
     tele[rank ? rank[v] : v] = w;
 }
 
+constexpr int64_t kSolveAlign = 256;
+int64_t align_up(int64_t v) { return (v + kSolveAlign - 1) / kSolveAlign * kSolveAlign; }
+// SpMV work items of g's largest launch (partial slots needed)
+int64_t solve_grid(const Graph& g) {
+    return std::max<int64_t>(std::max<int64_t>(g.n_hseg + g.n_tiles, g.n_pb_sum), 1);
+}
+
 }  // namespace
+
+int64_t solve_scratch_bytes(const Graph& g, int32_t precision) {
+    const int64_t elem = precision == EGX_PRECISION_FP64 ? 8 : 4;
+    const int64_t vec = align_up((int64_t)std::max(g.n, 1) * elem);
+    return 4 * vec + align_up((int64_t)sizeof(PrState)) + align_up(solve_grid(g) * 3 * 8) +
+           align_up((int64_t)kMaxRed * 3 * 8);
+}
+
+void carve_solve_bufs(const Graph& g, int32_t precision, char* base, SolveBufs& b) {
+    const int64_t elem = precision == EGX_PRECISION_FP64 ? 8 : 4;
+    const int64_t vec = align_up((int64_t)std::max(g.n, 1) * elem);
+    char* p = base;
+    for (int i = 0; i < 2; ++i) {
+        b.x[i] = p;
+        p += vec;
+        b.s[i] = p;
+        p += vec;
+    }
+    b.state = reinterpret_cast<PrState*>(p);
+    p += align_up((int64_t)sizeof(PrState));
+    b.partials = reinterpret_cast<double*>(p);
+    p += align_up(solve_grid(g) * 3 * 8);
+    b.red_part = reinterpret_cast<double*>(p);
+}
+
 
 int ensure_tele(Graph& g, const std::vector<int32_t>& seeds) {
     hipStream_t s = g.stream;
@@ -1445,6 +1490,11 @@ int solve_pagerank(Graph& g, const SolveParams& p_in, double* host_out, SolveSta
         // no stored arcs: uniform ranks, as pagerank.py:93-95 and pagerank.cu:535-539
         if (host_out)
             for (int32_t i = 0; i < g.n; ++i) host_out[i] = 1.0 / (double)g.n;
+        if (p.out_device) {
+            k_fill_f64<<<nblk(g.n, 256), 256, 0, g.stream>>>(p.out_device, g.n, 1.0 / (double)g.n);
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+            EGX_HIP_TRY(hipStreamSynchronize(g.stream), EGX_DEVICE_ERR);
+        }
         return EGX_SUCC;
     }
     // adjacency graphs have out-weight 1 everywhere, so the unweighted path

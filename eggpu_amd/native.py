@@ -212,6 +212,7 @@ SIGNATURES = {
     "egx_graph_pagerank": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, _i32p, _f64p]),
     "egx_graph_pagerank_ex": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), _f64p, ctypes.POINTER(SolveStats),
                                              ctypes.c_int32]),
+    "egx_graph_pagerank_noalloc": (ctypes.c_int, [_vp, ctypes.POINTER(SolveOpts), ctypes.c_void_p, _i32p]),
     "egx_graph_ppr_single": (ctypes.c_int, [_vp, _i32p, ctypes.c_int32, ctypes.POINTER(SolveOpts), _f64p,
                                             ctypes.POINTER(SolveStats), ctypes.c_int32]),
     "egx_multi_pagerank": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int32, ctypes.POINTER(SolveOpts),

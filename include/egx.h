@@ -213,6 +213,23 @@ int egx_graph_download_csr(const egx_graph *g, int32_t *V, int32_t *E, double *W
 int egx_graph_pagerank(egx_graph *g, const egx_solve_opts *opts, double *PR, int32_t *iterations,
                        double *kernel_seconds);
 
+/* egx_graph_pagerank with no allocation in the call.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-solve workspace that egx_graph_pagerank allocates on a graph's first
+ * solve comes from a static 64 MiB device array instead (one per GPU, held
+ * until the process ends): FP32 graphs up to about 4M nodes, FP64 up to
+ * about 2M; larger graphs return EGX_UNSUPPORTED_GRAPH with the exact need in
+ * egx_last_error.  opts as egx_graph_pagerank (check_every as there; no
+ * per-launch timing, no graph replay).  PR_device: caller-owned device memory
+ * on the graph's device, n doubles, written with the normalised ranks in
+ * caller node order (the values egx_graph_pagerank returns, bit for bit).
+ * iterations may be NULL.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU race, as do overlapping calls on one graph
+ * through any entry point. */
+int egx_graph_pagerank_noalloc(egx_graph *g, const egx_solve_opts *opts, double *PR_device,
+                               int32_t *iterations);
+
 /* Solve statistics; spmv/reduce sums are filled with EGX_SOLVE_TIME_LAUNCHES
  * (a hipEvent pair around every launch on the graph's stream). */
 typedef struct {
