@@ -13,13 +13,14 @@ from .exception import EasyGraphError, EasyGraphException, EasyGraphNotImplement
 from .native import DeviceGraph, NativeLibraryMissing
 from .eigenvector import eigenvector_centrality, set_cpu_eigenvector
 from .katz import katz_centrality, set_cpu_katz
-from .pagerank import pagerank, set_cpu_pagerank
+from .pagerank import pagerank, set_cpp_pagerank, set_cpu_pagerank
 
 __version__ = "0.1.0"
 
 __all__ = [
     "pagerank",
     "set_cpu_pagerank",
+    "set_cpp_pagerank",
     "eigenvector_centrality",
     "katz_centrality",
     "set_cpu_eigenvector",

@@ -1,8 +1,9 @@
-"""Compile the HIP sources of eggpu_amd into ``eggpu_amd/libegx.so`` (gfx950).
+"""Compile the HIP sources of eggpu_amd into ``eggpu_amd/libegx.so`` (gfx950)
+and the host extension ``eggpu_amd/_egx_host*.so`` (gcc, CPython C API).
 
 Run ``python -m eggpu_amd.build``.  Every call recompiles; objects go to
-``eggpu_amd/_build/`` and the library is written in-tree, beside the
-package that loads it.
+``eggpu_amd/_build/`` and the libraries are written in-tree, beside the
+package that loads them.
 """
 from __future__ import annotations
 
@@ -61,7 +62,29 @@ def _compile(src):
     return obj
 
 
+def build_host(verbose=False):
+    """The CPython extension that reads dict adjacencies into CSR arrays
+    (csrc/egx_host.c), built with the host C compiler beside the package."""
+    import sysconfig
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        raise RuntimeError("a C compiler (gcc) is needed for eggpu_amd's host extension")
+    out = os.path.join(HERE, "_egx_host" + sysconfig.get_config_var("EXT_SUFFIX"))
+    tmp = out + ".tmp"
+    cmd = [cc, "-O2", "-Wall", "-Wextra", "-shared", "-fPIC", "-I", sysconfig.get_paths()["include"],
+           os.path.join(CSRC, "egx_host.c"), "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"host extension build failed:\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 def build(verbose=False):
+    build_host(verbose)
     os.makedirs(BUILD_DIR, exist_ok=True)
     workers = min(len(SOURCES), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:

@@ -21,54 +21,38 @@ import copy
 import weakref
 
 
-class _CacheAwareEdgeAttrDict(dict):
-    """Edge attributes that invalidate graph-derived state when mutated."""
+def _mutator(name):
+    """dict method `name`, followed by a notice to the owning graph."""
+    base = getattr(dict, name)
 
-    def __init__(self, owner, *args, **kwargs):
-        self._owner_ref = weakref.ref(owner)
-        dict.__init__(self, *args, **kwargs)
-
-    def _invalidate(self):
-        owner = self._owner_ref()
+    def method(self, *args, **kwargs):
+        out = base(self, *args, **kwargs)
+        owner = self._owner()
         if owner is not None:
             owner._clear_cache()
+        return out
 
-    def __setitem__(self, key, value):
-        dict.__setitem__(self, key, value)
-        self._invalidate()
+    method.__name__ = name
+    method.__qualname__ = f"_EdgeData.{name}"
+    return method
 
-    def __delitem__(self, key):
-        dict.__delitem__(self, key)
-        self._invalidate()
 
-    def clear(self):
-        if self:
-            dict.clear(self)
-            self._invalidate()
+class _EdgeData(dict):
+    """The attribute dict of one edge.  A plain dict for every read; every
+    dict method that can write is wrapped so that the owning graph drops its
+    derived state (cache and generation) afterwards.  A write that turns out
+    to change nothing still notifies, which only costs a rebuild.  Pickles and
+    deep-copies as the plain dict it holds, without the owner."""
 
-    def pop(self, key, *args):
-        existed = key in self
-        value = dict.pop(self, key, *args)
-        if existed:
-            self._invalidate()
-        return value
+    __slots__ = ("_owner",)
 
-    def popitem(self):
-        value = dict.popitem(self)
-        self._invalidate()
-        return value
+    def __init__(self, owner, *args, **kwargs):
+        dict.__init__(self, *args, **kwargs)
+        self._owner = weakref.ref(owner)
 
-    def setdefault(self, key, default=None):
-        if key in self:
-            return dict.__getitem__(self, key)
-        value = dict.setdefault(self, key, default)
-        self._invalidate()
-        return value
-
-    def update(self, *args, **kwargs):
-        if args or kwargs:
-            dict.update(self, *args, **kwargs)
-            self._invalidate()
+    for _name in ("__setitem__", "__delitem__", "__ior__", "update", "setdefault", "pop", "popitem", "clear"):
+        locals()[_name] = _mutator(_name)
+    del _name
 
     def __reduce__(self):
         return (dict, (dict(self),))
@@ -174,7 +158,7 @@ class Graph:
     def _link(self, u, v, attrs):
         data = self._adj[u].get(v)
         if data is None:
-            data = _CacheAwareEdgeAttrDict(self)
+            data = _EdgeData(self)
             self._adj[u][v] = data
             self._adj[v][u] = data
         dict.update(data, attrs)
@@ -252,7 +236,7 @@ class DiGraph(Graph):
     def _link(self, u, v, attrs):
         data = self._adj[u].get(v)
         if data is None:
-            data = _CacheAwareEdgeAttrDict(self)
+            data = _EdgeData(self)
             self._adj[u][v] = data
             self._pred[v][u] = data
         dict.update(data, attrs)

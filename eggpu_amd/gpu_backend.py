@@ -1,59 +1,81 @@
-"""PageRank slice of the GPU backend, over the MI355X library.
+"""PageRank slice of EasyGraph's GPU backend, over the MI355X library.
 
-Mirrors the contract of ``easygraph/utils/gpu_eggpu_backend.py`` for
-PageRank, so this module can stand where the reference backend stands:
+This module can stand where ``easygraph/utils/gpu_eggpu_backend.py`` stands
+for PageRank: the same entry (``pagerank(G, alpha, max_iter, eps, weight)``,
+``:1782-1821``), the same switches, the same prepared-context cache key and
+dense-result contract (``:254-348,1092-1126``), which the reference's own
+contract tests address by name.  Below that surface it is built differently:
 
-* ``pagerank(G, alpha=0.85, max_iter=200, eps=1e-6, weight=None)``
-  (``gpu_eggpu_backend.py:1782-1821``): context, opt-in result cache, native
-  call, read-only dense mapping, ``NotImplementedError`` for weighted
-  PageRank on a topology-only bulk graph (``:1784-1788``);
-* the prepared graph context and its validity rules
-  (``:1590-1716``): mutation generation, node count, and an O(|V|) stamp
-  re-checked every ``EASYGRAPH_GPU_CTX_VALIDATE_EVERY``-th reuse;
-* the dense-result contract (``_normalize_dense_array`` ``:1092-1126``,
-  ``_DenseValueDict`` ``:254-348``): exact length or a leading sentinel,
-  otherwise fail closed.
-
-What differs is below the native call: instead of a pybind C++ graph and
-``cuda_pagerank``, the graph's out-CSR is built with numpy in
-``Graph::gen_CSR`` order and handed to ``libegx.so`` once; the pull CSC,
-bins and iteration workspace stay resident in HBM on a
-:class:`eggpu_amd.native.DeviceGraph` cached in ``G.cache`` until the graph
-mutates.
+* a graph's context is one dict, validated by the mutation generation, the
+  node count and, every ``EASYGRAPH_GPU_CTX_VALIDATE_EVERY``-th reuse, a
+  64-bit fingerprint of node keys, order, degrees and first neighbours,
+  computed in C;
+* the adjacency is read into int32/float64 CSR arrays by a small C
+  extension (``csrc/egx_host.c``), not by per-arc Python code, and handed to
+  ``libegx.so`` once; unweighted graphs skip the per-row column sort, which
+  cannot change the pull CSC built from them (sources ascend in every
+  destination row whatever the column order inside a source row), while
+  weighted graphs keep ``Graph::gen_CSR`` order, since the out-weight sums
+  follow it;
+* the pull CSC, bins and workspace stay in HBM on a
+  :class:`eggpu_amd.native.DeviceGraph` cached beside the context until the
+  graph mutates;
+* results are read-only mappings over the dense rank vector.
 """
 from __future__ import annotations
 
 import copy
 import importlib
+import itertools
 import os
-import time
 import weakref
 from collections.abc import Mapping
-
-import itertools
 
 import numpy as np
 
 from .gpu_runtime import gpu_runtime_enabled
 
+try:  # the C adjacency reader; absent (not built) -> the same arrays from Python
+    from . import _egx_host as _host
+except ImportError:  # pragma: no cover - exercised only without the extension
+    _host = None
 
-def _env_true(name, default="FALSE"):
-    return os.environ.get(name, default).strip().upper() in {"1", "TRUE", "ON", "YES"}
+
+# ---------------------------------------------------------------------------
+# switches: names, defaults and parsing are the reference's contract
+# (gpu_eggpu_backend.py:12-74)
+# ---------------------------------------------------------------------------
+_TRUE_WORDS = frozenset({"1", "TRUE", "ON", "YES"})
 
 
-_LAST_KERNEL_SECONDS = {}
-_LAST_ITERATIONS = {}
-_PREPARED_CACHE_KEY = "__eggpu_prepared_graph__"
-_DEVICE_GRAPH_KEY = "__egx_device_graph__"
-_HASH_MASK = (1 << 64) - 1
-_RESULT_CACHE_ENABLED = _env_true("EASYGRAPH_GPU_RESULT_CACHE")
-_RESULT_CACHE_RETURN_COPY = _env_true("EASYGRAPH_GPU_RESULT_CACHE_RETURN_COPY", "TRUE")
-_RESULT_CACHE_MAX_ITEMS = max(1, int(os.environ.get("EASYGRAPH_GPU_RESULT_CACHE_MAX_ITEMS", "8")))
-_RESULT_CACHE_MAX_NODES = max(0, int(os.environ.get("EASYGRAPH_GPU_RESULT_CACHE_MAX_NODES", "2000000")))
-_GRAPH_CONTEXT_CACHE_DISABLED = _env_true("EASYGRAPH_GPU_DISABLE_GRAPH_CONTEXT_CACHE", "")
-_CPP_GRAPH_CACHE_DISABLED = _env_true("EASYGRAPH_GPU_DISABLE_CPP_GRAPH_CACHE", "")
-_GRAPH_CTX_VALIDATE_EVERY = max(1, int(os.environ.get("EASYGRAPH_GPU_CTX_VALIDATE_EVERY", "32")))
-_FALLBACK_GRAPH_CACHE = weakref.WeakKeyDictionary()
+def _flag(name, default=False):
+    raw = os.environ.get(name)
+    return bool(default) if raw is None else raw.strip().upper() in _TRUE_WORDS
+
+
+def _count(name, default, floor):
+    return max(floor, int(os.environ.get(name, str(default))))
+
+
+_RESULT_CACHE_ENABLED = _flag("EASYGRAPH_GPU_RESULT_CACHE")
+_RESULT_CACHE_RETURN_COPY = _flag("EASYGRAPH_GPU_RESULT_CACHE_RETURN_COPY", True)
+_RESULT_CACHE_MAX_ITEMS = _count("EASYGRAPH_GPU_RESULT_CACHE_MAX_ITEMS", 8, 1)
+_RESULT_CACHE_MAX_NODES = _count("EASYGRAPH_GPU_RESULT_CACHE_MAX_NODES", 2000000, 0)
+_CONTEXT_CACHE_OFF = _flag("EASYGRAPH_GPU_DISABLE_GRAPH_CONTEXT_CACHE")
+_DEVICE_GRAPH_CACHE_OFF = _flag("EASYGRAPH_GPU_DISABLE_CPP_GRAPH_CACHE")
+_VALIDATE_EVERY = _count("EASYGRAPH_GPU_CTX_VALIDATE_EVERY", 32, 1)
+# the dict-returning native route: the PageRank switch, else the global one
+# when that is set, else off
+_PR_PREFER_DICT = _flag(
+    "EASYGRAPH_GPU_PR_PREFER_CPP_DICT",
+    _flag("EASYGRAPH_GPU_PREFER_CPP_DICT") if "EASYGRAPH_GPU_PREFER_CPP_DICT" in os.environ else False,
+)
+
+_CTX_KEY = "__eggpu_prepared_graph__"  # the reference's key in G.cache
+_DEVICE_KEY = "__egx_device_graph__"
+_ctx_ids = itertools.count(1)
+_side_caches = weakref.WeakKeyDictionary()  # G.cache stand-in for graphs without one
+_last_solve = {}  # name -> {"seconds": ..., "iterations": ...}
 
 
 def eggpu_backend_enabled():
@@ -61,24 +83,25 @@ def eggpu_backend_enabled():
 
 
 def set_last_kernel_time(key, seconds):
+    entry = _last_solve.setdefault(key, {})
     if seconds is None:
-        _LAST_KERNEL_SECONDS.pop(key, None)
+        entry.pop("seconds", None)
     else:
-        _LAST_KERNEL_SECONDS[key] = float(seconds)
+        entry["seconds"] = float(seconds)
 
 
 def get_last_kernel_time(key):
-    return _LAST_KERNEL_SECONDS.get(key)
+    return _last_solve.get(key, {}).get("seconds")
 
 
 def get_last_iterations(key):
-    """Iterations the last solve performed (new; the reference does not expose it)."""
-    return _LAST_ITERATIONS.get(key)
+    """Iterations of the last solve (new; the reference does not expose it)."""
+    return _last_solve.get(key, {}).get("iterations")
 
 
 def _native():
-    # Looked up through sys.modules so tests can substitute a fake module,
-    # as the reference's tests do for cpp_easygraph.
+    # resolved through sys.modules on every call, so a test can put a fake
+    # native module in place, as the reference's tests do for cpp_easygraph
     return importlib.import_module("eggpu_amd.native")
 
 
@@ -87,495 +110,437 @@ def _is_bulk_graph(G):
 
 
 def _bulk_weight_available(G, weight):
-    return (
-        _is_bulk_graph(G)
-        and weight is not None
-        and getattr(G, "weights_path", None) is not None
-        and getattr(G, "weight_key", None) == str(weight)
-    )
+    if not _is_bulk_graph(G) or weight is None:
+        return False
+    return getattr(G, "weights_path", None) is not None and getattr(G, "weight_key", None) == str(weight)
 
 
 # ---------------------------------------------------------------------------
 # dense results
 # ---------------------------------------------------------------------------
 class _DenseResultShapeError(ValueError):
-    """A native dense result violates its documented shape."""
-
-
-def _dense_shape_message(result_name, expected_shapes, actual_shape):
-    expected = " or ".join(str(tuple(s)) for s in expected_shapes)
-    return f"EGGPU {result_name} returned invalid dense shape {tuple(actual_shape)}; expected {expected}"
+    """A native dense result does not have its documented shape."""
 
 
 def _normalize_dense_array(raw_values, target_len, dtype="float64", *, allow_leading_sentinel=False,
                            result_name="dense vector"):
-    """Accept exactly ``(target_len,)`` or, when allowed, ``(target_len+1,)``
-    with a leading sentinel that is dropped; anything else raises."""
+    """The result as a 1-D array of exactly ``target_len`` entries; with
+    ``allow_leading_sentinel`` also ``target_len + 1`` entries, whose first is
+    dropped.  Any other shape raises :class:`_DenseResultShapeError`."""
     try:
         arr = np.asarray(raw_values, dtype=dtype)
-    except Exception as exc:
-        raise _DenseResultShapeError(
-            f"EGGPU {result_name} could not be converted to a one-dimensional dense result: {exc}"
-        ) from exc
-    exact = (int(target_len),)
-    with_sentinel = (int(target_len) + 1,)
-    if arr.ndim == 1 and arr.shape == exact:
+    except (TypeError, ValueError) as exc:
+        raise _DenseResultShapeError(f"EGGPU {result_name} is not a one-dimensional numeric array: {exc}") from exc
+    n = int(target_len)
+    accepted = ((n,), (n + 1,)) if allow_leading_sentinel else ((n,),)
+    if arr.shape == accepted[0]:
         return arr
-    if allow_leading_sentinel and arr.ndim == 1 and arr.shape == with_sentinel:
+    if arr.shape in accepted:
         return arr[1:]
-    shapes = [exact] + ([with_sentinel] if allow_leading_sentinel else [])
-    raise _DenseResultShapeError(_dense_shape_message(result_name, shapes, arr.shape))
+    raise _DenseResultShapeError(
+        f"EGGPU {result_name} returned invalid dense shape {arr.shape}; expected "
+        + " or ".join(str(s) for s in accepted)
+    )
 
 
 class _DenseValueDict(Mapping):
-    """Read-only node -> value mapping over a dense float64 vector.
+    """Read-only ``node -> value`` view of one dense result vector.
 
-    A ``Mapping`` and not a ``dict`` subclass: equality, ``dict(result)``,
-    pickling and iteration behave like the logical dict, mutation raises
-    ``TypeError``, and no per-node Python objects exist until asked for."""
+    It is a Mapping, not a dict: comparisons, ``dict(result)``, iteration and
+    pickling behave as for the logical dict, writes raise ``TypeError``, and
+    no per-node objects exist until a value is read.  A node is located
+    through ``node_to_idx`` when given, arithmetically when the nodes are
+    ``range(n)``, or through an index built on first use."""
+
+    __slots__ = ("_keys", "_vals", "_index", "_arith")
 
     def __init__(self, nodes, values, node_to_idx=None, dtype=float):
-        self._nodes = nodes if isinstance(nodes, range) else list(nodes)
-        self._values = _normalize_dense_array(values, len(self._nodes), dtype=dtype,
-                                              result_name="dense node-value mapping")
-        self._values.setflags(write=False)
-        self._node_to_idx = node_to_idx if isinstance(node_to_idx, dict) else None
-        self._range_ids = (isinstance(self._nodes, range) and self._nodes.start == 0
-                           and self._nodes.step == 1)
-        self._cast = int if self._values.dtype.kind in "iu" else float
-        self._lazy_index = None
+        self._keys = nodes if isinstance(nodes, range) else list(nodes)
+        vals = _normalize_dense_array(values, len(self._keys), dtype=np.dtype(dtype),
+                                      result_name="dense node-value mapping")
+        vals = vals.view()
+        vals.flags.writeable = False
+        self._vals = vals
+        self._index = node_to_idx if isinstance(node_to_idx, dict) else None
+        k = self._keys
+        self._arith = self._index is None and isinstance(k, range) and k.start == 0 and k.step == 1
 
-    def _pos(self, key):
-        if self._node_to_idx is not None:
-            return self._node_to_idx.get(key)
-        if self._range_ids:
-            if isinstance(key, (int, np.integer)) and not isinstance(key, bool) and 0 <= key < len(self._nodes):
-                return int(key)
-            return None
-        if self._lazy_index is None:
-            self._lazy_index = {node: i for i, node in enumerate(self._nodes)}
-        try:
-            return self._lazy_index.get(key)
-        except TypeError:
-            return None
-
-    def __len__(self):
-        return len(self._nodes)
-
-    def __iter__(self):
-        return iter(self._nodes)
-
-    def __contains__(self, key):
-        return self._pos(key) is not None
+    def _slot(self, key):
+        if self._index is not None:
+            return self._index.get(key)
+        if self._arith:
+            if isinstance(key, bool) or not isinstance(key, (int, np.integer)):
+                return None
+            return int(key) if 0 <= key < len(self._keys) else None
+        self._index = dict(zip(self._keys, range(len(self._keys))))
+        return self._slot(key)
 
     def __getitem__(self, key):
-        i = self._pos(key)
+        try:
+            i = self._slot(key)
+        except TypeError:  # unhashable key
+            i = None
         if i is None:
             raise KeyError(key)
-        return self._cast(self._values[int(i)])
+        return self._vals[i].item()
 
-    def get(self, key, default=None):
-        i = self._pos(key)
-        return default if i is None else self._cast(self._values[int(i)])
+    def __contains__(self, key):
+        try:
+            return self._slot(key) is not None
+        except TypeError:
+            return False
 
-    def keys(self):
-        return self._nodes if self._range_ids else list(self._nodes)
+    def __iter__(self):
+        return iter(self._keys)
+
+    def __len__(self):
+        return len(self._keys)
 
     def values(self):
-        return self._values
-
-    def items(self):
-        for i, node in enumerate(self._nodes):
-            yield node, self._cast(self._values[i])
-
-    def copy(self):
-        return dict(self.items())
-
-    def to_dict(self):
-        return dict(self.items())
+        """The dense vector itself (read-only), in node order."""
+        return self._vals
 
     def to_numpy(self, copy=False):
-        return self._values.copy() if copy else self._values
+        return self._vals.copy() if copy else self._vals
+
+    def to_dict(self):
+        return dict(zip(self._keys, self._vals.tolist()))
+
+    copy = to_dict
 
     def tolist(self):
-        return self._values.tolist()
+        return self._vals.tolist()
 
     def __array__(self, dtype=None, copy=None):
-        return np.asarray(self._values, dtype=dtype)
+        return np.asarray(self._vals, dtype=dtype)
 
     def __reduce__(self):
-        return (_DenseValueDict, (self._nodes, np.array(self._values), self._node_to_idx))
+        return (type(self), (self._keys, np.array(self._vals), self._index, self._vals.dtype))
 
     def __repr__(self):
-        if len(self._nodes) > 1000:
-            return (f"{type(self).__name__}(nodes={len(self._nodes)}, dtype={self._values.dtype}, "
-                    "materialized=False)")
+        if len(self._keys) > 1000:
+            return f"{type(self).__name__}({len(self._keys)} nodes, {self._vals.dtype})"
         return repr(self.to_dict())
 
 
 # ---------------------------------------------------------------------------
-# prepared graph context (gpu_eggpu_backend.py:1590-1716)
+# the prepared graph context (the contract of gpu_eggpu_backend.py:1590-1716)
 # ---------------------------------------------------------------------------
 def _graph_cache_dict(G):
     cache = getattr(G, "cache", None)
     if isinstance(cache, dict):
         return cache
     try:
-        cache = _FALLBACK_GRAPH_CACHE.get(G)
-        if cache is None:
-            cache = {}
-            _FALLBACK_GRAPH_CACHE[G] = cache
-        return cache
-    except TypeError:
+        return _side_caches.setdefault(G, {})
+    except TypeError:  # neither a cache attribute nor weak-referenceable
         return None
 
 
 def _adj_dict(G):
-    if hasattr(G, "_adj"):
-        return G._adj
-    return G.adj
+    return G._adj if hasattr(G, "_adj") else G.adj
 
 
-def _mix64(h, x):
-    h ^= (int(x) + 0x9E3779B97F4A7C15 + ((h << 6) & _HASH_MASK) + (h >> 2)) & _HASH_MASK
-    return h & _HASH_MASK
+def _node_list(G):
+    nodes = G.nodes
+    return list(nodes.keys()) if hasattr(nodes, "keys") else list(nodes)
+
+
+def _fingerprint_py(adj):
+    """Python form of _egx_host.stamp (used only without the extension)."""
+    import hashlib
+
+    h = hashlib.blake2b(digest_size=8)
+    for i, (u, nbrs) in enumerate(adj.items()):
+        first = next(iter(nbrs), None)
+        h.update(np.array([hash(u), len(nbrs), i, hash(first)], dtype=np.int64).tobytes())
+    return int.from_bytes(h.digest(), "little")
 
 
 def _graph_stamp(G):
-    """O(|V|) fingerprint: node count, arc slots, and a position-dependent
-    hash of every node's hash and degree plus the first neighbours of the
-    first 32 nodes.  Node hashes and degrees are gathered at C speed (map)
-    and mixed with wrapping uint64 arithmetic in numpy."""
+    """(directed, nodes, arc slots, fingerprint): what a context re-checks
+    every _VALIDATE_EVERY-th reuse, to see changes made behind the mutation
+    counter (direct writes into the adjacency dicts)."""
     adj = _adj_dict(G)
-    n = len(adj)
-    hu = np.fromiter(map(hash, adj), dtype=np.int64, count=n).view(np.uint64)
-    dg = np.fromiter(map(len, adj.values()), dtype=np.int64, count=n).view(np.uint64)
-    slots = int(dg.sum(dtype=np.uint64)) if n else 0
-    with np.errstate(over="ignore"):
-        x = hu * np.uint64(0x9E3779B97F4A7C15) + dg * np.uint64(0xC2B2AE3D27D4EB4F) + \
-            np.arange(n, dtype=np.uint64) * np.uint64(0x165667B19E3779F9)
-        x ^= x >> np.uint64(31)
-        x *= np.uint64(0xBF58476D1CE4E5B9)
-        x ^= x >> np.uint64(29)
-    h = 1469598103934665603
-    if n:
-        h = _mix64(h, int(np.bitwise_xor.reduce(x)))
-        h = _mix64(h, int(x.sum(dtype=np.uint64)))
-    for i, (u, nbrs) in enumerate(adj.items()):
-        if i >= 32:
-            break
-        for j, v in enumerate(nbrs):
-            if j >= 4:
-                break
-            h = _mix64(h, hash(v))
-    return (bool(G.is_directed()), n, slots, h)
+    if _host is not None and type(adj) is dict:
+        fp, slots = _host.stamp(adj)
+    else:
+        fp, slots = _fingerprint_py(adj), sum(map(len, adj.values()))
+    return (bool(G.is_directed()), len(adj), slots, fp)
 
 
-def _prepared_graph_usable(G, prepared):
-    if not isinstance(prepared, dict):
-        return False
-    if bool(G.is_directed()) != bool(prepared.get("directed")):
+def _identity_labels(prepared):
+    """True when the nodes are exactly the ints 0..n-1 in order, so that a
+    node is its own index (computed once per context)."""
+    flag = prepared.get("identity_labels")
+    if flag is None:
+        nodes = prepared["nodes"]
+        flag = set(map(type, nodes)) <= {int} and nodes == list(range(len(nodes)))
+        prepared["identity_labels"] = flag
+    return flag
+
+
+def _context_current(G, ctx):
+    if not isinstance(ctx, dict) or bool(ctx.get("directed")) != bool(G.is_directed()):
         return False
     if _is_bulk_graph(G):
-        return prepared.get("bulk_signature") == getattr(G, "_eggpu_bulk_signature", None)
-    gen = getattr(G, "_mutation_generation", None)
-    if gen is not None and prepared.get("mutation_generation") != gen:
+        return ctx.get("bulk_signature") == getattr(G, "_eggpu_bulk_signature", None)
+    if ctx.get("mutation_generation") != getattr(G, "_mutation_generation", None):
         return False
-    if len(_adj_dict(G)) != int(prepared.get("node_count", -1)):
+    if len(_adj_dict(G)) != ctx.get("node_count"):
         return False
-    hits = int(prepared.get("reuse_hits", 0)) + 1
-    prepared["reuse_hits"] = hits
-    if hits % _GRAPH_CTX_VALIDATE_EVERY != 0:
-        return True
-    return prepared.get("stamp") == _graph_stamp(G)
+    ctx["uses"] = ctx.get("uses", 0) + 1
+    return ctx["uses"] % _VALIDATE_EVERY != 0 or ctx.get("stamp") == _graph_stamp(G)
+
+
+def _new_context(G):
+    directed = bool(G.is_directed())
+    if _is_bulk_graph(G):
+        return {"ctx_id": next(_ctx_ids), "directed": directed, "bulk_csr": True,
+                "bulk_signature": getattr(G, "_eggpu_bulk_signature", None), "nodes": G.nodes,
+                "node_to_idx": None, "node_count": int(len(G)), "mutation_generation": None, "stamp": None,
+                "uses": 0, "result_cache": {}}
+    nodes = _node_list(G)
+    ctx = {"ctx_id": next(_ctx_ids), "directed": directed, "nodes": nodes, "node_count": len(nodes),
+           "mutation_generation": getattr(G, "_mutation_generation", None), "stamp": _graph_stamp(G),
+           "uses": 0, "result_cache": {}}
+    # nodes 0..n-1 in order are their own indices: no index dict is needed
+    ctx["node_to_idx"] = None if _identity_labels(ctx) else dict(zip(nodes, range(len(nodes))))
+    return ctx
 
 
 def _prepare_graph_cache(G):
-    cache = None if _GRAPH_CONTEXT_CACHE_DISABLED else _graph_cache_dict(G)
-    prepared = cache.get(_PREPARED_CACHE_KEY) if cache is not None else None
-    if prepared is not None and _prepared_graph_usable(G, prepared):
-        return prepared
-    directed = bool(G.is_directed())
-    if _is_bulk_graph(G):
-        n = int(len(G))
-        prepared = {
-            "ctx_id": time.time_ns(),
-            "stamp": (directed, n, int(getattr(G, "num_entries", 0)),
-                      hash(getattr(G, "_eggpu_bulk_signature", None)) & _HASH_MASK),
-            "bulk_csr": True,
-            "bulk_signature": getattr(G, "_eggpu_bulk_signature", None),
-            "node_count": n,
-            "directed": directed,
-            "nodes": G.nodes,
-            "node_to_idx": None,
-            "result_cache": {},
-            "result_cache_order": [],
-            "reuse_hits": 0,
-            "mutation_generation": None,
-        }
-    else:
-        nodes = list(G.nodes.keys()) if hasattr(G.nodes, "keys") else list(G.nodes)
-        prepared = {
-            "ctx_id": time.time_ns(),
-            "stamp": _graph_stamp(G),
-            "node_count": len(nodes),
-            "directed": directed,
-            "nodes": nodes,
-            "node_to_idx": {node: i for i, node in enumerate(nodes)},
-            "result_cache": {},
-            "result_cache_order": [],
-            "reuse_hits": 0,
-            "mutation_generation": getattr(G, "_mutation_generation", None),
-        }
+    """The graph's context: the cached one while still current, else a new
+    one (stored in G.cache unless the context cache is switched off)."""
+    cache = None if _CONTEXT_CACHE_OFF else _graph_cache_dict(G)
     if cache is not None:
-        cache[_PREPARED_CACHE_KEY] = prepared
-    return prepared
+        ctx = cache.get(_CTX_KEY)
+        if ctx is not None and _context_current(G, ctx):
+            return ctx
+    ctx = _new_context(G)
+    if cache is not None:
+        cache[_CTX_KEY] = ctx
+    return ctx
 
 
-def _graph_context(G):
-    return _prepare_graph_cache(G)
+_graph_context = _prepare_graph_cache
 
 
 # ---------------------------------------------------------------------------
-# opt-in result cache (gpu_eggpu_backend.py:1723-1779)
+# opt-in result cache (the contract of gpu_eggpu_backend.py:1723-1779): an
+# insertion-ordered dict per context, least recently stored evicted first,
+# holding snapshots so that callers cannot change what later calls receive
 # ---------------------------------------------------------------------------
-def _cache_key(name, *parts):
-    return tuple([name] + [round(float(p), 12) if isinstance(p, float) else p for p in parts])
-
-
-def _allow_result_cache(prepared):
-    if not _RESULT_CACHE_ENABLED:
-        return False
-    nodes = prepared.get("nodes")
-    if nodes is None:
-        return False
-    return not (_RESULT_CACHE_MAX_NODES > 0 and len(nodes) > _RESULT_CACHE_MAX_NODES)
-
-
-def _copy_for_return(obj):
+def _snapshot(obj):
     return copy.deepcopy(obj) if _RESULT_CACHE_RETURN_COPY else obj
 
 
 def _result_cache_get(prepared, key):
-    cache = prepared.get("result_cache")
-    if not isinstance(cache, dict):
+    store = prepared.get("result_cache")
+    hit = store.get(key) if isinstance(store, dict) else None
+    if hit is None:
         return None
-    entry = cache.get(key)
-    if entry is None:
-        return None
-    return _copy_for_return(entry.get("result")), entry.get("kernel")
+    result, kernel = hit
+    return _snapshot(result), kernel
 
 
 def _result_cache_put(prepared, key, result, kernel):
-    if not _allow_result_cache(prepared):
+    if not _RESULT_CACHE_ENABLED or prepared.get("nodes") is None:
         return
-    cache = prepared.setdefault("result_cache", {})
-    order = prepared.setdefault("result_cache_order", [])
-    cache[key] = {"result": _copy_for_return(result), "kernel": kernel}
-    if key in order:
-        order.remove(key)
-    order.append(key)
-    while len(order) > _RESULT_CACHE_MAX_ITEMS:
-        cache.pop(order.pop(0), None)
+    if 0 < _RESULT_CACHE_MAX_NODES < len(prepared["nodes"]):
+        return
+    store = prepared.setdefault("result_cache", {})
+    store.pop(key, None)
+    store[key] = (_snapshot(result), kernel)
+    while len(store) > _RESULT_CACHE_MAX_ITEMS:
+        del store[next(iter(store))]
 
 
 # ---------------------------------------------------------------------------
-# graph -> out-CSR (Graph::gen_CSR order) -> DeviceGraph
+# graph -> CSR arrays
 # ---------------------------------------------------------------------------
-def _identity_labels(prepared):
-    """True when the node list is exactly the ints 0..n-1 in order (so a
-    node is its own index); computed once per graph context."""
-    flag = prepared.get("identity_labels")
-    if flag is None:
-        nodes = prepared["nodes"]
-        flag = all(type(x) is int for x in nodes) and nodes == list(range(len(nodes)))
-        prepared["identity_labels"] = flag
-    return flag
+def _walk_py(adj, index, n, drop_self, weight):
+    """Python form of _egx_host.arcs (used only without the extension)."""
+    V = np.zeros(n + 1, dtype=np.int32)
+    cols, wts = [], []
+    for row, nbrs in enumerate(adj.values()):
+        for v, data in nbrs.items():
+            col = v if index is None else index[v]
+            if drop_self and col == row:
+                continue
+            cols.append(col)
+            if weight is not None:
+                wts.append(float(data.get(weight, 1.0)) if isinstance(data, dict) else 1.0)
+        V[row + 1] = len(cols)
+    E = np.asarray(cols, dtype=np.int32)
+    W = np.asarray(wts, dtype=np.float64) if weight is not None else None
+    return V, E, W
+
+
+def _out_arcs(G, prepared, weight, sort):
+    """(V, E, W) of the out-arcs: rows in node order, columns in adjacency
+    order or, with sort, ascending; directed graphs keep self-loops
+    (graph_convert.cpp:110-134), undirected ones drop them and hold every
+    pair in both directions (graph_convert.cpp:72-108)."""
+    adj = _adj_dict(G)
+    n = len(prepared["nodes"])
+    index = None if _identity_labels(prepared) else prepared["node_to_idx"]
+    drop_self = not prepared["directed"]
+    if _host is not None and type(adj) is dict:
+        stamp = prepared.get("stamp")
+        cap = stamp[2] if stamp and prepared.get("uses", 0) == 0 else sum(map(len, adj.values()))
+        if cap >= 2**31 - 1:
+            raise ValueError("adjacency entry count exceeds the signed 32-bit EGGPU CSR limit")
+        V = np.empty(n + 1, dtype=np.int32)
+        E = np.empty(max(cap, 1), dtype=np.int32)
+        W = np.empty(max(cap, 1), dtype=np.float64) if weight is not None else None
+        m = _host.arcs(adj, index, n, drop_self, V, E, weight, W)
+        E = E[:m]
+        W = W[:m] if W is not None else None
+    else:
+        V, E, W = _walk_py(adj, index, n, drop_self, weight)
+    if sort and len(E) > 1:
+        if _host is not None:
+            _host.sort_rows(V, E, W)
+        else:
+            rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(V))
+            order = np.lexsort((E, rows))
+            E = E[order]
+            W = W[order] if W is not None else None
+    return V, E, W
+
+
+def graph_out_csr(G, prepared, weight=None):
+    """Out-CSR in ``Graph::gen_CSR`` order (``graph.cpp:824-886``,
+    ``:719-822``): rows follow the node order, columns ascend within a row;
+    missing weight attributes count as 1.0."""
+    if _is_bulk_graph(G):
+        return G.csr_arrays(weight)
+    return _out_arcs(G, prepared, weight, sort=True)
 
 
 def adjacency_csr(G, prepared):
     """(V, E) of the adjacency as Katz centrality reads it
     (``katz_centrality.py:73-81``): row u lists every adjacency key of u in
     adjacency order, self-loops included, both directions of an undirected
-    edge; no sorting and no weights."""
+    edge; no sorting, no weights."""
     if _is_bulk_graph(G):
         V, E, _ = G.csr_arrays(None)
         return V, E
-    idx = prepared["node_to_idx"]
-    nodes = prepared["nodes"]
     adj = _adj_dict(G)
-    identity = _identity_labels(prepared)
-    get = idx.__getitem__
-    counts, dst = [], []
-    for u in nodes:
-        nbrs = adj[u]
-        counts.append(len(nbrs))
-        dst.extend(nbrs if identity else map(get, nbrs))
-    V = np.zeros(len(nodes) + 1, dtype=np.int64)
-    if counts:
-        V[1:] = np.cumsum(np.asarray(counts, dtype=np.int64))
-    if V[-1] >= 2**31 - 1:
-        raise ValueError("adjacency entry count exceeds the signed 32-bit EGGPU CSR limit")
-    return V.astype(np.int32), np.fromiter(dst, dtype=np.int64, count=len(dst)).astype(np.int32)
+    n = len(prepared["nodes"])
+    index = None if _identity_labels(prepared) else prepared["node_to_idx"]
+    if _host is not None and type(adj) is dict:
+        cap = sum(map(len, adj.values()))
+        V = np.empty(n + 1, dtype=np.int32)
+        E = np.empty(max(cap, 1), dtype=np.int32)
+        m = _host.arcs(adj, index, n, False, V, E, None, None)
+        return V, E[:m]
+    V, E, _ = _walk_py(adj, index, n, False, None)
+    return V, E
 
 
 def adjacency_weights(G, prepared, weight):
-    """Weights aligned with :func:`adjacency_csr`: ``attrs.get(weight, 1.0)``
-    for every adjacency entry, as eigenvector centrality reads them
+    """Weights aligned with :func:`adjacency_csr`, ``attrs.get(weight, 1.0)``
+    per adjacency entry, as eigenvector centrality reads them
     (``eigenvector.py:80-89``)."""
     if _is_bulk_graph(G):
         _, _, W = G.csr_arrays(weight)
         return np.asarray(W, dtype=np.float64)
     adj = _adj_dict(G)
-    vals = []
-    for u in prepared["nodes"]:
-        vals.extend(float(d.get(weight, 1.0)) for d in adj[u].values())
-    return np.asarray(vals, dtype=np.float64)
-
-
-def graph_out_csr(G, prepared, weight=None):
-    """Out-CSR of a protocol graph in the reference's order.
-
-    Rows follow the node order, columns ascend within each row
-    (``graph.cpp:824-886``, ``:719-822``).  Directed graphs keep self-loops
-    (``graph_convert.cpp:110-134``); undirected graphs drop self-loops and
-    store each pair in both directions (``graph_convert.cpp:72-108``).
-    Missing weight attributes count as 1.0."""
-    if _is_bulk_graph(G):
-        return G.csr_arrays(weight)
-    idx = prepared["node_to_idx"]
     n = len(prepared["nodes"])
-    adj = _adj_dict(G)
-    src, dst, wts = [], [], []
-    weighted = weight is not None
-    if prepared["directed"] and not weighted and _identity_labels(prepared) and list(adj) == prepared["nodes"]:
-        # nodes 0..n-1 in adjacency order: rows and columns come straight from
-        # the dicts at C speed (map / chain), no per-arc Python work
-        counts = np.fromiter(map(len, adj.values()), dtype=np.int64, count=n)
-        d = np.fromiter(itertools.chain.from_iterable(adj.values()), dtype=np.int64, count=int(counts.sum()))
-        return _finish_out_csr(np.repeat(np.arange(n, dtype=np.int64), counts), d, None, n)
-    if prepared["directed"]:
-        # whole rows at a time: one extend per node instead of one append per arc;
-        # nodes 0..n-1 in order are their own indices, so no per-arc lookup
-        rows, counts = [], []
-        get = idx.__getitem__
-        identity = _identity_labels(prepared)
-        for u, nbrs in adj.items():
-            if not nbrs:
-                continue
-            rows.append(u if identity else idx[u])
-            counts.append(len(nbrs))
-            dst.extend(nbrs if identity else map(get, nbrs))
-            if weighted:
-                wts.extend(float(d.get(weight, 1.0)) for d in nbrs.values())
-        src = np.repeat(np.asarray(rows, dtype=np.int64), np.asarray(counts, dtype=np.int64))
-    else:
-        seen = set()
-        for u, nbrs in adj.items():
-            iu = idx[u]
-            for v, data in nbrs.items():
-                iv = idx[v]
-                if iu == iv:
-                    continue
-                key = (iu, iv) if iu < iv else (iv, iu)
-                if key in seen:
-                    continue
-                seen.add(key)
-                src.append(iu)
-                dst.append(iv)
-                src.append(iv)
-                dst.append(iu)
-                if weighted:
-                    w = float(data.get(weight, 1.0))
-                    wts.append(w)
-                    wts.append(w)
-    s = np.asarray(src, dtype=np.int64)
-    d = np.fromiter(dst, dtype=np.int64, count=len(dst))
-    W = np.fromiter(wts, dtype=np.float64, count=len(wts)) if weighted else None
-    return _finish_out_csr(s, d, W, n)
+    index = None if _identity_labels(prepared) else prepared["node_to_idx"]
+    if _host is not None and type(adj) is dict:
+        cap = sum(map(len, adj.values()))
+        V = np.empty(n + 1, dtype=np.int32)
+        E = np.empty(max(cap, 1), dtype=np.int32)
+        W = np.empty(max(cap, 1), dtype=np.float64)
+        m = _host.arcs(adj, index, n, False, V, E, weight, W)
+        return W[:m]
+    return _walk_py(adj, index, n, False, weight)[2]
 
 
-def _finish_out_csr(s, d, W, n):
-    """(V, E, W) from arc rows s and columns d: rows ascending, columns
-    ascending within a row.  Usually already so (adjacency in node order,
-    neighbours inserted in order); else one stable sort on the (row, column)
-    key."""
-    if len(d) > 1 and not bool(np.all((s[1:] > s[:-1]) | ((s[1:] == s[:-1]) & (d[1:] > d[:-1])))):
-        order = np.argsort(s * max(n, 1) + d, kind="stable")
-        s = s[order]
-        d = d[order]
-        if W is not None:
-            W = W[order]
-    E = d.astype(np.int32)
-    V = np.zeros(n + 1, dtype=np.int64)
-    if n:
-        V[1:] = np.cumsum(np.bincount(s, minlength=n))
-    if V[-1] >= 2**31 - 1:
-        raise ValueError("adjacency entry count exceeds the signed 32-bit EGGPU CSR limit")
-    return V.astype(np.int32), E, W
-
-
+# ---------------------------------------------------------------------------
+# device graph and the native call
+# ---------------------------------------------------------------------------
 def _get_cached_cpp_graph(G, prepared=None, weight=None, alpha=0.85):
-    """Device-resident graph for (G, weight), rebuilt when the context changes."""
+    """The device-resident graph of (G, weight), built when the context is
+    new and kept beside it in G.cache."""
     if prepared is None:
         prepared = _prepare_graph_cache(G)
-    cache = None if _CPP_GRAPH_CACHE_DISABLED else _graph_cache_dict(G)
-    key = f"{_DEVICE_GRAPH_KEY}:{'' if weight is None else str(weight)}"
+    cache = None if _DEVICE_GRAPH_CACHE_OFF else _graph_cache_dict(G)
+    slot = f"{_DEVICE_KEY}:{'' if weight is None else weight}"
     if cache is not None:
-        entry = cache.get(key)
-        if entry is not None and entry[0] == prepared.get("ctx_id"):
-            return entry[1]
+        held = cache.get(slot)
+        if held is not None and held[0] == prepared.get("ctx_id"):
+            return held[1]
     if _is_bulk_graph(G) and hasattr(G, "device_graph"):
-        dg = G.device_graph(weight, alpha=alpha)  # artifact files streamed straight into HBM
+        dg = G.device_graph(weight, alpha=alpha)  # artifact files streamed into HBM
     else:
-        V, E, W = graph_out_csr(G, prepared, weight)
+        # weighted: gen_CSR column order, which the out-weight sums follow
+        V, E, W = _out_arcs(G, prepared, weight, sort=weight is not None)
         dg = _native().DeviceGraph.from_csr(V, E, W, alpha=alpha)
     if cache is not None:
-        cache[key] = (prepared.get("ctx_id"), dg)
+        cache[slot] = (prepared.get("ctx_id"), dg)
     return dg
 
 
 def _cpp_gpu_pagerank(prepared, G, alpha, max_iter, eps, weight):
-    """Native call + dense wrap (``gpu_eggpu_backend.py:636-704``)."""
+    """The native call and its result (``gpu_eggpu_backend.py:636-704``):
+    ``(mapping, kernel_seconds)``, or None when the native side declines.
+    With the dict switch on, the dict-returning native function is tried
+    first; an empty answer for a non-empty graph without kernel time counts
+    as declined and the dense function is used."""
     graph = _get_cached_cpp_graph(G, prepared=prepared, weight=weight, alpha=alpha)
     if graph is None:
         return None
+    nat = _native()
     nodes = prepared.get("nodes", [])
-    out = _native().cpp_gpu_pagerank_dense(graph, alpha=float(alpha), max_iterator=int(max_iter),
-                                           threshold=float(eps), weight=weight)
+    kw = dict(alpha=float(alpha), max_iterator=int(max_iter), threshold=float(eps), weight=weight)
+    if _PR_PREFER_DICT and hasattr(nat, "cpp_gpu_pagerank"):
+        out = nat.cpp_gpu_pagerank(graph, nodes=nodes, **kw)
+        values = out.get("values") if isinstance(out, dict) else None
+        ks = out.get("kernel_seconds") if isinstance(out, dict) else None
+        declined = len(nodes) > 0 and isinstance(values, dict) and not values and not ks
+        if isinstance(values, dict) and not declined:
+            _record(out)
+            return values, (float(ks) if ks is not None else None)
+    out = nat.cpp_gpu_pagerank_dense(graph, **kw)
     if not isinstance(out, dict) or out.get("values_dense") is None:
         return None
-    vals = _normalize_dense_array(out["values_dense"], len(nodes), dtype="float64",
-                                  allow_leading_sentinel=True, result_name="PageRank values_dense")
-    values = _DenseValueDict(nodes, vals, node_to_idx=prepared.get("node_to_idx"), dtype=float)
+    vals = _normalize_dense_array(out["values_dense"], len(nodes), dtype="float64", allow_leading_sentinel=True,
+                                  result_name="PageRank values_dense")
+    _record(out)
     ks = out.get("kernel_seconds")
-    it = out.get("iterations")
-    if it is not None:
-        _LAST_ITERATIONS["pagerank"] = int(it)
-    return values, (float(ks) if ks is not None else None)
+    index = prepared.get("node_to_idx")
+    keys = range(len(nodes)) if index is None and prepared.get("identity_labels") else nodes
+    return (_DenseValueDict(keys, vals, node_to_idx=index, dtype=float), float(ks) if ks is not None else None)
+
+
+def _record(out):
+    if out.get("iterations") is not None:
+        _last_solve.setdefault("pagerank", {})["iterations"] = int(out["iterations"])
 
 
 def pagerank(G, alpha=0.85, max_iter=200, eps=1e-6, weight=None):
-    """GPU PageRank (``gpu_eggpu_backend.py:1782-1821``)."""
+    """GPU PageRank (``gpu_eggpu_backend.py:1782-1821``): a read-only
+    mapping node -> rank."""
     prepared = _graph_context(G)
     if prepared.get("bulk_csr") and weight is not None and not _bulk_weight_available(G, weight):
         raise NotImplementedError(
             f"weighted PageRank requires an EGGPU CSR artifact containing the {weight!r} edge attribute"
         )
-    key = _cache_key("pagerank", bool(prepared["directed"]), alpha, int(max_iter), eps, str(weight))
-    cached = _result_cache_get(prepared, key)
-    if cached is not None:
-        result, ks = cached
-        set_last_kernel_time("pagerank", ks)
-        return result
-    nodes = prepared["nodes"]
-    if not len(nodes):
+    if not prepared["node_count"]:
         set_last_kernel_time("pagerank", 0.0)
         return {}
-    hit = _cpp_gpu_pagerank(prepared, G, alpha=alpha, max_iter=max_iter, eps=eps, weight=weight)
-    if hit is None:
+    key = ("pagerank", bool(prepared["directed"]), float(alpha), int(max_iter), float(eps), str(weight))
+    hit = _result_cache_get(prepared, key)
+    if hit is not None:
+        set_last_kernel_time("pagerank", hit[1])
+        return hit[0]
+    solved = _cpp_gpu_pagerank(prepared, G, alpha=alpha, max_iter=max_iter, eps=eps, weight=weight)
+    if solved is None:
         raise RuntimeError("native HIP PageRank backend unavailable")
-    result, ks = hit
+    result, ks = solved
     set_last_kernel_time("pagerank", ks)
     _result_cache_put(prepared, key, result, ks)
     return result

@@ -739,3 +739,15 @@ def cpp_gpu_pagerank_dense(graph, alpha=0.85, max_iterator=200, threshold=1e-6, 
     accepted for signature compatibility (weights were fixed at build)."""
     values, iters, ks = graph.pagerank(alpha=alpha, max_iter=max_iterator, tol=threshold)
     return {"values_dense": values, "kernel_seconds": ks, "iterations": iters}
+
+
+def cpp_gpu_pagerank(graph, alpha=0.85, max_iterator=200, threshold=1e-6, weight=None, nodes=None):
+    """Shim with the keys of the dict-returning ``cpp_gpu_pagerank``
+    (``pagerank.cpp:124-174``): ``{"values": {node: rank}, "kernel_seconds":
+    s}``.  Node i of the device graph is ``nodes[i]`` (the backend passes its
+    node list; without it the keys are the indices)."""
+    values, iters, ks = graph.pagerank(alpha=alpha, max_iter=max_iterator, tol=threshold)
+    keys = range(len(values)) if nodes is None else nodes
+    if len(keys) != len(values):
+        raise ValueError(f"cpp_gpu_pagerank: {len(keys)} nodes for {len(values)} ranks")
+    return {"values": dict(zip(keys, values.tolist())), "kernel_seconds": ks, "iterations": iters}

@@ -8,9 +8,18 @@ function registered with :func:`set_cpu_pagerank`, else EasyGraph's own
 ``_pagerank_power_iteration`` when EasyGraph is importable.  With neither,
 the call raises instead of computing anything on the CPU here.  A missing
 native library always raises, in strict and non-strict mode alike.
+
+C++ graphs (``G.cflag`` set: EasyGraph's ``GraphC`` / ``DiGraphC``) take the
+reference's ``@hybrid("cpp_pagerank")`` route before any of this
+(``pagerank.py:13``, ``decorators.py:140-163``): the call goes, with the
+caller's own arguments, to ``cpp_easygraph.cpp_pagerank`` (or the function
+registered with :func:`set_cpp_pagerank`), ``max_iter`` and ``tol`` renamed
+``max_iterator`` and ``threshold``.  Without such a function the call stays
+on the route above.
 """
 from __future__ import annotations
 
+import functools
 import os
 
 from . import gpu_backend as eggpu_backend
@@ -18,9 +27,47 @@ from .exception import EasyGraphNotImplemented
 from .gpu_runtime import gpu_runtime_enabled, gpu_strict_errors
 from .native import NativeLibraryMissing
 
-__all__ = ["pagerank", "set_cpu_pagerank"]
+__all__ = ["pagerank", "set_cpu_pagerank", "set_cpp_pagerank"]
 
 _CPU_PAGERANK = None
+_CPP_PAGERANK = None
+
+
+def set_cpp_pagerank(fn):
+    """Register the C++-graph route ``fn(G, *args, **kwargs)``, called as
+    ``cpp_easygraph.cpp_pagerank`` would be (None: look that up again)."""
+    global _CPP_PAGERANK
+    _CPP_PAGERANK = fn
+
+
+def _cpp_graph_route():
+    if _CPP_PAGERANK is not None:
+        return _CPP_PAGERANK
+    try:
+        import cpp_easygraph  # type: ignore
+
+        return getattr(cpp_easygraph, "cpp_pagerank", None)
+    except ImportError:
+        return None
+
+
+def _cpp_graphs_to_cpp(fn):
+    """The @hybrid("cpp_pagerank") rule: a graph with cflag set is handed,
+    with the caller's arguments as given, to the C++ route."""
+
+    @functools.wraps(fn)
+    def entry(*args, **kwargs):
+        G = args[0] if args else kwargs.get("G")
+        if G is not None and getattr(G, "cflag", 0):
+            route = _cpp_graph_route()
+            if route is not None:
+                renamed = {("max_iterator" if k == "max_iter" and "max_iterator" not in kwargs else
+                            "threshold" if k == "tol" and "threshold" not in kwargs else k): v
+                           for k, v in kwargs.items()}
+                return route(*args, **renamed)
+        return fn(*args, **kwargs)
+
+    return entry
 
 
 def set_cpu_pagerank(fn):
@@ -41,6 +88,20 @@ def _cpu_route():
         return None
 
 
+def _not_for_multigraphs(fn):
+    # @not_implemented_for("multigraph") (pagerank.py:12), outside the C++ route
+    @functools.wraps(fn)
+    def entry(*args, **kwargs):
+        G = args[0] if args else kwargs.get("G")
+        if G is not None and G.is_multigraph():
+            raise EasyGraphNotImplemented("not implemented for multigraph type")
+        return fn(*args, **kwargs)
+
+    return entry
+
+
+@_not_for_multigraphs
+@_cpp_graphs_to_cpp
 def pagerank(G, alpha=0.85, weight=None, max_iter=None, tol=None):
     """Returns the PageRank value of each node in G.
 
@@ -48,8 +109,6 @@ def pagerank(G, alpha=0.85, weight=None, max_iter=None, tol=None):
     ``EASYGRAPH_CPU_PR_MAX_ITER`` (200) and ``tol`` to ``EASYGRAPH_CPU_PR_TOL``
     (1e-6); the iteration stops once the L1 change is below ``tol * n``.
     Returns a read-only ``Mapping`` node -> score."""
-    if G.is_multigraph():  # @not_implemented_for("multigraph"), pagerank.py:12
-        raise EasyGraphNotImplemented("not implemented for multigraph type")
     if max_iter is None:
         max_iter_eff = max(1, int(os.environ.get("EASYGRAPH_CPU_PR_MAX_ITER", "200")))
     else:

@@ -103,6 +103,35 @@ def test_csc_build_bit_exact(gpu_ready, name, make, alpha):
     assert np.array_equal(dang, r_dang)
 
 
+@pytest.mark.parametrize("directed", [True, False])
+@pytest.mark.parametrize("weight", [None, "weight"])
+def test_dict_graph_device_csc_bit_exact(gpu_ready, directed, weight):
+    # the backend hands unweighted graphs over in adjacency order (no per-row
+    # column sort): the pull CSC must still be the one built from the
+    # gen_CSR-ordered out-CSR, bit for bit; weighted graphs go over sorted
+    rng = np.random.default_rng(21)
+    n = 3000
+    nodes = list(range(n))
+    rng.shuffle(nodes)
+    G = DiGraph() if directed else Graph()
+    G.add_nodes_from(nodes)
+    for _ in range(12 * n):
+        G.add_edge(nodes[int(rng.integers(n))], nodes[int(rng.integers(n))], weight=float(rng.uniform(0.1, 2)))
+    prep = gpu_backend._prepare_graph_cache(G)
+    Vu, Eu, _ = gpu_backend._out_arcs(G, prep, None, sort=False)
+    Vs, Es, _ = gpu_backend.graph_out_csr(G, prep, None)
+    assert not np.array_equal(Eu, Es)  # the case: rows really are out of order
+    _say(f"dict graph directed={directed} weight={weight} n={n} m={len(Es)}")
+    g = gpu_backend._get_cached_cpp_graph(G, prepared=prep, weight=weight, alpha=0.85)
+    in_row, in_col, in_val, dang = g.download_csc()
+    V, E, W = gpu_backend.graph_out_csr(G, prep, weight)
+    r_row, r_col, r_val, r_dang, _ = O.build_inbound(V, E, W, 0.85)
+    assert np.array_equal(in_row, r_row)
+    assert np.array_equal(in_col, r_col)
+    assert np.array_equal(in_val.view(np.uint32), r_val.view(np.uint32))
+    assert np.array_equal(dang, r_dang)
+
+
 def test_alpha_change_refreshes_values_bit_exact(gpu_ready):
     V, E = O.rmat_csr(10, 16, seed=2)
     W = O.edge_weights(V, E, 4)
