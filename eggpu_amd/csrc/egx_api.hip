@@ -90,11 +90,19 @@ int graph_from_host_csr(const int32_t* V, const int32_t* E, const double* W, int
     if (n > 0) EGX_HIP_TRY(hipMemcpy(dV.p, V, ((size_t)n + 1) * 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
     else EGX_HIP_TRY(hipMemset(dV.p, 0, 4), EGX_DEVICE_ERR);
     if (m > 0) EGX_HIP_TRY(hipMemcpy(dE.p, E, (size_t)m * 4, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
-    if (W) {
+    // weights that are all exactly 1 (the reference's unweighted_W, gen_CSR
+    // graph.cpp:881) give the unweighted graph: the same pull CSC bit for bit
+    // (in_val = fl32(alpha * (1/outdeg)) either way), iterated by the
+    // unweighted kernels, which read no per-arc value
+    bool unit = W != nullptr;
+    for (int32_t i = 0; unit && i < m; ++i) unit = W[i] == 1.0;
+    const bool upload_w = W && (!unit || keep_csr);
+    if (upload_w) {
         EGX_TRY(dW.alloc((size_t)std::max(m, 1) * 8));
         if (m > 0) EGX_HIP_TRY(hipMemcpy(dW.p, W, (size_t)m * 8, hipMemcpyHostToDevice), EGX_DEVICE_ERR);
     }
-    EGX_TRY(build_from_device_csr(g, dV.as<int32_t>(), dE.as<int32_t>(), W ? dW.as<double>() : nullptr, n, m, alpha));
+    EGX_TRY(build_from_device_csr(g, dV.as<int32_t>(), dE.as<int32_t>(), (W && !unit) ? dW.as<double>() : nullptr,
+                                  n, m, alpha));
     if (keep_csr) {
         g.csr_V = std::move(dV);
         g.csr_E = std::move(dE);

@@ -49,7 +49,9 @@ int egx_device_count(void);
  * and the C++ wrapper gpu_easygraph::pagerank (Easy-Graph/gpu_easygraph/gpu_easygraph.h:95-104).
  *
  * V[len_V+1], E[len_E]: out-CSR of the graph (Graph::gen_CSR order).  W[len_E]
- * edge weights or NULL for unit weights.  len_V is the node count (the
+ * edge weights or NULL for unit weights; weights that are all exactly 1 (the
+ * reference's unweighted_W) are taken as NULL, which gives the same pull CSC
+ * and runs the unweighted kernels.  len_V is the node count (the
  * reference wrapper passes V.size()-1).  graph_id plays the part of
  * active_device_graph_id() (device_graph_cache.h:34-48): the prepared device
  * graph is cached per host thread and reused while graph_id, the V/E/W
@@ -145,8 +147,9 @@ typedef struct {
     int32_t check_every;    /* 0: adaptive host checks of the device stop flag */
 } egx_solve_opts;
 
-/* Build from a host out-CSR (same contract as egx_pagerank). keep_csr keeps
- * the out-CSR on the device so egx_graph_download_csr can return it. */
+/* Build from a host out-CSR (same contract as egx_pagerank, all-ones weights
+ * included). keep_csr keeps the out-CSR on the device so
+ * egx_graph_download_csr can return it. */
 int egx_graph_from_csr(const int32_t *V, const int32_t *E, const double *W, int32_t n,
                        int32_t m, double alpha, int32_t keep_csr, egx_graph **out);
 

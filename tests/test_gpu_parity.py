@@ -132,6 +132,32 @@ def test_dict_graph_device_csc_bit_exact(gpu_ready, directed, weight):
     assert np.array_equal(dang, r_dang)
 
 
+def test_all_ones_weights_take_the_unweighted_path(gpu_ready):
+    # the reference's C++ route hands unweighted_W (m ones, graph.cpp:881) as W
+    V, E = O.rmat_csr(13, 16, seed=12)
+    ones = np.ones(len(E))
+    g1 = native.DeviceGraph.from_csr(V, E, ones)
+    g0 = native.DeviceGraph.from_csr(V, E, None)
+    assert g1.info()["weighted"] == 0
+    in1, in0 = g1.download_csc(), g0.download_csc()
+    r = O.build_inbound(V, E, ones, 0.85)
+    for got in (in1, in0):
+        assert np.array_equal(got[0], r[0]) and np.array_equal(got[1], r[1]) and np.array_equal(got[3], r[3])
+        assert np.array_equal(got[2].view(np.uint32), r[2].view(np.uint32))
+    p1, i1, _ = g1.pagerank(0.85, 100, 1e-6)
+    p0, i0, _ = g0.pagerank(0.85, 100, 1e-6)
+    assert i1 == i0 and np.array_equal(p1.view(np.uint64), p0.view(np.uint64))
+    # one weight off 1: weighted
+    w = ones.copy()
+    w[7] = 0.5
+    assert native.DeviceGraph.from_csr(V, E, w).info()["weighted"] == 1
+    # through the raw C++-route entry as well
+    out, it, _ = native.pagerank_csr(V, E, ones, 0.85, 100, 1e-6, graph_id=77)
+    assert it == i0 and np.array_equal(out.view(np.uint64), p0.view(np.uint64))
+    g1.close()
+    g0.close()
+
+
 def test_alpha_change_refreshes_values_bit_exact(gpu_ready):
     V, E = O.rmat_csr(10, 16, seed=2)
     W = O.edge_weights(V, E, 4)

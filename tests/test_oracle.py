@@ -535,3 +535,48 @@ def test_isolated_loops_host_rewrite():
     Vc = np.array([0, 1, 2], dtype=np.int64)
     Ec = np.array([1, 0], dtype=np.int32)
     assert with_isolated_loops(Vc, Ec, None)[2] is None
+
+
+@pytest.mark.parametrize("alpha", [0.85, 0.7, 0.95])
+def test_build_inbound_values_pinned_bit_for_bit(alpha):
+    """in_val = fl32(alpha * (w / outW)) (pagerank.cu:344), outW the sum of
+    the source's sanitised weights in CSR order (pagerank.cu:308-316).
+
+    Every source here has two arcs, weights w and 1.0, with w chosen so that
+    alpha * (w / (w + 1)) lands within a few double ulps of a midpoint
+    between two FP32 numbers: there the FP32 bits depend on the order of the
+    double operations.  The expected value is formed from Python's IEEE
+    doubles in the reference's order and rounded to FP32; the arcs are kept
+    where (alpha * w) / outW, alpha * w * (1 / outW) or an FP32 division
+    give other bits, so an oracle with any of those orders fails."""
+    rng = np.random.default_rng(int(alpha * 100))
+    ws, seps = [], [0, 0, 0]
+    while len(ws) < 200:
+        f = np.float32(rng.uniform(0.05, 0.6) * alpha)
+        mid = (float(f) + float(np.nextafter(f, np.float32(1.0)))) / 2.0  # an FP32 rounding midpoint
+        w0 = mid / (alpha - mid)  # alpha * w / (w + 1) = mid
+        for k in range(-40, 41):
+            w = float(np.nextafter(w0, np.inf) if k > 0 else w0)
+            for _ in range(abs(k) - 1):
+                w = float(np.nextafter(w, np.inf if k > 0 else -np.inf))
+            s_ = w + 1.0
+            v = np.float32(alpha * (w / s_))
+            alts = (np.float32((alpha * w) / s_), np.float32(alpha * w * (1.0 / s_)),
+                    np.float32(np.float32(alpha) * (np.float32(w) / np.float32(s_))))
+            if any(a != v for a in alts):
+                for j, a in enumerate(alts):
+                    seps[j] += int(a != v)
+                ws.append((w, v))
+                break
+    assert min(seps) >= 10, seps  # each alternative order is caught on some arcs
+    n = len(ws) + 2
+    V = np.arange(0, 2 * len(ws) + 1, 2, dtype=np.int32)
+    V = np.concatenate([V, [V[-1], V[-1]]]).astype(np.int32)  # two sinks
+    E = np.tile(np.array([n - 2, n - 1], np.int32), len(ws))
+    W = np.empty(2 * len(ws))
+    W[0::2] = [w for w, _ in ws]
+    W[1::2] = 1.0
+    in_row, in_col, in_val, dang, outw = O.build_inbound(V, E, W, alpha=alpha)
+    row = in_val[in_row[n - 2]:in_row[n - 1]]  # the w arcs, sources ascending
+    assert np.array_equal(in_col[in_row[n - 2]:in_row[n - 1]], np.arange(len(ws)))
+    assert np.array_equal(row.view(np.uint32), np.array([v for _, v in ws], np.float32).view(np.uint32))
