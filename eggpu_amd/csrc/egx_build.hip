@@ -839,7 +839,8 @@ int build_from_device_csr(Graph& g, const int32_t* dV, const int32_t* dE, const 
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     g.m = m_valid;
 
-    EGX_TRY(g.in_col.alloc((size_t)std::max(m_valid, 1) * 4));
+    EGX_TRY(g.in_col.alloc(((size_t)std::max(m_valid, 1) + kArcPad) * 4));
+    EGX_HIP_TRY(hipMemsetAsync(g.in_col.as<int32_t>() + m_valid, 0, kArcPad * 4, s), EGX_DEVICE_ERR);
     if (g.weighted) {
         EGX_TRY(g.in_val.alloc((size_t)std::max(m_valid, 1) * 4));
         EGX_TRY(g.w_csc.alloc((size_t)std::max(m_valid, 1) * 8));
@@ -955,7 +956,8 @@ int build_layout(Graph& g, const BinCfg& cfg) {
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     cnt.release();
 
-    EGX_TRY(g.p_col.alloc((size_t)std::max<int64_t>(m, 1) * 4));
+    EGX_TRY(g.p_col.alloc(((size_t)std::max<int64_t>(m, 1) + kArcPad) * 4));
+    EGX_HIP_TRY(hipMemsetAsync(g.p_col.as<int32_t>() + m, 0, kArcPad * 4, s), EGX_DEVICE_ERR);
     if (g.weighted) {
         EGX_TRY(g.p_val.alloc((size_t)std::max<int64_t>(m, 1) * 4));
         EGX_TRY(g.p_w.alloc((size_t)std::max<int64_t>(m, 1) * 8));

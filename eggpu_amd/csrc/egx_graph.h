@@ -76,6 +76,11 @@ struct PrState {
 // threshold must not exceed it.
 constexpr int32_t kTileLds = 7936;
 
+// Zeroed slots past the m arcs of the iteration arc arrays (in_col, p_col,
+// p_nval): the width-64 SpMM reads arc indices and weights 8 at a time in
+// scalar loads, up to two steps past a wavefront's last arc.
+constexpr int64_t kArcPad = 64;
+
 struct BinCfg {
     int32_t thread_row_max = 12;  // tiles with mean row length <= this: thread per row, else wavefront per row
     int32_t wave_row_max = 2048;  // hub threshold: rows with more arcs go to the hub bin

@@ -15,7 +15,7 @@
 // one contiguous range of rows as a single stream, arc indices loaded 64 at a
 // time and broadcast with shuffles, the next step's row loads issued before
 // the current step is summed.  Row sums, c_j and every reduction are FP64.
-// Rows are split over wavefronts by a merge-path split of (arcs + 2*rows);
+// Rows are split over wavefronts by a merge-path split of (arcs + 8*rows);
 // hub rows (the prefix of the iteration layout) are cut into segments whose
 // FP64 partials the reduce launch adds in segment order.  A column whose stop flag is set is copied through
 // unchanged, so its values do not depend on which other columns share the
@@ -41,6 +41,11 @@ constexpr int kRowThreads = 256;     // 4 wavefronts
 constexpr int kRowWaves = kRowThreads / kWave;
 constexpr int kSpmmHubSeg = 2048;    // arcs per hub segment (one workgroup)
 constexpr int kMinWidth = 8;         // narrowest X row (columns)
+// merge-path weight of a row against one arc: finishing a row (class tree,
+// new value, three column sums, the store) costs several times an arc's
+// gather and fused multiply-add, and half the rows of an R-MAT graph have
+// no arcs, so a weight of 2 left the row-heavy wavefronts as the long pole
+constexpr int kRowItems = 8;
 constexpr size_t kArenaBytes = (size_t)1 << 20;  // pinned staging for queued batch updates
 
 struct ColState {
@@ -89,6 +94,7 @@ struct SpmmArgs {
     uint32_t grid;
     uint32_t n_waves;
     uint32_t n_red;            // reduce-part workgroups
+    int32_t off32;             // width 64: X rows and arc slots addressable by 32-bit byte offsets
 };
 
 // Width-C launches: a wavefront holds G = 64/C lane groups of C columns;
@@ -122,6 +128,13 @@ __device__ __forceinline__ RowIn row_inputs(const SpmmArgs& a, int32_t v) {
     return r;
 }
 
+// New value of one column of one row from its arc sum and teleport weight:
+// one definition (a product, then one fused multiply-add) shared by every
+// width and path, so the rounding is the same everywhere.
+__device__ __forceinline__ float next_value(double alpha, double acc, double cj, double tele) {
+    return (float)__builtin_fma(alpha, acc, cj * tele);
+}
+
 // Column c (= lane % C) of row v: new value, and this row's contributions to
 // the column's L1 change, sum and dangling mass.
 template <int C>
@@ -137,7 +150,7 @@ __device__ __forceinline__ void finalize(const SpmmArgs& a, const ColState& cs, 
     double tele;
     if (cs.uniform) tele = inv_n;
     else tele = (in.ts >= 0) ? (double)a.ptab[(size_t)in.ts * kCols + c] : 0.0;
-    const float xn = (float)(cs.alpha * acc + cj * tele);
+    const float xn = next_value(cs.alpha, acc, cj, tele);
     e = fabs((double)xn - (double)in.xo);
     su = (double)xn;
     if (in.deg == 0.0f) dg = (double)xn;
@@ -184,53 +197,56 @@ __device__ __forceinline__ double across_groups(double v) {
 // call on_end(k, sum) for each segment in order, sum being its canonical
 // class-tree sum.  The arc indices of C arcs at a time come in one load (the
 // next C are prefetched), and the gathers of batch i+1 are issued before
-// batch i is consumed: 16 row loads in flight per group.  Within a batch,
+// batch i is consumed, into the other of two register sets (no copy of a
+// load in flight, so nothing waits on the batch just issued).  Chunk slots
+// outside [A0, A1) hold a clamped arc, so loads and gathers are
+// unconditional; no segment's bounds take such a slot in.  Within a batch,
 // arcs [lo, hi) belong to the current segment.
 template <int C, class SegEnd, class OnEnd>
 __device__ __forceinline__ void group_stream(const SpmmArgs& a, int32_t A0, int32_t A1, int32_t nseg, SegEnd&& seg_end,
                                              OnEnd&& on_end) {
     if (nseg <= 0) return;
+    if constexpr (C == kWave) {
+        A0 = __builtin_amdgcn_readfirstlane(A0);
+        A1 = __builtin_amdgcn_readfirstlane(A1);
+        nseg = __builtin_amdgcn_readfirstlane(nseg);
+    }
     const int lane = lane_id();
     const int c = lane % C;
     const int gb = lane - c;
     const int32_t P0 = A0 & ~7;
-    // C arcs [cb, cb+C): lane c of the group holds arc cb+c
+    // C arcs [cb, cb+C): lane c of the group holds arc cb+c (col and nval hold at least one slot)
     auto load_chunk = [&](int32_t cb, int32_t& u_l, float& w_l) {
-        const int32_t p = cb + c;
-        const bool ok = p >= A0 && p < A1;
-        u_l = ok ? __builtin_nontemporal_load(&a.col[p]) : 0;
-        w_l = ok ? __builtin_nontemporal_load(&a.nval[p]) : 0.0f;
+        const int32_t p = max(min(max(cb + c, A0), A1 - 1), 0);
+        u_l = __builtin_nontemporal_load(&a.col[p]);
+        w_l = __builtin_nontemporal_load(&a.nval[p]);
     };
-    // the 8 gathers of batch [p0, p0+8), whose arcs sit at lanes k..k+7 of the group's chunk
-    auto issue = [&](int32_t p0, int32_t k, int32_t u_l, float w_l, float (&g)[8], float (&w)[8]) {
+    // the 8 gathers of a batch whose arcs sit at lanes k..k+7 of the group's chunk
+    auto issue = [&](int32_t k, int32_t u_l, float w_l, float (&g)[8], float (&w)[8]) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             int32_t u;
-            float wj;
             if constexpr (C == kWave) {
                 u = __builtin_amdgcn_readlane(u_l, k + j);
-                wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), k + j));
+                w[j] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), k + j));
             } else {
                 u = __shfl(u_l, gb + k + j, kWave);
-                wj = __shfl(w_l, gb + k + j, kWave);
+                w[j] = __shfl(w_l, gb + k + j, kWave);
             }
-            const int32_t p = p0 + j;
-            const bool ok = p >= A0 && p < A1;
-            g[j] = ok ? a.x_in[(size_t)u * C + c] : 0.0f;
-            w[j] = ok ? wj : 0.0f;
+            g[j] = a.x_in[(size_t)u * C + c];
         }
     };
     int32_t cu, cn;
     float cw, cwn;
     load_chunk(P0, cu, cw);
     load_chunk(P0 + C, cn, cwn);
-    float g[8], w[8];
-    issue(P0, 0, cu, cw, g, w);
+    float gA[8], wA[8], gB[8], wB[8];
+    issue(0, cu, cw, gA, wA);
     int32_t k = 0;
     int32_t kend = seg_end(0);
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int32_t p0 = P0; k < nseg; p0 += 8) {
-        // next batch first, so its loads overlap this batch's use
+    // issue batch p0+8 into (gn, wn), then consume batch p0 from (gc, wc); false after the last segment
+    auto batch = [&](int32_t p0, const float (&gc)[8], const float (&wc)[8], float (&gn)[8], float (&wn)[8]) {
         const int32_t p1 = p0 + 8;
         const int32_t k1 = (p1 - P0) & (C - 1);
         if (k1 == 0) {
@@ -238,30 +254,25 @@ __device__ __forceinline__ void group_stream(const SpmmArgs& a, int32_t A0, int3
             cw = cwn;
             load_chunk(p1 + C, cn, cwn);
         }
-        float g2[8], w2[8];
-        issue(p1, k1, cu, cw, g2, w2);
-        double t[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) t[j] = (double)w[j] * (double)g[j];
+        issue(k1, cu, cw, gn, wn);
         int32_t lo = max(A0, p0) - p0;
         for (;;) {
             const int32_t hi = min(kend, p0 + 8) - p0;
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                if (j >= lo && j < hi) acc[j] += t[j];
-            if (kend >= p0 + 8) break;  // the segment goes on past this batch
+                if (j >= lo && j < hi) acc[j] += (double)wc[j] * (double)gc[j];
+            if (kend >= p0 + 8) return true;  // the segment goes on past this batch
             on_end(k, class_tree(acc));
 #pragma unroll
             for (int j = 0; j < 8; ++j) acc[j] = 0.0;
             lo = max(lo, hi);
-            if (++k >= nseg) break;
+            if (++k >= nseg) return false;
             kend = seg_end(k);
         }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            g[j] = g2[j];
-            w[j] = w2[j];
-        }
+    };
+    for (int32_t p0 = P0;; p0 += 16) {
+        if (!batch(p0, gA, wA, gB, wB)) break;
+        if (!batch(p0 + 8, gB, wB, gA, wA)) break;
     }
 }
 
@@ -284,77 +295,83 @@ __device__ __forceinline__ double hub_quarter(const SpmmArgs& a, int32_t q0, int
     return across_groups<C>(f.result(L));
 }
 
-// Rows [r0, r1) of one wavefront, as one uniform stream over their arcs.
-// An instruction gathers G = 64/C consecutive arcs x C columns (lane =
-// group*C + column, group = arc), so every row end is wavefront-uniform.
-// Steps of S arcs (aligned to 8) keep 16 row loads in flight per lane (8 at
-// C = 8); arc p of a step lands in class p mod 8, held by its group in
-// accumulator slot (p mod 8) / G.  A row that ends inside a step is
-// finished (classes combined across groups, then across slots, in the fixed
-// tree) and the rest of the step goes to the next row.  Row ends, degrees
-// and slots come 64 rows per load, old values G rows per load.  Row
-// contributions are folded into the column sums in row order.
+// Rows [r0, r1) of one wavefront, as one uniform stream over their arcs
+// (widths below 64; wave_stream64 is the width-64 form).  An instruction
+// gathers G = 64/C consecutive arcs x C columns (lane = group*C + column,
+// group = arc), so every row end is wavefront-uniform.  Steps of S arcs
+// (aligned to 8) alternate between two register sets: the gathers of step
+// i+1 are issued into one while step i is summed from the other.  Chunk
+// slots outside [A0, A1) hold a clamped arc, so loads and gathers are
+// unconditional; no row's bounds take such a slot in.  Arc p of a step
+// lands in class p mod 8, held by its group in accumulator slot
+// (p mod 8) / G.  A row that ends inside a step is finished (classes
+// combined across groups, then across slots, in the fixed tree) and the
+// rest of the step goes to the next row.  Row ends, degrees and slots come
+// 64 rows per load and old values G rows per load; the next window is
+// loaded when the current one is half done and copied in when it ends, so
+// no copy waits on a load just issued.  Row contributions are folded into
+// the column sums in row order.
 template <int C>
 __device__ __forceinline__ void wave_stream(const SpmmArgs& a, const ColState& cs, double cj, double inv_n,
                                             int32_t r0, int32_t r1, double& err, double& sum, double& dang) {
     constexpr int G = kWave / C;               // arcs per instruction
     constexpr int R = 8 / G;                   // class slots per lane
-    constexpr int S = (16 * G < 64) ? 16 * G : 64;  // arcs per step
+    constexpr int S = 8 * G;                   // arcs per step
     constexpr int NI = S / G;                  // gather instructions per step
+    static_assert(G > 1, "width 64 runs wave_stream64");
+    r0 = __builtin_amdgcn_readfirstlane(r0);
+    r1 = __builtin_amdgcn_readfirstlane(r1);
     if (r0 >= r1) return;
     const int lane = lane_id();
     const int grp = lane / C;
     const int c = lane % C;
-    const int32_t A0 = a.row[r0], A1 = a.row[r1];
+    const int32_t A0 = __builtin_amdgcn_readfirstlane(a.row[r0]);
+    const int32_t A1 = __builtin_amdgcn_readfirstlane(a.row[r1]);
     const int32_t P0 = A0 & ~7;
-    // arc indices, 64 per load: lane l holds arc cb + l
+    // arc indices, 64 per load: lane l holds arc cb + l (col and nval hold at least one slot)
     auto load_chunk = [&](int32_t cb, int32_t& u_l, float& w_l) {
-        const int32_t p = cb + lane;
-        const bool ok = p >= A0 && p < A1;
-        u_l = ok ? __builtin_nontemporal_load(&a.col[p]) : 0;
-        w_l = ok ? __builtin_nontemporal_load(&a.nval[p]) : 0.0f;
+        const int32_t p = max(min(max(cb + lane, A0), A1 - 1), 0);
+        u_l = __builtin_nontemporal_load(&a.col[p]);
+        w_l = __builtin_nontemporal_load(&a.nval[p]);
     };
-    // gathers of step [ps, ps+S), whose arcs sit at chunk lanes k..k+S-1
-    auto issue = [&](int32_t ps, int32_t k, int32_t u_l, float w_l, float (&g)[NI], float (&w)[NI]) {
+    // gathers of a step whose arcs sit at chunk lanes k..k+S-1
+    auto issue = [&](int32_t k, int32_t u_l, float w_l, float (&g)[NI], float (&w)[NI]) {
 #pragma unroll
         for (int i = 0; i < NI; ++i) {
             const int off = i * G + grp;
-            int32_t u;
-            float wj;
-            if constexpr (G == 1) {
-                u = __builtin_amdgcn_readlane(u_l, k + i);
-                wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w_l), k + i));
-            } else {
-                u = __shfl(u_l, k + off, kWave);
-                wj = __shfl(w_l, k + off, kWave);
-            }
-            const int32_t p = ps + off;
-            const bool ok = p >= A0 && p < A1;
-            g[i] = ok ? a.x_in[(size_t)u * C + c] : 0.0f;
-            w[i] = ok ? wj : 0.0f;
+            const int32_t u = __shfl(u_l, k + off, kWave);
+            w[i] = __shfl(w_l, k + off, kWave);
+            g[i] = a.x_in[(size_t)u * C + c];
         }
     };
+    // row ends, degrees and slots of rows [b, b+64); old values of rows [b, b+G)
+    // (clamped loads: lanes past r1 are never read)
+    auto load_rows = [&](int32_t b, int32_t& e, float& d, int32_t& t) {
+        e = a.row[min(b + 1 + lane, r1)];
+        d = a.deg[min(b + lane, r1 - 1)];
+        t = a.tslot[min(b + lane, r1 - 1)];
+    };
+    auto load_old = [&](int32_t b) { return a.x_in[(size_t)min(b + grp, r1 - 1) * C + c]; };
     int32_t cu, cn;
     float cw, cwn;
     load_chunk(P0, cu, cw);
     load_chunk(P0 + 64, cn, cwn);
-    float g[NI], w[NI];
-    issue(P0, 0, cu, cw, g, w);
+    float gA[NI], wA[NI], gB[NI], wB[NI];
+    issue(0, cu, cw, gA, wA);
 
-    // row windows: ends, degrees and slots of rows [wb, wb+64); old values of rows [xb, xb+G)
-    int32_t wb = r0;
-    int32_t end_l = (wb + 1 + lane <= r1) ? a.row[wb + 1 + lane] : A1;
-    float deg_l = (wb + lane < r1) ? a.deg[wb + lane] : 1.0f;
-    int32_t ts_l = (wb + lane < r1) ? a.tslot[wb + lane] : -1;
+    int32_t wb = r0, end_l, end_n = 0, ts_l, ts_n = 0;
+    float deg_l, deg_n = 0.0f;
+    load_rows(wb, end_l, deg_l, ts_l);
     int32_t xb = r0;
-    float xo_l = (xb + grp < r1) ? a.x_in[(size_t)(xb + grp) * C + c] : 0.0f;
+    float xo_l = load_old(xb), xo_n = 0.0f;
     int32_t v = r0;
     int32_t vend = __builtin_amdgcn_readlane(end_l, 0);
     double acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
-    for (int32_t ps = P0; v < r1; ps += S) {
-        // next step first, so its loads overlap this step's use
+    // issue step ps+S into (gn, wn), then sum step ps from (gc, wc) and
+    // finish the rows ending in it; false once row r1-1 is done
+    auto step = [&](int32_t ps, const float (&gc)[NI], const float (&wc)[NI], float (&gn)[NI], float (&wn)[NI]) {
         const int32_t p1 = ps + S;
         const int32_t k1 = (p1 - P0) & 63;
         if (k1 == 0) {
@@ -362,20 +379,16 @@ __device__ __forceinline__ void wave_stream(const SpmmArgs& a, const ColState& c
             cw = cwn;
             load_chunk(p1 + 64, cn, cwn);
         }
-        float g2[NI], w2[NI];
-        issue(p1, k1, cu, cw, g2, w2);
-        double t[NI];
-#pragma unroll
-        for (int i = 0; i < NI; ++i) t[i] = (double)w[i] * (double)g[i];
+        issue(k1, cu, cw, gn, wn);
         int32_t lo = max(A0, ps) - ps;
         for (;;) {
             const int32_t hi = min(vend, ps + S) - ps;
 #pragma unroll
             for (int i = 0; i < NI; ++i) {
                 const int off = i * G + grp;
-                if (off >= lo && off < hi) acc[i % R] += t[i];
+                if (off >= lo && off < hi) acc[i % R] += (double)wc[i] * (double)gc[i];
             }
-            if (vend >= ps + S) break;  // the row goes on past this step
+            if (vend >= ps + S) return true;  // the row goes on past this step
             // row v ends inside this step: classes across groups, then slots
 #pragma unroll
             for (int sh = C; sh < kWave; sh <<= 1) {
@@ -389,7 +402,7 @@ __device__ __forceinline__ void wave_stream(const SpmmArgs& a, const ColState& c
             }
             const double total = acc[0];
             RowIn in;
-            in.xo = (G == 1) ? xo_l : __shfl(xo_l, (v - xb) * C + c, kWave);
+            in.xo = __shfl(xo_l, (v - xb) * C + c, kWave);
             in.deg = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(deg_l), v - wb));
             in.ts = __builtin_amdgcn_readlane(ts_l, v - wb);
             if (lane < C) {
@@ -402,26 +415,213 @@ __device__ __forceinline__ void wave_stream(const SpmmArgs& a, const ColState& c
 #pragma unroll
             for (int r = 0; r < R; ++r) acc[r] = 0.0;
             lo = max(lo, hi);
-            if (++v >= r1) break;
-            if (v - wb >= kWave) {
-                wb = v;
-                end_l = (wb + 1 + lane <= r1) ? a.row[wb + 1 + lane] : A1;
-                deg_l = (wb + lane < r1) ? a.deg[wb + lane] : 1.0f;
-                ts_l = (wb + lane < r1) ? a.tslot[wb + lane] : -1;
+            if (++v >= r1) return false;
+            // rows advance one at a time, so each window passes its midpoint before it ends
+            if (v - wb == kWave / 2) {
+                load_rows(wb + kWave, end_n, deg_n, ts_n);
+            } else if (v - wb == kWave) {
+                wb += kWave;
+                end_l = end_n;
+                deg_l = deg_n;
+                ts_l = ts_n;
             }
-            if (v - xb >= G) {
-                xb = v;
-                xo_l = (xb + grp < r1) ? a.x_in[(size_t)(xb + grp) * C + c] : 0.0f;
+            if (v - xb == G / 2) {
+                xo_n = load_old(xb + G);
+            }
+            if (v - xb == G) {
+                xb += G;
+                xo_l = xo_n;
             }
             vend = __builtin_amdgcn_readlane(end_l, v - wb);
         }
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            g[i] = g2[i];
-            w[i] = w2[i];
-        }
+    };
+    for (int32_t ps = P0;; ps += 2 * S) {
+        if (!step(ps, gA, wA, gB, wB)) break;
+        if (!step(ps + S, gB, wB, gA, wA)) break;
     }
 }
+
+// Width 64 (lane = column, one arc per gather instruction).  Every arc
+// index, weight, row bound and counter is wavefront-uniform and kept in
+// scalar registers: the 8 arc indices and weights of a step come in scalar
+// loads (issued one step ahead; the arc arrays carry kArcPad zeroed slots
+// past m, so a step's loads stay inside them), and a gather is the scalar
+// row address plus the lane's column -- per arc one vector load, two
+// conversions and one fused multiply-add (exact product of two floats, so
+// the same bits as a product and an add).  Step slots outside [A0, A1) hold
+// other rows' arcs or the zeroed pad; no row's bounds take them in.  Steps
+// of 8 arcs (step slot j is class j) alternate between two register sets:
+// the gathers of step i+1 are issued into one while step i is summed from
+// the other.  Old values come in windows of XR rows, the next window loaded
+// when the current one is entered, and row ends, degrees and slots in
+// windows of 64, the next loaded at the current one's midpoint: a run of
+// rows without arcs (half the rows of an R-MAT graph) waits on memory once
+// per window rather than once per row.  Arithmetic and fold order are
+// those of wave_stream.
+using arc_idx8 = int32_t __attribute__((ext_vector_type(8)));
+using arc_val8 = float __attribute__((ext_vector_type(8)));
+using scalar_idx8 = const __attribute__((address_space(4))) arc_idx8;
+using scalar_val8 = const __attribute__((address_space(4))) arc_val8;
+using global_float = const __attribute__((address_space(1))) float;
+
+// Width-64 X rows and arc slots addressable by 32-bit byte offsets from
+// their arrays' bases.
+constexpr int64_t kOff32Rows = (int64_t)1 << 24;
+constexpr int64_t kOff32Arcs = (int64_t)1 << 29;
+
+// Element (u, lane) of a width-64 X.  Off32: the scalar base plus a 32-bit
+// vector offset built by one vector shift-or (the scalar unit issues one
+// instruction per cycle per CU and is the scarcer resource in this loop);
+// otherwise a scalar 64-bit row address plus the lane's column.
+template <bool Off32>
+__device__ __forceinline__ float gather64(const float* x, int32_t u, int lane) {
+    if constexpr (Off32) {
+        const uint32_t off = ((uint32_t)u << 8) | ((uint32_t)lane << 2);
+        return *(global_float*)((uintptr_t)x + (uint64_t)off);
+    } else {
+        const uint64_t b = (uint64_t)(uintptr_t)x + (uint64_t)(uint32_t)u * (uint64_t)(kWave * sizeof(float));
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+        return ((global_float*)(uintptr_t)(((uint64_t)hi << 32) | lo))[lane];
+    }
+}
+
+template <int XR, bool Off32>
+__device__ __forceinline__ void wave_stream64(const SpmmArgs& a, const ColState& cs, double cj, double inv_n,
+                                              int32_t r0, int32_t r1, double& err, double& sum, double& dang) {
+    constexpr int C = kWave;
+    constexpr int S = 8;
+    r0 = __builtin_amdgcn_readfirstlane(r0);
+    r1 = __builtin_amdgcn_readfirstlane(r1);
+    if (r0 >= r1) return;
+    const int lane = lane_id();
+    const float* __restrict__ xin = a.x_in;
+    const int32_t A0 = __builtin_amdgcn_readfirstlane(a.row[r0]);
+    const int32_t A1 = __builtin_amdgcn_readfirstlane(a.row[r1]);
+    const int32_t P0 = A0 & ~7;
+    auto arc_off = [&](int32_t p) -> uint64_t {
+        if constexpr (Off32) return (uint64_t)((uint32_t)p << 2);
+        else return (uint64_t)(uint32_t)p << 2;
+    };
+    auto load_idx = [&](int32_t p) { return *(scalar_idx8*)((uintptr_t)a.col + arc_off(p)); };
+    auto load_val = [&](int32_t p) { return *(scalar_val8*)((uintptr_t)a.nval + arc_off(p)); };
+    auto issue = [&](const arc_idx8& u, float (&g)[S]) {
+#pragma unroll
+        for (int j = 0; j < S; ++j) g[j] = gather64<Off32>(xin, u[j], lane);
+    };
+    // row ends, degrees and slots of rows [b, b+64) (clamped loads: lanes past r1 are never read)
+    auto load_rows = [&](int32_t b, int32_t& e, float& d, int32_t& t) {
+        e = a.row[min(b + 1 + lane, r1)];
+        d = a.deg[min(b + lane, r1 - 1)];
+        t = a.tslot[min(b + lane, r1 - 1)];
+    };
+    auto load_old = [&](int32_t b, float (&x)[XR]) {
+#pragma unroll
+        for (int i = 0; i < XR; ++i) x[i] = xin[(size_t)min(b + i, r1 - 1) * C + lane];
+    };
+    float gA[S], gB[S];
+    arc_val8 wA = load_val(P0), wB;
+    issue(load_idx(P0), gA);
+    arc_idx8 u_next = load_idx(P0 + S);  // the step after the one in flight
+    arc_val8 w_next = load_val(P0 + S);
+    int32_t wb = r0, end_l, end_n = 0, ts_l, ts_n = 0;
+    float deg_l, deg_n = 0.0f;
+    load_rows(wb, end_l, deg_l, ts_l);
+    int32_t xb = r0;
+    float xo[XR], xn[XR];
+    load_old(xb, xo);
+    load_old(xb + XR, xn);
+    int32_t v = r0;
+    int32_t vend = __builtin_amdgcn_readlane(end_l, 0);
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // per column: whether it iterates, and its teleport weight off the seed set
+    const bool live = cs.active && !cs.done;
+    const double tele_def = cs.uniform ? inv_n : 0.0;
+    // row v ends here: finalize<C>'s arithmetic with the loop-invariant parts
+    // hoisted, straight-line (a row without arcs has class tree +0, which
+    // gives the same value as its short form); false after row r1-1
+    auto finish_row = [&]() {
+        const int32_t d = v - xb;
+        float xo_v = xo[0];
+#pragma unroll
+        for (int i = 1; i < XR; ++i)
+            if (d == i) xo_v = xo[i];
+        const float deg_v = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(deg_l), v - wb));
+        const int32_t ts_v = __builtin_amdgcn_readlane(ts_l, v - wb);
+        const double total = class_tree(acc);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[r] = 0.0;
+        double tele = tele_def;
+        if (ts_v >= 0) {
+            const double pt = (double)a.ptab[(size_t)ts_v * kCols + lane];
+            if (!cs.uniform) tele = pt;
+        }
+        const float xv = next_value(cs.alpha, total, cj, tele);
+        a.x_out[(size_t)v * C + lane] = live ? xv : xo_v;
+        const double xd = (double)xv;
+        err += live ? fabs(xd - (double)xo_v) : 0.0;
+        sum += live ? xd : 0.0;
+        dang += (live && deg_v == 0.0f) ? xd : 0.0;
+        if (++v >= r1) return false;
+        // rows advance one at a time, so each window passes its midpoint before it ends
+        if (v - wb == kWave / 2) {
+            load_rows(wb + kWave, end_n, deg_n, ts_n);
+        } else if (v - wb == kWave) {
+            wb += kWave;
+            end_l = end_n;
+            deg_l = deg_n;
+            ts_l = ts_n;
+        }
+        if (v - xb >= XR) {
+            xb += XR;
+#pragma unroll
+            for (int i = 0; i < XR; ++i) xo[i] = xn[i];
+            load_old(xb + XR, xn);
+        }
+        vend = __builtin_amdgcn_readlane(end_l, v - wb);
+        return true;
+    };
+    // issue step ps+S into (gn, wn), then add step ps from (gc, wc) one row
+    // segment at a time, finishing each row that ends inside it; false once
+    // row r1-1 is done.  A segment's slots keep their weights and the others
+    // get weight 0 (an exact no-op: the gathered values are finite and
+    // non-negative), so there is one update of the accumulators whatever the
+    // segment (separate full-step and partial paths measured slower: the
+    // accumulators then move between register homes at every join).  A row
+    // ending at ps+S exactly is finished by the next step.
+    auto step = [&](int32_t ps, const float (&gc)[S], const arc_val8& wc, float (&gn)[S], arc_val8& wn) {
+        issue(u_next, gn);
+        wn = w_next;
+        u_next = load_idx(ps + 2 * S);
+        w_next = load_val(ps + 2 * S);
+        double gd[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) gd[k] = (double)gc[k];
+        int32_t j0 = 0;  // first slot of the current row in this step
+        for (;;) {
+            const int32_t e = min(vend - ps, S);
+            const uint32_t m = (0xFFu << j0) & ~(0xFFu << e);
+#pragma unroll
+            for (int k = 0; k < S; ++k) {
+                const float wz = ((m >> k) & 1u) ? wc[k] : 0.0f;
+                acc[k] = __builtin_fma((double)wz, gd[k], acc[k]);
+            }
+            if (vend >= ps + S) return true;
+            if (!finish_row()) return false;
+            j0 = e;
+        }
+    };
+    // slots of the first step before A0 hold other wavefronts' arcs: weight 0 adds +0
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+        if (P0 + j < A0) wA[j] = 0.0f;
+    for (int32_t ps = P0;; ps += 2 * S) {
+        if (!step(ps, gA, wA, gB, wB)) break;
+        if (!step(ps + S, gB, wB, gA, wA)) break;
+    }
+}
+
+constexpr int kW64OldRows = 4;  // old-value rows per window of wave_stream64
 
 // Hub segments (a launch of their own: their register needs differ from
 // the row kernel's).  Block b is segment b; its partial slots are zeroed.
@@ -470,7 +670,18 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
     const double inv_n = 1.0 / (double)a.n;
     double err = 0.0, sum = 0.0, dang = 0.0;
     const uint32_t gw = b * kRowWaves + w;
-    if (gw < a.n_waves) wave_stream<C>(a, cs, cj, inv_n, a.wave_rows[gw], a.wave_rows[gw + 1], err, sum, dang);
+    if (gw < a.n_waves) {
+        if constexpr (C == kWave) {
+            if (a.off32)
+                wave_stream64<kW64OldRows, true>(a, cs, cj, inv_n, a.wave_rows[gw], a.wave_rows[gw + 1], err, sum,
+                                                 dang);
+            else
+                wave_stream64<kW64OldRows, false>(a, cs, cj, inv_n, a.wave_rows[gw], a.wave_rows[gw + 1], err,
+                                                  sum, dang);
+        }
+        else
+            wave_stream<C>(a, cs, cj, inv_n, a.wave_rows[gw], a.wave_rows[gw + 1], err, sum, dang);
+    }
     sh[0][w][lane] = owner ? err : 0.0;
     sh[1][w][lane] = owner ? sum : 0.0;
     sh[2][w][lane] = owner ? dang : 0.0;
@@ -651,7 +862,7 @@ __global__ void k_nval(const int32_t* __restrict__ col, const double* __restrict
     nval[q] = (d > 0.0) ? (float)(num / d) : 0.0f;
 }
 
-// wave_rows[k] = first row v in [n_hub, n] with (row[v]-row[n_hub]) + 2*(v-n_hub) >= k*per_wave
+// wave_rows[k] = first row v in [n_hub, n] with (row[v]-row[n_hub]) + kRowItems*(v-n_hub) >= k*per_wave
 __global__ void k_wave_rows(const int32_t* __restrict__ row, int32_t n_hub, int32_t n, int64_t per_wave,
                             int64_t n_waves, int32_t* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -665,7 +876,7 @@ __global__ void k_wave_rows(const int32_t* __restrict__ row, int32_t n_hub, int3
     int32_t lo = n_hub, hi = n;
     while (lo < hi) {
         const int32_t mid = lo + ((hi - lo) >> 1);
-        const int64_t d = ((int64_t)row[mid] - base) + 2 * (int64_t)(mid - n_hub);
+        const int64_t d = ((int64_t)row[mid] - base) + kRowItems * (int64_t)(mid - n_hub);
         if (d < target) lo = mid + 1;
         else hi = mid;
     }
@@ -741,8 +952,10 @@ struct Batch {
 };
 
 int ensure_nval(Graph& g) {
-    if (g.p_nval.p && g.p_nval.bytes >= (size_t)std::max<int64_t>(g.m, 1) * 4) return EGX_SUCC;
-    EGX_TRY(g.p_nval.alloc((size_t)std::max<int64_t>(g.m, 1) * 4));
+    const size_t bytes = ((size_t)std::max<int64_t>(g.m, 1) + kArcPad) * 4;
+    if (g.p_nval.p && g.p_nval.bytes >= bytes) return EGX_SUCC;
+    EGX_TRY(g.p_nval.alloc(bytes));
+    EGX_HIP_TRY(hipMemsetAsync(g.p_nval.as<float>() + g.m, 0, kArcPad * 4, g.stream), EGX_DEVICE_ERR);
     if (g.m > 0) {
         k_nval<<<nblk(g.m, 256), 256, 0, g.stream>>>(g.it_col, g.it_w, g.it_outw, g.m, g.weighted ? 1 : 0,
                                                       g.p_nval.as<float>());
@@ -833,7 +1046,7 @@ int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
     EGX_HIP_TRY(hipMemcpy(&row_end_hub, g.it_row + g.n_hub, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipMemcpy(&row_end, g.it_row + n, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
     // wavefront k holds the rows whose merge-path diagonal starts in its share
-    const int64_t work = (int64_t)(row_end - row_end_hub) + 2 * (int64_t)(n - g.n_hub);
+    const int64_t work = (int64_t)(row_end - row_end_hub) + kRowItems * (int64_t)(n - g.n_hub);
     int64_t waves = std::min<int64_t>(16384, std::max<int64_t>(1, (work + 511) / 512));
     const int64_t per_wave = std::max<int64_t>(1, (work + waves - 1) / waves);
     waves = std::max<int64_t>(1, (work + per_wave - 1) / per_wave);
@@ -1109,6 +1322,7 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
     a.n_red = b.n_red;
     a.n_hrows = b.n_hrows;
     a.n = g.n;
+    a.off32 = (g.n <= kOff32Rows && g.m + kArcPad <= kOff32Arcs) ? 1 : 0;
     a.nblk_h = b.nblk_h;
     a.grid = b.grid;
     a.n_waves = b.n_waves;
