@@ -327,14 +327,17 @@ def _serving_requests(graphs, count, seed):
     return reqs
 
 
-def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, step_iters=4, repeats=3):
-    """Poisson arrivals against a device-resident pool; p50/p99 latency at a
-    fixed QPS beside the closed-loop solve rate and the one-call-per-solve
-    rate on the same request stream.  The closed loop runs with request
-    coalescing (identical requests in flight share a column; the server
-    default) and with every request on a column of its own; closed loops and
-    the one-call-per-request sample each report the best of `repeats`
-    passes on warm state."""
+SERVING_QPS = (2000.0, 4000.0, 8000.0, 12000.0)
+
+
+def serving(pool_size=32, requests=600, qps=SERVING_QPS, seed=77, plain_sample=200, step_iters=4, repeats=3):
+    """Poisson arrivals against a device-resident pool at each offered rate in
+    `qps` (requests/s), with request coalescing (identical requests in
+    flight share a column; the server default) and with every request on a
+    column of its own, side by side: p50/p99 latency and the achieved solve
+    rate at each.  Beside them the closed loops (every request submitted at
+    once, best of `repeats` passes on warm state) and the one-call-per-
+    request rate on the same request stream."""
     specs = workloads.serving_pool_specs(pool_size, seed=seed)
     graphs = [s.build(native) for s in specs]
     arcs = [int(g.info()["m"]) for g in graphs]
@@ -360,21 +363,6 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
         plain_walls.append(time.perf_counter() - t0)
     plain_rate = len(sample) / min(plain_walls)
 
-    def run(rate, coalesce=True, passes=1):
-        """The request stream against a warm server, `passes` times; returns
-        the best pass (solves/s, latencies, stats of that pass)."""
-        best = None
-        with native.Server(graphs, step_iters=step_iters, coalesce=coalesce) as srv:
-            # warm: open every graph's session (steady-state serving keeps them open)
-            for gi in range(len(graphs)):
-                for rid in [srv.submit(gi, seeds, alpha=0.85, max_iter=100, tol=1e-6) for seeds in ((), (0,))]:
-                    srv.wait(rid)
-            for _ in range(passes):
-                res = _one_pass(srv, rate)
-                if best is None or res[0] > best[0]:
-                    best = res
-        return best
-
     def _one_pass(srv, rate):
         st0 = srv.stats()
         ids = [None] * len(reqs)
@@ -397,42 +385,53 @@ def serving(pool_size=32, requests=600, qps=None, seed=77, plain_sample=200, ste
         th = threading.Thread(target=feed)
         th.start()
         th.join()
-        lat, widths = [], []
+        lat = []
         for rid in ids:
             _, info = srv.wait(rid)
             lat.append(info["latency_seconds"])
-            widths.append(info["width"])
         wall = time.perf_counter() - t0
         st = srv.stats()
         st = {k: (v - st0[k] if k in ("coalesced", "sessions_opened", "sessions_closed", "steps") else v)
               for k, v in st.items()}
         return len(reqs) / wall, np.array(lat), st
 
-    closed_rate, closed_lat, closed_st = run(None, passes=repeats)
-    solo_rate, solo_lat, solo_st = run(None, coalesce=False, passes=repeats)  # every request on its own column
-    target = qps if qps else 0.5 * closed_rate
-    rate, lat, st = run(target)
+    def run(rate, coalesce, passes):
+        """The request stream against a warm server, `passes` times; the best
+        pass (solves/s, latencies, stats of that pass)."""
+        best = None
+        with native.Server(graphs, step_iters=step_iters, coalesce=coalesce) as srv:
+            # warm: open every graph's session (steady-state serving keeps them open)
+            for gi in range(len(graphs)):
+                for rid in [srv.submit(gi, seeds, alpha=0.85, max_iter=100, tol=1e-6) for seeds in ((), (0,))]:
+                    srv.wait(rid)
+            for _ in range(passes):
+                res = _one_pass(srv, rate)
+                if best is None or res[0] > best[0]:
+                    best = res
+        return best
+
+    def summary(res):
+        rate, lat, st = res
+        return {"solves_per_s": round(rate, 1), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "coalesced_requests": st["coalesced"],
+                "peak_active_columns": st["peak_active_columns"]}
+
+    closed = {"coalesced": summary(run(None, True, repeats)), "no_coalescing": summary(run(None, False, repeats))}
+    sweep = []
+    for q in qps:
+        sweep.append({"qps_offered": q, "coalesced": summary(run(q, True, 1)),
+                      "no_coalescing": summary(run(q, False, 1))})
     res = {
         "workload": f"pool of {pool_size} R-MAT graphs ({min(arcs)}-{max(arcs)} arcs, every other weighted), "
-                    f"{requests} requests: 70% global, 30% personalised (1-16 seeds), alpha 0.85/0.9, tol 1e-6",
-        "qps_offered": round(target, 1),
-        "solves_per_s": round(rate, 1),
-        "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
-        "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
-        "closed_loop_solves_per_s": round(closed_rate, 1),
-        "closed_loop_p50_ms": round(float(np.percentile(closed_lat, 50)) * 1e3, 3),
-        "closed_loop_p99_ms": round(float(np.percentile(closed_lat, 99)) * 1e3, 3),
-        "closed_loop_coalesced_requests": closed_st["coalesced"],
-        "closed_loop_no_coalescing_solves_per_s": round(solo_rate, 1),
-        "closed_loop_no_coalescing_p99_ms": round(float(np.percentile(solo_lat, 99)) * 1e3, 3),
+                    f"{requests} requests: 70% global, 30% personalised (1-16 seeds), alpha 0.85/0.9, tol 1e-6, "
+                    f"Poisson arrivals",
+        "qps_sweep": sweep,
+        "closed_loop": closed,
         "plain_solves_per_s": round(plain_rate, 1),
         "plain_calls_timed": len(sample),
         "best_of_passes": repeats,
-        "speedup_vs_plain": round(closed_rate / plain_rate, 2),
-        "speedup_vs_plain_no_coalescing": round(solo_rate / plain_rate, 2),
-        "peak_active_columns": st["peak_active_columns"],
-        "sessions_opened": st["sessions_opened"],
-        "coalesced_requests": st["coalesced"],
+        "speedup_vs_plain": round(closed["coalesced"]["solves_per_s"] / plain_rate, 2),
+        "speedup_vs_plain_no_coalescing": round(closed["no_coalescing"]["solves_per_s"] / plain_rate, 2),
     }
     for g in graphs:
         g.close()

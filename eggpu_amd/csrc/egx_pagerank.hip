@@ -411,7 +411,8 @@ constexpr int kPbThreads = 1024;  // one scatter workgroup per CU: its chunk tak
 // from a wave prefix sum) and stored by consecutive lanes to consecutive
 // slots.  Otherwise each lane stores its terms from its group's first value
 // slot on.
-constexpr int kPbGroups = 4;
+constexpr int kPbGroups = 8;   // 4 measured 3% slower at R-MAT 24 (less of the latency covered)
+constexpr int kPbGroupsW = 2;  // weighted: 32 more bytes per group (4 groups spill)
 constexpr int kPbStage = 504;  // staged terms per wavefront (64 groups emit at most 512)
 typedef uint32_t pb_u32x4 __attribute__((ext_vector_type(4)));
 typedef float pb_f32x4 __attribute__((ext_vector_type(4)));
@@ -503,11 +504,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // The source slots of kPbGroups groups per lane, loaded together; the
 // weighted arrays only when W.
-// Weighted graphs carry 32 more bytes per group, so they keep half as
-// many groups per batch (the double-buffered batches then fit in registers).
+// Weighted graphs carry 32 more bytes per group, so they keep fewer groups
+// per batch (the double-buffered batches then fit in registers).
 template <bool W>
 struct PbScatterBatch {
-    static constexpr int kG = W ? kPbGroups / 2 : kPbGroups;
+    static constexpr int kG = W ? kPbGroupsW : kPbGroups;
     pb_u32x4 u[kG];
     pb_f32x4 wa[W ? kG : 1], wb[W ? kG : 1];
     int32_t vg[kG];
