@@ -394,6 +394,7 @@ int egx_graph_set_bins(egx_graph* gh, const egx_bin_opts* opts) {
     if (opts->tile_sort >= 0) cfg.tile_sort = opts->tile_sort ? 1 : 0;
     if (opts->hot_sources != 0) cfg.hot_sources = opts->hot_sources > 0 ? opts->hot_sources : 0;
     if (opts->blocked >= 0) cfg.blocked = opts->blocked ? 1 : 0;
+    if (opts->blocked_shared >= 0) cfg.blocked_shared = opts->blocked_shared ? 1 : 0;
     return egx::rebuild_bins(g, cfg);
 }
 

@@ -495,12 +495,14 @@ int64_t pb_item_slots(int64_t total, int64_t items, int64_t lo, int64_t hi) {
 // and sum work items.
 int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     // the blocked layout depends only on the iteration CSC
-    if (cfg.blocked && g.pb_ready && !layout_changed) return EGX_SUCC;
+    if (cfg.blocked && g.pb_ready && !layout_changed && g.pb_shared == cfg.blocked_shared) return EGX_SUCC;
     g.pb_ready = false;
     for (DevBuf* d : {&g.pb_src, &g.pb_dst, &g.pb_vals, &g.pb_vgrp, &g.pb_mask, &g.pb_scatter, &g.pb_sum,
                       &g.pb_hrows, &g.pb_hub_part, &g.pb_w, &g.pb_arc})
         d->release();
     g.n_pb_scatter = g.n_pb_sum = g.n_pb_hrows = 0;
+    g.pb_scatter_size.clear();
+    g.pb_sum_size.clear();
     if (!cfg.blocked || g.n <= 0 || g.m <= 0) return EGX_SUCC;
     const int32_t n = g.n;
     const int64_t m = g.m;
@@ -615,9 +617,12 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     EGX_HIP_TRY(hipMemcpy(&bad_h, bad.p, 4, hipMemcpyDeviceToHost), EGX_DEVICE_ERR);
 
     // scatter work: contiguous group ranges of at most scat_slots slots per chunk
-    const int32_t scat_groups =
-        (int32_t)(pb_item_slots(slots, kPbScatterItems, kPbScatterMin, kPbScatterItem) / kPbVec);
-    const int32_t sum_slots = (int32_t)pb_item_slots(nvals, kPbSumItems, kPbSumMin, kPbItem);
+    // (a graph laid out for shared launches takes the largest items: the set fills the device)
+    const int32_t scat_groups = (int32_t)(pb_item_slots(slots, cfg.blocked_shared ? 1 : kPbScatterItems,
+                                                        kPbScatterMin, kPbScatterItem) /
+                                          kPbVec);
+    const int32_t sum_slots =
+        (int32_t)pb_item_slots(nvals, cfg.blocked_shared ? 1 : kPbSumItems, kPbSumMin, kPbItem);
     std::vector<PbScatter> scat;
     for (int64_t c = 0; c < S; ++c) {
         const int32_t c0 = (int32_t)(soff_h[(size_t)(c * B)] / kPbVec);
@@ -688,6 +693,11 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     g.n_pb_scatter = (int64_t)scat.size();
     g.n_pb_sum = (int64_t)sums.size();
     g.n_pb_hrows = (int64_t)hrows.size();
+    g.pb_scatter_size.resize(scat.size());
+    for (size_t i = 0; i < scat.size(); ++i) g.pb_scatter_size[i] = scat[i].g1 - scat[i].g0;
+    g.pb_sum_size.resize(sums.size());
+    for (size_t i = 0; i < sums.size(); ++i) g.pb_sum_size[i] = sums[i].e1 - sums[i].e0;
+    g.pb_shared = cfg.blocked_shared;
     g.pb_ready = true;
     return EGX_SUCC;
 }

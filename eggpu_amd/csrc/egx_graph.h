@@ -90,6 +90,7 @@ struct BinCfg {
     int32_t tile_sort = 0;        // gather each tile's arcs in source order (LDS scatter back to row order)
     int32_t hot_sources = 0;      // > 0: sources past this label are gathered non-temporally
     int32_t blocked = 0;          // propagation-blocked iterations (unweighted, FP32)
+    int32_t blocked_shared = 0;   // blocked items sized for shared multi-graph launches
 };
 
 // Propagation-blocked iteration (bins.blocked).  Sources are cut into chunks
@@ -157,6 +158,10 @@ struct Graph {
     bool pb_ready = false;
     DevBuf pb_src, pb_dst, pb_vals, pb_vgrp, pb_mask, pb_scatter, pb_sum, pb_hrows, pb_hub_part, pb_w, pb_arc;
     int64_t n_pb_scatter = 0, n_pb_sum = 0, n_pb_hrows = 0;
+    // host copies of the items' sizes (groups per scatter item, value slots
+    // per sum item), for ordering them in multi-graph launches
+    std::vector<int32_t> pb_scatter_size, pb_sum_size;
+    int32_t pb_shared = -1;  // item sizing the blocked layout was built with (BinCfg::blocked_shared)
     double alpha_vals = 0.0;   // alpha baked into it_val
     double alpha_orig = 0.0;   // alpha baked into in_val
     int32_t max_in_degree = 0;

@@ -569,13 +569,13 @@ __device__ __forceinline__ void pb_scatter_batch(const PbScatterBatch<W>& cur, i
     }
 }
 
+// Scatter item `item` of one graph, in a workgroup of kPbThreads threads
+// with sc (kPbChunk + 4 floats) and stage (kPbThreads / kWave rows of
+// kPbStage floats) in LDS.
 template <typename T, bool W>
-__global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs pb) {
-    __shared__ __align__(16) float sc[kPbChunk + 4];
-    __shared__ float stage[kPbThreads / kWave][kPbStage];
-    if (a.state->done) return;
-    bind(a);
-    const PbScatter it = pb.scatter[blockIdx.x];
+__device__ __forceinline__ void pb_scatter_item(const IterArgs<T>& a, const PbArgs& pb, uint32_t item, float* sc,
+                                                float (*stage)[kPbStage]) {
+    const PbScatter it = pb.scatter[item];
     const int l = lane_id();
     // every lane of a wavefront runs the loop the same number of times (the
     // bound depends on the wavefront's first thread only), so the shuffles
@@ -613,6 +613,31 @@ __global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs
         if (g0 + 2 * kStep < it.g1) pb_scatter_load<W>(pb, g0 + 2 * kStep, it.g1, l, nxt);
         pb_scatter_batch<W>(nx2, g0 + kStep, it.g1, sc, stg, pb.vals, l);
     }
+}
+
+template <typename T, bool W>
+__global__ __launch_bounds__(kPbThreads) void k_pb_scatter(IterArgs<T> a, PbArgs pb) {
+    __shared__ __align__(16) float sc[kPbChunk + 4];
+    __shared__ float stage[kPbThreads / kWave][kPbStage];
+    if (a.state->done) return;
+    bind(a);
+    pb_scatter_item<T, W>(a, pb, blockIdx.x, sc, stage);
+}
+
+// Many graphs, one launch: workgroup b runs scatter item item_local[b] of
+// graph item_graph[b].
+template <typename T, bool W>
+__global__ __launch_bounds__(kPbThreads) void k_pb_scatter_multi(const IterArgs<T>* __restrict__ table,
+                                                                 const PbArgs* __restrict__ pbs,
+                                                                 const int32_t* __restrict__ item_graph,
+                                                                 const uint32_t* __restrict__ item_local) {
+    __shared__ __align__(16) float sc[kPbChunk + 4];
+    __shared__ float stage[kPbThreads / kWave][kPbStage];
+    const int32_t gi = item_graph[blockIdx.x];
+    IterArgs<T> a = table[gi];
+    if (a.state->done) return;
+    bind(a);
+    pb_scatter_item<T, W>(a, pbs[gi], item_local[blockIdx.x], sc, stage);
 }
 
 // One sum item (one workgroup of kPbSumWaves wavefronts): its value slots
@@ -713,13 +738,12 @@ __device__ __forceinline__ void finalize_rows_fix(const IterArgs<T>& a, int32_t 
     }
 }
 
+// Sum item `item` of one graph (partial slot `item`), in a workgroup of
+// kPbSumThreads threads with acc (kPbRows) and sh_red (3 * kPbSumWaves) in LDS.
 template <typename T, bool W>
-__global__ __launch_bounds__(kPbSumThreads) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
-    __shared__ unsigned long long acc[kPbRows];
-    __shared__ double sh_red[3 * kPbSumWaves];
-    if (a.state->done) return;
-    bind(a);
-    const PbSum it = pb.sums[blockIdx.x];
+__device__ __forceinline__ void pb_sum_item(const IterArgs<T>& a, const PbArgs& pb, uint32_t item,
+                                            unsigned long long* acc, double* sh_red) {
+    const PbSum it = pb.sums[item];
     const int t = threadIdx.x;
     const int l = lane_id();
     for (int32_t i = t; i < kPbRows; i += kPbSumThreads) acc[i] = 0ull;
@@ -775,10 +799,35 @@ __global__ __launch_bounds__(kPbSumThreads) void k_pb_sum(IterArgs<T> a, PbArgs 
             a.hub_part[(size_t)it.part0 + (size_t)i * it.pieces + it.piece] = (double)acc[i] * kPbUnfix;
     }
     if (t == 0) {
-        a.partials[blockIdx.x] = err;
-        a.partials[(size_t)a.grid + blockIdx.x] = sum;
-        a.partials[2 * (size_t)a.grid + blockIdx.x] = dang;
+        a.partials[item] = err;
+        a.partials[(size_t)a.grid + item] = sum;
+        a.partials[2 * (size_t)a.grid + item] = dang;
     }
+}
+
+template <typename T, bool W>
+__global__ __launch_bounds__(kPbSumThreads) void k_pb_sum(IterArgs<T> a, PbArgs pb) {
+    __shared__ unsigned long long acc[kPbRows];
+    __shared__ double sh_red[3 * kPbSumWaves];
+    if (a.state->done) return;
+    bind(a);
+    pb_sum_item<T, W>(a, pb, blockIdx.x, acc, sh_red);
+}
+
+// Many graphs, one launch: workgroup b runs sum item item_local[b] of graph
+// item_graph[b].
+template <typename T, bool W>
+__global__ __launch_bounds__(kPbSumThreads) void k_pb_sum_multi(const IterArgs<T>* __restrict__ table,
+                                                                const PbArgs* __restrict__ pbs,
+                                                                const int32_t* __restrict__ item_graph,
+                                                                const uint32_t* __restrict__ item_local) {
+    __shared__ unsigned long long acc[kPbRows];
+    __shared__ double sh_red[3 * kPbSumWaves];
+    const int32_t gi = item_graph[blockIdx.x];
+    IterArgs<T> a = table[gi];
+    if (a.state->done) return;
+    bind(a);
+    pb_sum_item<T, W>(a, pbs[gi], item_local[blockIdx.x], acc, sh_red);
 }
 
 // Many graphs, one launch: block b works for graph blk_graph[b], as its
@@ -1249,7 +1298,64 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
         total += table[(size_t)i].grid;
         nbase[(size_t)i + 1] = nbase[(size_t)i] + nblk(gs[(size_t)i]->n, 256);
     }
-    if (total > 0x7FFFFFFFull) {
+    // Members with a propagation-blocked layout (FP32 iterates; weighted ones
+    // with their per-slot arc values) iterate as they do alone: their
+    // scatter items and sum items go into two table-driven launches, and
+    // only the others' work items into the pull launch.
+    constexpr bool kBlockable = std::is_same<T, float>::value;
+    std::vector<PbArgs> pbt((size_t)count);
+    std::vector<char> is_pb((size_t)count, 0);
+    std::vector<int32_t> sc_graph, sum_graph;
+    std::vector<uint32_t> sc_local, sum_local;
+    struct PbItem {
+        int32_t size, graph;
+        uint32_t local;
+    };
+    std::vector<PbItem> sc_items, sum_items;
+    if constexpr (kBlockable) {
+        for (int32_t i = 0; i < count; ++i) {
+            Graph& g = *gs[(size_t)i];
+            if (!g.pb_ready || (W && g.pb_w.p == nullptr)) continue;
+            is_pb[(size_t)i] = 1;
+            IterArgs<T>& a = table[(size_t)i];
+            total -= a.grid;
+            EGX_TRY(ensure_workspace(g, (uint32_t)g.n_pb_sum));
+            a.partials = g.partials.as<double>();
+            a.hrows = g.pb_hrows.as<HubRow>();
+            a.hub_part = g.pb_hub_part.as<double>();
+            a.n_hrows = g.n_pb_hrows;
+            a.nblk_h = 0;
+            a.grid = (uint32_t)g.n_pb_sum;
+            a.n_red = (uint32_t)std::min<int64_t>(
+                kMaxRed, std::max<int64_t>(1, (a.n_hrows + a.grid + kRedItems - 1) / kRedItems));
+            PbArgs& pb = pbt[(size_t)i];
+            pb.src = g.pb_src.as<uint16_t>();
+            pb.dst = g.pb_dst.as<uint16_t>();
+            pb.vals = g.pb_vals.as<float>();
+            pb.vgrp = g.pb_vgrp.as<int32_t>();
+            pb.mask = g.pb_mask.as<uint8_t>();
+            pb.scatter = g.pb_scatter.as<PbScatter>();
+            pb.sums = g.pb_sum.as<PbSum>();
+            pb.w = W ? g.pb_w.as<float>() : nullptr;
+            for (int64_t k = 0; k < g.n_pb_scatter; ++k)
+                sc_items.push_back({g.pb_scatter_size[(size_t)k], i, (uint32_t)k});
+            for (int64_t k = 0; k < g.n_pb_sum; ++k) sum_items.push_back({g.pb_sum_size[(size_t)k], i, (uint32_t)k});
+        }
+        // largest items first, so the small ones fill in behind them rather
+        // than a large one running alone at the end of the launch
+        auto by_size = [](const PbItem& x, const PbItem& y) { return x.size > y.size; };
+        std::stable_sort(sc_items.begin(), sc_items.end(), by_size);
+        std::stable_sort(sum_items.begin(), sum_items.end(), by_size);
+        for (const PbItem& it : sc_items) {
+            sc_graph.push_back(it.graph);
+            sc_local.push_back(it.local);
+        }
+        for (const PbItem& it : sum_items) {
+            sum_graph.push_back(it.graph);
+            sum_local.push_back(it.local);
+        }
+    }
+    if (total > 0x7FFFFFFFull || sc_graph.size() > 0x7FFFFFFFull || sum_graph.size() > 0x7FFFFFFFull) {
         set_last_error("too many work items for one launch");
         return EGX_UNSUPPORTED_GRAPH;
     }
@@ -1271,6 +1377,7 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
         std::vector<uint64_t> load(kXcds, 0);
         const bool placement = p.xcd_placement;
         for (int32_t gi : order) {
+            if (is_pb[(size_t)gi]) continue;
             const uint32_t grid = table[(size_t)gi].grid;
             const bool affine = placement && (int64_t)gs[(size_t)gi]->n * (int64_t)sizeof(T) <= kAffineBytes;
             if (affine) {
@@ -1316,7 +1423,25 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     std::vector<int32_t> rblk((size_t)rtotal);
     for (int32_t i = 0; i < count; ++i)
         std::fill(rblk.begin() + rbase[(size_t)i], rblk.begin() + rbase[(size_t)i] + table[(size_t)i].n_red, i);
-    DevBuf d_table, d_blk, d_base, d_rblk, d_rbase, d_nbase, d_states;
+    DevBuf d_table, d_blk, d_base, d_rblk, d_rbase, d_nbase, d_states, d_pbt, d_scg, d_scl, d_sumg, d_suml;
+    const uint32_t n_sc = (uint32_t)sc_graph.size(), n_sum = (uint32_t)sum_graph.size();
+    if (n_sc > 0) {
+        EGX_TRY(d_pbt.alloc(sizeof(PbArgs) * (size_t)count));
+        EGX_TRY(d_scg.alloc((size_t)n_sc * 4));
+        EGX_TRY(d_scl.alloc((size_t)n_sc * 4));
+        EGX_TRY(d_sumg.alloc((size_t)n_sum * 4));
+        EGX_TRY(d_suml.alloc((size_t)n_sum * 4));
+        EGX_HIP_TRY(hipMemcpyAsync(d_pbt.p, pbt.data(), sizeof(PbArgs) * (size_t)count, hipMemcpyHostToDevice, s),
+                    EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipMemcpyAsync(d_scg.p, sc_graph.data(), (size_t)n_sc * 4, hipMemcpyHostToDevice, s),
+                    EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipMemcpyAsync(d_scl.p, sc_local.data(), (size_t)n_sc * 4, hipMemcpyHostToDevice, s),
+                    EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipMemcpyAsync(d_sumg.p, sum_graph.data(), (size_t)n_sum * 4, hipMemcpyHostToDevice, s),
+                    EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipMemcpyAsync(d_suml.p, sum_local.data(), (size_t)n_sum * 4, hipMemcpyHostToDevice, s),
+                    EGX_DEVICE_ERR);
+    }
     EGX_TRY(d_table.alloc(sizeof(IterArgs<T>) * (size_t)count));
     EGX_TRY(d_nbase.alloc(nbase.size() * 4));
     EGX_TRY(d_states.alloc(sizeof(PrState) * (size_t)count));
@@ -1342,6 +1467,23 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     PrState* hs = static_cast<PrState*>(pinned);
 
     const int32_t chunk = p.check_every > 0 ? p.check_every : 8;
+    // the SpMV of one iteration for every member: blocked members' scatter,
+    // the others' pull work items, then the blocked members' sums
+    auto launch_spmv = [&]() {
+        if constexpr (kBlockable) {
+            if (n_sc > 0)
+                k_pb_scatter_multi<T, W><<<n_sc, kPbThreads, 0, s>>>(d_table.as<IterArgs<T>>(), d_pbt.as<PbArgs>(),
+                                                                      d_scg.as<int32_t>(), d_scl.as<uint32_t>());
+        }
+        if (total > 0)
+            k_pr_spmv_multi<T, W><<<(unsigned)total, kThreads, 0, s>>>(d_table.as<IterArgs<T>>(), d_blk.as<int32_t>(),
+                                                                      d_base.as<uint32_t>());
+        if constexpr (kBlockable) {
+            if (n_sum > 0)
+                k_pb_sum_multi<T, W><<<n_sum, kPbSumThreads, 0, s>>>(d_table.as<IterArgs<T>>(), d_pbt.as<PbArgs>(),
+                                                                     d_sumg.as<int32_t>(), d_suml.as<uint32_t>());
+        }
+    };
     CapturedRun cap;
     cap.stream = s;
     // alpha refreshes run on each graph's own stream
@@ -1358,8 +1500,7 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
             if (!cap.exec) {
                 EGX_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), EGX_DEVICE_ERR);
                 for (int32_t q = 0; q < chunk; ++q) {
-                    k_pr_spmv_multi<T, W><<<(unsigned)total, kThreads, 0, s>>>(
-                        d_table.as<IterArgs<T>>(), d_blk.as<int32_t>(), d_base.as<uint32_t>());
+                    launch_spmv();
                     k_pr_reduce_multi<T, W><<<(unsigned)rtotal, kReduceThreads, 0, s>>>(
                         d_table.as<IterArgs<T>>(), d_rblk.as<int32_t>(), d_rbase.as<uint32_t>());
                     k_pr_final_multi<T><<<(unsigned)count, kWave, 0, s>>>(d_table.as<IterArgs<T>>());
@@ -1370,8 +1511,7 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
             EGX_HIP_TRY(hipGraphLaunch(cap.exec, s), EGX_DEVICE_ERR);
         } else {
             for (int32_t q = 0; q < todo; ++q) {
-                k_pr_spmv_multi<T, W><<<(unsigned)total, kThreads, 0, s>>>(d_table.as<IterArgs<T>>(),
-                                                                          d_blk.as<int32_t>(), d_base.as<uint32_t>());
+                launch_spmv();
                 EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
                 k_pr_reduce_multi<T, W><<<(unsigned)rtotal, kReduceThreads, 0, s>>>(
                     d_table.as<IterArgs<T>>(), d_rblk.as<int32_t>(), d_rbase.as<uint32_t>());

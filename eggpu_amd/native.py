@@ -90,6 +90,7 @@ class BinOpts(ctypes.Structure):
         ("tile_sort", ctypes.c_int32),
         ("hot_sources", ctypes.c_int32),
         ("blocked", ctypes.c_int32),
+        ("blocked_shared", ctypes.c_int32),
     ]
 
 
@@ -390,12 +391,14 @@ class DeviceGraph:
         return int(self.info()["n"])
 
     def set_bins(self, thread_row_max=0, wave_row_max=0, hub_segment=0, tile_items=0, reorder=-1, tile_sort=-1,
-                 hot_sources=0, blocked=-1):
+                 hot_sources=0, blocked=-1, blocked_shared=-1):
         """Re-bin rows (values <= 0 keep the current setting; reorder,
-        tile_sort and blocked < 0 keep theirs; hot_sources 0 keeps it, < 0
-        turns non-temporal cold gathers off)."""
+        tile_sort, blocked and blocked_shared < 0 keep theirs; hot_sources 0
+        keeps it, < 0 turns non-temporal cold gathers off).  blocked_shared=1
+        sizes the blocked layout's work items for graphs solved together in
+        multi_pagerank (the largest items); 0 for a graph solved alone."""
         o = BinOpts(int(thread_row_max), int(wave_row_max), int(hub_segment), int(tile_items), int(reorder),
-                    int(tile_sort), int(hot_sources), int(blocked))
+                    int(tile_sort), int(hot_sources), int(blocked), int(blocked_shared))
         _check(lib().egx_graph_set_bins(self.handle, ctypes.byref(o)))
 
     def download_csc(self):

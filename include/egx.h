@@ -137,6 +137,11 @@ typedef struct {
                                iterates): sources are scattered by destination bin
                                through LDS, then summed per bin in LDS, with no
                                random gathers; 0: pull SpMV; < 0: keep setting */
+    int32_t blocked_shared; /* 1: size the blocked layout's scatter and sum items for
+                               graphs solved together (egx_multi_pagerank): the
+                               largest items, since the set fills the device; 0: size
+                               them so that the graph alone fills it (the default);
+                               < 0: keep setting */
 } egx_bin_opts;
 
 typedef struct {

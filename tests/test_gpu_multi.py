@@ -97,6 +97,39 @@ def test_multi_fixed_iterations_and_empty_graph(gpu_ready):
         assert it == 17 and np.array_equal(outs[i], ref)
 
 
+@pytest.mark.parametrize("shared", [0, 1])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_multi_blocked_members_equal_single(gpu_ready, weighted, shared):
+    """Members with the propagation-blocked layout run its scatter and sum
+    inside the shared launches (table-driven), the others their pull work
+    items: every member bit-identical to its solo solve, with the solo
+    blocked solves against the oracle."""
+    specs = _small_set(10, seed=21 + int(weighted), weighted=weighted)
+    graphs = [s.build(native) for s in specs]
+    blocked = [i for i in range(len(graphs)) if i % 3 != 1]
+    for i in blocked:
+        graphs[i].set_bins(blocked=1, blocked_shared=shared)
+    assert all(bool(graphs[i].info()["blocked"]) == (i in blocked) for i in range(len(graphs)))
+    _say(f"multi with blocked members {blocked} of {len(graphs)} weighted={weighted} shared={shared}")
+    for K, tol in ((20, 0.0), (100, 1e-6)):
+        alone = [g.pagerank(0.85, K, tol) for g in graphs]
+        for hip_graph in (False, True):
+            outs, iters, st = native.multi_pagerank(graphs, 0.85, K, tol, hip_graph=hip_graph)
+            for i in range(len(graphs)):
+                assert iters[i] == alone[i][1], (i, K, hip_graph)
+                assert np.array_equal(outs[i], alone[i][0]), (i, K, hip_graph)
+    for i in blocked[:3]:
+        V, E, W = _spec_csr(specs[i])
+        ref, it_ref, _ = O.pagerank_csr(V, E, W, 0.85, 100, 1e-6)
+        pr, it, _ = graphs[i].pagerank(0.85, 100, 1e-6)
+        assert it == it_ref
+        d = np.abs(pr - ref)
+        _say(f"blocked member {i} (n={len(V) - 1}, m={len(E)}) vs oracle: max-abs {d.max():.3e} L1 {d.sum():.3e}")
+        assert d.max() <= 1e-6 and d.sum() <= 1e-5
+    for g in graphs:
+        g.close()
+
+
 def test_multi_rejects_mixed_weighting(gpu_ready):
     a = native.DeviceGraph.er(1000, 8000, seed=1)
     b = native.DeviceGraph.er(1000, 8000, seed=2, weighted=True, wseed=3)
