@@ -20,6 +20,10 @@ import numpy as np
 from . import autotune, native, workloads
 
 
+# multi-graph solves replay captured HIP graphs unless --eager clears this
+_HIP_GRAPH = True
+
+
 def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -114,22 +118,24 @@ def multi_graph(count=512, K=20, seed=2024, plain_sample=512):
     # warm both launch orders over the whole set, then time them alternately
     # (best of two each), so neither gains from running after the other
     for xcd in (True, False):
-        native.multi_pagerank(graphs, max_iter=2, tol=0.0, download=False, xcd_placement=xcd)
+        native.multi_pagerank(graphs, max_iter=2, tol=0.0, download=False, xcd_placement=xcd,
+                              hip_graph=_HIP_GRAPH)
     walls = {True: [], False: []}
     st = None
     for _ in range(2):
         for xcd in (True, False):
             t0 = time.perf_counter()
-            _, iters, s = native.multi_pagerank(graphs, max_iter=K, tol=0.0, download=False, xcd_placement=xcd)
+            _, iters, s = native.multi_pagerank(graphs, max_iter=K, tol=0.0, download=False, xcd_placement=xcd,
+                                                hip_graph=_HIP_GRAPH)
             walls[xcd].append(time.perf_counter() - t0)
             assert (iters == K).all()
             if not xcd:
                 st = s
     wall = min(walls[False])  # the default launch order
     wall_xcd = min(walls[True])
-    native.multi_pagerank(graphs, max_iter=100, tol=1e-6, download=False)  # warm the captured chunk
+    native.multi_pagerank(graphs, max_iter=100, tol=1e-6, download=False, hip_graph=_HIP_GRAPH)  # warm
     t0 = time.perf_counter()
-    _, it_tol, _ = native.multi_pagerank(graphs, max_iter=100, tol=1e-6, download=False)
+    _, it_tol, _ = native.multi_pagerank(graphs, max_iter=100, tol=1e-6, download=False, hip_graph=_HIP_GRAPH)
     tol_wall = time.perf_counter() - t0
     for g, b in zip(graphs, tuned_blocked):
         g.set_bins(blocked=b, blocked_shared=0)
@@ -453,15 +459,41 @@ def serving(pool_size=32, requests=600, qps=SERVING_QPS, seed=77, plain_sample=2
 
 
 def main(argv=None):
-    """python -m eggpu_amd.benchmarks [batched_ppr|multi_graph|serving|katz|eigenvector|ingest] ... -- one JSON line each."""
+    """python -m eggpu_amd.benchmarks [batched_ppr|multi_graph|serving|katz|eigenvector|ingest] ... -- one JSON line each.
+
+    --eager runs batch sessions and multi-graph solves as plain launches
+    instead of replaying captured HIP graphs (for kernel traces and counter
+    runs); --calib-copy BYTES runs one device copy first (counter
+    calibration); --pmc-traffic FETCH_DIR WRITE_DIR --kernels a,b does no GPU
+    work and prints the HBM bytes per launch of those kernels from two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) made with --calib-copy."""
     import argparse
     import json
 
+    global _HIP_GRAPH
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("sections", nargs="*",
                     default=["batched_ppr", "multi_graph", "serving", "katz", "eigenvector", "ingest"])
+    ap.add_argument("--eager", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"))
+    ap.add_argument("--kernels", default="")
     args = ap.parse_args(argv)
+    if args.pmc_traffic:
+        from .profiling import pmc_traffic
+
+        out = pmc_traffic(args.pmc_traffic[0], args.pmc_traffic[1], args.calib_copy or (1 << 30),
+                          [k for k in args.kernels.split(",") if k])
+        print(json.dumps(out, indent=1), flush=True)
+        return
+    if args.eager:
+        os.environ["EGGPU_AMD_BATCH_GRAPHS"] = "0"
+        _HIP_GRAPH = False
     native.lib()
+    if args.calib_copy > 0:
+        from .profiling import calib_copy
+
+        calib_copy(args.calib_copy)
     for name in args.sections:
         if name not in ("batched_ppr", "multi_graph", "serving", "katz", "eigenvector", "ingest"):
             raise SystemExit(f"unknown section {name}")
