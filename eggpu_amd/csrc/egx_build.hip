@@ -15,6 +15,7 @@
 //   * out-weights are summed per row, sequentially in CSR order, in FP64, so
 //     fl32(alpha * (w / outW)) (:344) rounds exactly as the reference does.
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <vector>
@@ -504,6 +505,14 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     g.pb_scatter_size.clear();
     g.pb_sum_size.clear();
     if (!cfg.blocked || g.n <= 0 || g.m <= 0) return EGX_SUCC;
+    // test hook: a real device allocation failure at the start of the layout
+    // build, for the fallback and error-state tests
+    if (const char* e = std::getenv("EGGPU_AMD_TEST_FAIL_BLOCKED_ALLOC")) {
+        if (e[0] == '1') {
+            DevBuf huge;
+            EGX_TRY(huge.alloc((size_t)1 << 52));
+        }
+    }
     const int32_t n = g.n;
     const int64_t m = g.m;
     const int64_t S = (n + kPbChunk - 1) / kPbChunk;

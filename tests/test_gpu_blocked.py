@@ -214,6 +214,31 @@ def test_blocked_default_policy(gpu_ready):
     w.close()
 
 
+def test_blocked_layout_allocation_failure_falls_back(gpu_ready, monkeypatch):
+    # a real device allocation failure inside the blocked-layout build (the
+    # test hook asks hipMalloc for 4 PB): a graph that would start blocked
+    # keeps the pull layout and solves correctly, with no stale HIP error
+    # reaching its first launch; an explicit set_bins(blocked=1) reports the
+    # failure and leaves the graph usable; without the hook it builds
+    monkeypatch.setenv("EGGPU_AMD_TEST_FAIL_BLOCKED_ALLOC", "1")
+    g = native.DeviceGraph.rmat(20, 16, seed=2)
+    assert g.info()["m"] >= 1 << 23 and g.info()["blocked"] == 0
+    a, ia, _ = g.pagerank(0.85, 100, 1e-6)
+    with pytest.raises(native.EgxError) as ei:
+        g.set_bins(blocked=1)
+    assert ei.value.status == native.EGX_FAILED_TO_ALLOCATE_DEVICE_MEM
+    assert g.info()["blocked"] == 0
+    b, ib, _ = g.pagerank(0.85, 100, 1e-6)
+    assert ia == ib and np.array_equal(a, b)
+    monkeypatch.delenv("EGGPU_AMD_TEST_FAIL_BLOCKED_ALLOC")
+    g.set_bins(blocked=1)
+    assert g.info()["blocked"] == 1
+    c, ic, _ = g.pagerank(0.85, 100, 1e-6)
+    assert ic == ia
+    _assert_close(c, a, "rmat20 blocked after the failed attempt vs pull", tol=1e-9, l1=1e-7)
+    g.close()
+
+
 def test_blocked_graph_in_multi_graph_and_batch_paths(gpu_ready):
     # the multi-graph launch and the SpMM batch sessions keep their own
     # kernels on a graph whose plain solves run blocked; all agree with the
