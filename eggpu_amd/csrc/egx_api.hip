@@ -377,6 +377,7 @@ int egx_graph_info_get(const egx_graph* gh, egx_graph_info* out) {
 
 int egx_graph_set_bins(egx_graph* gh, const egx_bin_opts* opts) {
     if (!gh || !opts) return EGX_UNSUPPORTED_GRAPH;
+    egx::drop_ppr_batch(gh);  // egx_graph_ppr's kept session would block the re-bin
     Graph& g = gh->g;
     std::lock_guard<std::mutex> lk(g.mu);
     egx::DeviceGuard dg(g.device);

@@ -304,6 +304,11 @@ int solve_pagerank_multi(std::vector<Graph*>& gs, const SolveParams& p, double* 
 
 struct egx_graph {
     egx::Graph g;
+    // the batch session egx_graph_ppr keeps for its next call (its X buffers
+    // and captured step graphs); closed before a re-bin and with the graph
+    egx_batch* ppr_batch = nullptr;
+    std::mutex ppr_mu;  // one egx_graph_ppr call at a time on this graph
+    ~egx_graph();
 };
 
 namespace egx {
@@ -315,5 +320,8 @@ int make_graph(std::unique_ptr<egx_graph>& out);
 // null) and records `done` (may be null) after the copy.  host_out must stay
 // valid until `done` completes.
 int batch_result_enqueue(egx_batch* b, int32_t slot, double* host_out, hipEvent_t done);
+
+// Close the batch session egx_graph_ppr keeps on the handle, if any.
+void drop_ppr_batch(egx_graph* gh);
 }  // namespace egx
 

@@ -782,11 +782,17 @@ __global__ __launch_bounds__(kWave) void k_spmm_reduce_final(SpmmArgs a) {
 // Column j of the current X (row stride C) := 1/n (networkx starts
 // personalised runs from uniform).  The current buffer is read from ctl, so
 // the launch can be queued behind iterations that have not run yet.
-__global__ void k_spmm_init_col(float* __restrict__ x0, float* __restrict__ x1, const BatchCtl* __restrict__ ctl,
-                                int32_t n, int32_t stride, int32_t j) {
-    const int32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+// x0 = 1/n in every column of `cols` (bit j: column j) of the current X, one
+// thread per (row, column): a row's columns are consecutive lanes, so the
+// writes of one launch are coalesced however many columns it initialises.
+__global__ void k_spmm_init_cols(float* __restrict__ x0, float* __restrict__ x1, const BatchCtl* __restrict__ ctl,
+                                 int32_t n, int32_t stride, uint64_t cols) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * stride) return;
+    const int32_t c = (int32_t)(t % stride);
+    if (!((cols >> c) & 1ull)) return;
     float* x = ctl->cur ? x1 : x0;
-    if (v < n) x[(size_t)v * stride + j] = (float)(1.0 / (double)n);
+    x[t] = (float)(1.0 / (double)n);
 }
 
 // A queued change of the batch state (teleport entries, slot table entries,
@@ -902,6 +908,9 @@ struct Batch {
     uint32_t nblk_h = 0, grid = 0, n_waves = 0;
     int64_t n_hrows = 0;
     int32_t ptab_cap = 0;
+    // columns assigned since X was last touched: their x0 is written by one
+    // launch before the next iteration, result read or width change
+    uint64_t init_pending = 0;
     // host mirror of the teleport table
     std::vector<std::vector<int32_t>> seeds;  // per column (iteration rows)
     std::vector<int32_t> host_tslot_rows;     // rows that own a table slot
@@ -1130,8 +1139,20 @@ int32_t width_for(int32_t need) {
 // Re-lay X for `want` columns per row: the current iterate is repacked into
 // the other buffer (or into new, larger buffers) and becomes current.  The
 // stream is idle afterwards.
+// Write x0 into the columns assigned since the last flush (one launch).
+int flush_init(Batch& b) {
+    if (!b.init_pending) return EGX_SUCC;
+    const int64_t total = (int64_t)b.n * b.width;
+    k_spmm_init_cols<<<nblk(total, 256), 256, 0, b.stream>>>(b.X[0].as<float>(), b.X[1].as<float>(),
+                                                              b.ctl.as<BatchCtl>(), b.n, b.width, b.init_pending);
+    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    b.init_pending = 0;
+    return EGX_SUCC;
+}
+
 int batch_set_width(Batch& b, int32_t want) {
     if (want == b.width) return EGX_SUCC;
+    EGX_TRY(flush_init(b));
     EGX_TRY(batch_poll(b));
     hipStream_t s = b.stream;
     const int64_t total = (int64_t)b.n * want;
@@ -1245,9 +1266,7 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
     u.h.cs = cs;
     u.h.set_any = 1;
     EGX_TRY(batch_queue_update(b, u));
-    k_spmm_init_col<<<nblk(g.n, 256), 256, 0, s>>>(b.X[0].as<float>(), b.X[1].as<float>(), b.ctl.as<BatchCtl>(),
-                                                    g.n, b.width, j);
-    EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+    b.init_pending |= 1ull << j;  // x0 is written with the other new columns' (flush_init)
     return EGX_SUCC;
 }
 
@@ -1304,6 +1323,7 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
     // narrow the launches when the high slots have been released
     const int32_t want = batch_needed_width(b);
     if (want != b.width) EGX_TRY(batch_set_width(b, want));
+    EGX_TRY(flush_init(b));
     SpmmArgs a{};
     a.row = g.it_row;
     a.col = g.it_col;
@@ -1435,6 +1455,7 @@ int batch_result(Batch& b, int32_t j, double* host_out) {
         set_last_error("batch slot holds no request");
         return EGX_UNSUPPORTED_GRAPH;
     }
+    EGX_TRY(flush_init(b));
     k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[0].as<float>(), b.X[1].as<float>(), b.ctl.as<BatchCtl>(),
                                                   b.width, b.cols.as<ColState>(),
                                                   g.reordered ? g.perm.as<int32_t>() : nullptr, g.n, j,
@@ -1459,6 +1480,18 @@ int batch_elapsed(Batch& b, double* seconds) {
 struct egx_batch {
     std::unique_ptr<egx::Batch> b;
 };
+
+egx_graph::~egx_graph() {
+    if (ppr_batch) egx_batch_close(ppr_batch);
+}
+
+void egx::drop_ppr_batch(egx_graph* gh) {
+    std::lock_guard<std::mutex> lk(gh->ppr_mu);
+    if (gh->ppr_batch) {
+        egx_batch_close(gh->ppr_batch);
+        gh->ppr_batch = nullptr;
+    }
+}
 
 namespace {
 
@@ -1510,6 +1543,7 @@ int egx::batch_result_enqueue(egx_batch* bh, int32_t j, double* host_out, hipEve
         return EGX_UNSUPPORTED_GRAPH;
     }
     // b.out is reused by the next result on this stream only after this copy
+    EGX_TRY(flush_init(b));
     k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[0].as<float>(), b.X[1].as<float>(), b.ctl.as<BatchCtl>(),
                                                   b.width, b.cols.as<ColState>(),
                                                   g.reordered ? g.perm.as<int32_t>() : nullptr, g.n, j,
@@ -1623,12 +1657,27 @@ int egx_graph_ppr(egx_graph* gh, int32_t k, const int32_t* seed_offsets, const i
         egx::set_last_error("egx_graph_ppr needs 1 <= k <= 64 columns and seed_offsets[k+1]");
         return EGX_UNSUPPORTED_GRAPH;
     }
-    egx_batch* bh = nullptr;
-    EGX_TRY(egx_batch_open(gh, &bh));
-    std::unique_ptr<egx_batch, void (*)(egx_batch*)> guard(bh, egx_batch_close);
+    // the session of the previous call is reused (its buffers, merge-path
+    // split and captured step graphs); a failed call drops it
+    std::lock_guard<std::mutex> lk(gh->ppr_mu);
+    if (!gh->ppr_batch) EGX_TRY(egx_batch_open(gh, &gh->ppr_batch));
+    egx_batch* bh = gh->ppr_batch;
+    struct DropOnError {
+        egx_graph* gh;
+        bool ok = false;
+        ~DropOnError() {
+            if (!ok && gh->ppr_batch) {
+                egx_batch_close(gh->ppr_batch);
+                gh->ppr_batch = nullptr;
+            }
+        }
+    } drop{gh};
     {
         BatchDevice dg(bh->b->device);
         if (!dg.ok) return EGX_DEVICE_ERR;
+        // columns past k from a previous call leave the session
+        for (int32_t j = k; j < egx::kCols; ++j)
+            if (bh->b->host_cols[(size_t)j].active) EGX_TRY(egx::batch_release(*bh->b, j));
         EGX_TRY(egx::batch_set_width(*bh->b, egx::width_for(k)));  // one layout for all k columns
     }
     for (int32_t j = 0; j < k; ++j) {
@@ -1644,6 +1693,7 @@ int egx_graph_ppr(egx_graph* gh, int32_t k, const int32_t* seed_offsets, const i
         if (iterations) iterations[j] = it[(size_t)j];
         if (PR) EGX_TRY(egx_batch_result(bh, j, PR + (size_t)j * gh->g.n));
     }
+    drop.ok = true;
     return EGX_SUCC;
 }
 

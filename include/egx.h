@@ -322,7 +322,10 @@ void egx_batch_close(egx_batch *b);
 
 /* One-shot: k columns, column j seeded by seeds[seed_offsets[j] .. seed_offsets[j+1])
  * (an empty range = global PageRank), alpha_j = alphas ? alphas[j] : alpha.
- * PR (k x n, row j = column j) and iterations (k) may be NULL. */
+ * PR (k x n, row j = column j) and iterations (k) may be NULL.  The batch
+ * session it runs in stays on the handle for the next call (its X buffers,
+ * 8*n*width bytes, and captured step graphs); egx_graph_set_bins and
+ * egx_graph_free close it.  Calls on one handle run one at a time. */
 int egx_graph_ppr(egx_graph *g, int32_t k, const int32_t *seed_offsets, const int32_t *seeds,
                   const double *alphas, double alpha, int32_t max_iter, double tol, double *PR,
                   int32_t *iterations, double *kernel_seconds);

@@ -155,6 +155,34 @@ def test_column_bits_do_not_depend_on_width(gpu_ready, slot):
         assert np.array_equal(b.result(slot), ref)
 
 
+def test_one_shot_ppr_reuses_its_session(gpu_ready):
+    """egx_graph_ppr keeps its batch session on the handle: calls with 64,
+    3 and 10 columns in a row (the later ones narrowing the X layout and
+    leaving the high slots behind) give the bits of the same calls on
+    fresh handles; a re-bin closes the kept session, and the next call
+    opens a new one."""
+    V, E = O.rmat_csr(12, 16, seed=23)
+    W = O.edge_weights(V, E, 4)
+    n = len(V) - 1
+    rng = np.random.default_rng(5)
+    sets = [sorted(rng.choice(n, int(rng.integers(1, 9)), replace=False).tolist()) for _ in range(64)]
+    sets[5] = []
+    g = native.DeviceGraph.from_csr(V, E, W)
+    for cols, mi, tol in ((sets, 40, 0.0), (sets[:3], 60, 1e-7), (sets[10:20], 40, 1e-6)):
+        out, it, _ = g.ppr(cols, alpha=0.85, max_iter=mi, tol=tol)
+        fresh = native.DeviceGraph.from_csr(V, E, W)
+        ref, it_ref, _ = fresh.ppr(cols, alpha=0.85, max_iter=mi, tol=tol)
+        fresh.close()
+        assert np.array_equal(it, it_ref) and np.array_equal(out, ref), len(cols)
+    g.set_bins(tile_sort=1)
+    out, it, _ = g.ppr(sets[:4], alpha=0.9, max_iter=50, tol=1e-6)
+    for j in range(4):
+        ref, it_ref, _ = O.ppr_rows(n, *O.csr_rows(V, E), W, _p(n, sets[j]), 0.9, 50, 1e-6)
+        assert it[j] == it_ref
+        _close(out[j], ref, f"one-shot ppr after a re-bin, column {j}")
+    g.close()
+
+
 def test_graph_replay_equals_eager_launches(gpu_ready, monkeypatch):
     """Steps replayed from a captured hipGraph give the bits of eager launches,
     replay after replay, with host work, re-assignment and a width change
