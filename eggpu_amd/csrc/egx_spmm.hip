@@ -1686,7 +1686,9 @@ int egx_graph_ppr(egx_graph* gh, int32_t k, const int32_t* seed_offsets, const i
         EGX_TRY(egx_batch_assign(bh, j, seeds ? seeds + b0 : nullptr, b1 - b0, alphas ? alphas[j] : alpha,
                                  max_iter, tol));
     }
-    EGX_TRY(egx_batch_run(bh, std::max(1, max_iter), 0, kernel_seconds));
+    // at tol 0 every column runs max_iter iterations: one completion check
+    const int32_t launches = std::max(1, max_iter);
+    EGX_TRY(egx_batch_run(bh, launches, tol > 0.0 ? 0 : launches, kernel_seconds));
     std::vector<int32_t> it(egx::kCols);
     EGX_TRY(egx_batch_poll(bh, nullptr, nullptr, it.data(), nullptr));
     for (int32_t j = 0; j < k; ++j) {
