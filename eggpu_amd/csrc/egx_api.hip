@@ -31,6 +31,53 @@ struct DeviceGuard {
     }
 };
 
+// Streams of freed graphs and finished multi-graph solves, kept per device
+// for the next ones: creating a HIP stream costs more than the rest of a
+// small graph's set-up.  At most kIdleStreams idle streams per device are
+// kept; the pool lives until the process ends.
+constexpr size_t kIdleStreams = 16;
+struct StreamPool {
+    std::mutex mu;
+    std::vector<std::pair<int, hipStream_t>> idle;
+};
+StreamPool& stream_pool() {
+    static StreamPool* p = new StreamPool();
+    return *p;
+}
+
+// A stream on the current device `dev`: an idle one, else a new one.
+hipError_t take_stream(int dev, hipStream_t* s) {
+    {
+        StreamPool& p = stream_pool();
+        std::lock_guard<std::mutex> lk(p.mu);
+        for (size_t i = p.idle.size(); i-- > 0;) {
+            if (p.idle[i].first == dev) {
+                *s = p.idle[i].second;
+                p.idle.erase(p.idle.begin() + (std::ptrdiff_t)i);
+                return hipSuccess;
+            }
+        }
+    }
+    return hipStreamCreate(s);
+}
+
+// Return stream s of device dev (current) once its work is done.
+void give_stream(int dev, hipStream_t s) {
+    if (!s) return;
+    (void)hipStreamSynchronize(s);
+    StreamPool& p = stream_pool();
+    {
+        std::lock_guard<std::mutex> lk(p.mu);
+        size_t on_dev = 0;
+        for (const auto& e : p.idle) on_dev += e.first == dev ? 1 : 0;
+        if (on_dev < kIdleStreams) {
+            p.idle.emplace_back(dev, s);
+            return;
+        }
+    }
+    (void)hipStreamDestroy(s);
+}
+
 int new_graph(std::unique_ptr<egx_graph>& out) {
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
@@ -41,7 +88,7 @@ int new_graph(std::unique_ptr<egx_graph>& out) {
     if (!h) return EGX_FAILED_TO_ALLOCATE_HOST_MEM;
     Graph& g = h->g;
     g.device = dev;
-    EGX_HIP_TRY(hipStreamCreate(&g.stream), EGX_DEVICE_ERR);
+    EGX_HIP_TRY(take_stream(dev, &g.stream), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipEventCreate(&g.ev0), EGX_DEVICE_ERR);
     EGX_HIP_TRY(hipEventCreate(&g.ev1), EGX_DEVICE_ERR);
     // the pinned slot a solve stages its state through, made here so that no
@@ -247,7 +294,7 @@ Graph::~Graph() {
     if (host_state) (void)hipHostFree(host_state);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    give_stream(device, stream);
 }
 
 size_t Graph::device_bytes() const {
@@ -542,7 +589,7 @@ int egx_multi_pagerank(egx_graph* const* graphs, int32_t count, const egx_solve_
     hipStream_t s = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int rc = EGX_SUCC;
-    if (hipStreamCreate(&s) != hipSuccess || hipEventCreate(&ev0) != hipSuccess ||
+    if (egx::take_stream(gs[0]->device, &s) != hipSuccess || hipEventCreate(&ev0) != hipSuccess ||
         hipEventCreate(&ev1) != hipSuccess) {
         egx::set_last_error("egx_multi_pagerank: stream/event creation failed");
         rc = EGX_DEVICE_ERR;
@@ -553,7 +600,7 @@ int egx_multi_pagerank(egx_graph* const* graphs, int32_t count, const egx_solve_
     if (s) (void)hipStreamSynchronize(s);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (s) (void)hipStreamDestroy(s);
+    egx::give_stream(gs[0]->device, s);
     copy_stats(st, stats);
     return rc;
 }

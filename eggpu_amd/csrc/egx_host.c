@@ -18,6 +18,9 @@
  *     Sorts every row's columns ascending (W along), stably, in place, as
  *     Graph::gen_CSR does (graph.cpp:855-878); rows already in order are
  *     only checked.
+ *   identity(nodes) -> bool
+ *     True when the list's i-th item is the int i for every i (exact int
+ *     type), so that a node is its own row index.
  *   stamp(adj) -> (fingerprint, arcs)
  *     A 64-bit fingerprint of the node keys' hashes, their order, their
  *     degrees and their first neighbours' hashes (splitmix64 finaliser over
@@ -306,7 +309,29 @@ static PyObject* stamp(PyObject* self, PyObject* args) {
     return Py_BuildValue("(Kn)", (unsigned long long)fp, arcs_total);
 }
 
+/* identity(nodes) -> bool: nodes is a list whose i-th item is the int i
+ * (exact int type, bool excluded), so that every node is its own index. */
+static PyObject* identity(PyObject* self, PyObject* args) {
+    (void)self;
+    PyObject* nodes;
+    if (!PyArg_ParseTuple(args, "O", &nodes)) return NULL;
+    if (!PyList_Check(nodes)) {
+        PyErr_SetString(PyExc_TypeError, "identity: nodes must be a list");
+        return NULL;
+    }
+    const Py_ssize_t n = PyList_GET_SIZE(nodes);
+    for (Py_ssize_t i = 0; i < n; ++i) {
+        PyObject* v = PyList_GET_ITEM(nodes, i);
+        if (!PyLong_CheckExact(v)) Py_RETURN_FALSE;
+        int overflow = 0;
+        const long long x = PyLong_AsLongLongAndOverflow(v, &overflow);
+        if (overflow || x != (long long)i) Py_RETURN_FALSE;
+    }
+    Py_RETURN_TRUE;
+}
+
 static PyMethodDef methods[] = {
+    {"identity", identity, METH_VARARGS, "nodes are exactly the ints 0..n-1 in order"},
     {"arcs", arcs, METH_VARARGS, "adjacency dicts -> out-CSR arrays"},
     {"sort_rows", sort_rows, METH_VARARGS, "sort each CSR row's columns ascending, in place"},
     {"stamp", stamp, METH_VARARGS, "fingerprint of node keys, order, degrees, first neighbours; arc count"},

@@ -274,7 +274,10 @@ def _identity_labels(prepared):
     flag = prepared.get("identity_labels")
     if flag is None:
         nodes = prepared["nodes"]
-        flag = set(map(type, nodes)) <= {int} and nodes == list(range(len(nodes)))
+        if _host is not None and type(nodes) is list:
+            flag = bool(_host.identity(nodes))
+        else:
+            flag = set(map(type, nodes)) <= {int} and nodes == list(range(len(nodes)))
         prepared["identity_labels"] = flag
     return flag
 
