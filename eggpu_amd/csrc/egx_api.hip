@@ -292,6 +292,7 @@ Graph::~Graph() {
     if (device >= 0) (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     if (host_state) (void)hipHostFree(host_state);
+    if (host_result) (void)hipHostFree(host_result);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     give_stream(device, stream);

@@ -209,6 +209,9 @@ struct Graph {
     DevBuf x[2], s[2], state, partials, red_part, out;
     int64_t partial_blocks = 0;
     void* host_state = nullptr;  // pinned PrState
+    // pinned staging of a packed PageRank result (kPackedHeader + 4 n bytes),
+    // made on the first download and kept
+    char* host_result = nullptr;
 
     ~Graph();
     size_t device_bytes() const;
@@ -315,11 +318,29 @@ namespace egx {
 // A new, empty handle bound to the current device, with its stream and events.
 int make_graph(std::unique_ptr<egx_graph>& out);
 
-// egx_batch_result without the wait: queues the normalised result of `slot`
-// on the batch's stream, copies it to host_out (pinned, n doubles; may be
-// null) and records `done` (may be null) after the copy.  host_out must stay
-// valid until `done` completes.
-int batch_result_enqueue(egx_batch* b, int32_t slot, double* host_out, hipEvent_t done);
+// egx_batch_result without the wait, and with half the copy: queues the
+// packed result of `slot` on the batch's stream -- the column's FP64 sum s,
+// then its FP32 iterate in caller order (n floats) -- copies it to host_out
+// (pinned, at least kPackedHeader + 4 n bytes; may be null) and records
+// `done` (may be null) after the copy.  unpack_result turns it into the
+// doubles egx_batch_result returns, bit for bit.  host_out must stay valid
+// until `done` completes.
+constexpr size_t kPackedHeader = 16;  // the sum, padded so the floats start 16-byte aligned
+int batch_result_enqueue(egx_batch* b, int32_t slot, void* host_out, hipEvent_t done);
+
+// out[v] = (double)x[v] * (1.0 / s) if s > 0, else 1.0 / n: the arithmetic
+// of the device's FP64 output (k_spmm_output), on a packed result.
+inline void unpack_result(const void* packed, int32_t n, double* out) {
+    const double s = *static_cast<const double*>(packed);
+    const float* x = reinterpret_cast<const float*>(static_cast<const char*>(packed) + kPackedHeader);
+    if (s > 0.0) {
+        const double inv = 1.0 / s;
+        for (int32_t v = 0; v < n; ++v) out[v] = (double)x[v] * inv;
+    } else {
+        const double u = 1.0 / (double)n;
+        for (int32_t v = 0; v < n; ++v) out[v] = u;
+    }
+}
 
 // Close the batch session egx_graph_ppr keeps on the handle, if any.
 void drop_ppr_batch(egx_graph* gh);

@@ -1128,6 +1128,21 @@ int prepare_graph(Graph& g, const SolveParams& p, hipStream_t s, IterArgs<T>& a)
     return EGX_SUCC;
 }
 
+// The packed form of the FP32 PageRank result: the iterate's sum (FP64) in
+// the header, then the FP32 iterate in caller order, half the bytes of the
+// FP64 result; unpack_result finishes it on the host with k_pr_output's
+// arithmetic, bit for bit.
+__global__ void k_pr_output_packed(const float* __restrict__ x0, const float* __restrict__ x1,
+                                   const PrState* __restrict__ st, const int32_t* __restrict__ perm, int32_t n,
+                                   char* __restrict__ out) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) *reinterpret_cast<double*>(out) = st->sum;
+    if (r >= n) return;
+    const float* x = (st->iter & 1) ? x1 : x0;
+    const int32_t v = perm ? perm[r] : r;
+    reinterpret_cast<float*>(out + kPackedHeader)[v] = x[r];
+}
+
 // Writes the result of graph g to a.out (and copies it to host_out):
 // PageRank's normalised iterate, Katz's unnormalised previous iterate, or
 // the eigenvector iterate.
@@ -1262,6 +1277,24 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
                 st->reduce_seconds += (double)b_ms * 1e-3;
             }
             st->spmv_launches = iters;
+        }
+    }
+    if constexpr (std::is_same<T, float>::value) {
+        // a PageRank result for the host only crosses PCIe packed (FP32
+        // iterate and its sum) and is finished here; a.out, the graph's own
+        // buffer of n >= 4 doubles, has room for the packed form
+        if (host_out && p.out_device == nullptr && p.bias == nullptr && p.eig_delta == 0 && g.n >= 4) {
+            const size_t bytes = kPackedHeader + (size_t)g.n * 4;
+            if (!g.host_result)
+                EGX_HIP_TRY(hipHostMalloc((void**)&g.host_result, bytes), EGX_FAILED_TO_ALLOCATE_HOST_MEM);
+            k_pr_output_packed<<<nblk(g.n, 256), 256, 0, s>>>(a.xb[0], a.xb[1], a.state,
+                                                             g.reordered ? g.perm.as<int32_t>() : nullptr, g.n,
+                                                             reinterpret_cast<char*>(a.out));
+            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
+            EGX_HIP_TRY(hipMemcpyAsync(g.host_result, a.out, bytes, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+            EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+            unpack_result(g.host_result, g.n, host_out);
+            return EGX_SUCC;
         }
     }
     EGX_TRY(emit_result<T>(g, a, s, host_out, p.bias != nullptr, p.eig_delta != 0));

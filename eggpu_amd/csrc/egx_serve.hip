@@ -50,7 +50,7 @@ double seconds_since(Clock::time_point t0, Clock::time_point t) {
 }
 
 constexpr int kSlots = 64;
-constexpr int kDeliveryThreads = 4;
+constexpr int kDeliveryThreads = 8;
 
 struct Request {
     int64_t id = 0;
@@ -93,7 +93,7 @@ int64_t session_bytes(const egx_graph_info& inf) {
 // A finished column on its way to the callers' buffers.
 struct Delivery {
     std::shared_ptr<Request> r;
-    double* staged = nullptr;  // pinned copy of the result (null: nobody asked for it)
+    char* staged = nullptr;    // pinned packed copy of the result (null: nobody asked for it)
     hipEvent_t ev = nullptr;   // recorded after the copy into `staged`
 };
 
@@ -124,8 +124,8 @@ struct egx_server {
     std::condition_variable cv_deliver;
     bool deliver_stop = false;
     std::vector<std::thread> deliverers;
-    std::vector<std::vector<double*>> staged_free;  // per graph: pinned buffers of n doubles
-    std::vector<double*> staged_all;
+    std::vector<std::vector<char*>> staged_free;  // per graph: pinned buffers of one packed result
+    std::vector<char*> staged_all;
     std::vector<hipEvent_t> events_free, events_all;
 
     ~egx_server() {
@@ -145,11 +145,11 @@ struct egx_server {
         for (Session& s : sessions)
             if (s.b) egx_batch_close(s.b);
         for (hipEvent_t e : events_all) (void)hipEventDestroy(e);
-        for (double* p : staged_all) (void)hipHostFree(p);
+        for (char* p : staged_all) (void)hipHostFree(p);
     }
 
     // A pinned buffer for one result of `graph` and an event, from the pools.
-    int take_staging(int32_t graph, double** buf, hipEvent_t* ev) {
+    int take_staging(int32_t graph, char** buf, hipEvent_t* ev) {
         *buf = nullptr;
         *ev = nullptr;
         {
@@ -166,9 +166,9 @@ struct egx_server {
         }
         if (!*buf) {
             void* p = nullptr;
-            const size_t bytes = (size_t)info[(size_t)graph].n * sizeof(double);
+            const size_t bytes = egx::kPackedHeader + (size_t)info[(size_t)graph].n * sizeof(float);
             if (hipHostMalloc(&p, bytes) != hipSuccess) return EGX_FAILED_TO_ALLOCATE_HOST_MEM;
-            *buf = static_cast<double*>(p);
+            *buf = static_cast<char*>(p);
             std::lock_guard<std::mutex> lk(mu);
             staged_all.push_back(*buf);
         }
@@ -184,14 +184,15 @@ struct egx_server {
         return EGX_SUCC;
     }
 
-    void give_back(int32_t graph, double* buf, hipEvent_t ev) {
+    void give_back(int32_t graph, char* buf, hipEvent_t ev) {
         std::lock_guard<std::mutex> lk(mu);
         if (buf) staged_free[(size_t)graph].push_back(buf);
         if (ev) events_free.push_back(ev);
     }
 
-    // Delivery thread: wait for a staged result, copy it to the request's
-    // buffer and to those of the requests riding on its column, complete them.
+    // Delivery thread: wait for a staged (packed) result, finish its FP64
+    // values into the request's buffer and copy them to those of the
+    // requests riding on its column, complete them.
     void deliver_loop() {
         (void)hipSetDevice(device);
         for (;;) {
@@ -212,10 +213,21 @@ struct egx_server {
                 }
             }
             if (st == EGX_SUCC && d.staged) {
-                const size_t bytes = (size_t)info[(size_t)d.r->graph].n * sizeof(double);
-                if (d.r->out) std::memcpy(d.r->out, d.staged, bytes);
-                for (auto& q : d.r->riders)
-                    if (q->out) std::memcpy(q->out, d.staged, bytes);
+                const int32_t n = info[(size_t)d.r->graph].n;
+                const double* first = nullptr;
+                if (d.r->out) {
+                    egx::unpack_result(d.staged, n, d.r->out);
+                    first = d.r->out;
+                }
+                for (auto& q : d.r->riders) {
+                    if (!q->out) continue;
+                    if (first) {
+                        std::memcpy(q->out, first, (size_t)n * sizeof(double));
+                    } else {
+                        egx::unpack_result(d.staged, n, q->out);
+                        first = q->out;
+                    }
+                }
             }
             give_back(d.r->graph, d.staged, d.ev);
             finish(d.r, st);

@@ -847,6 +847,20 @@ __global__ void k_spmm_output(const float* __restrict__ x0, const float* __restr
     out[v] = (s > 0.0) ? (double)x[(size_t)r * stride + j] * (1.0 / s) : 1.0 / (double)n;
 }
 
+// The packed form of column j's result: its sum (FP64) in the header, then
+// its FP32 iterate in caller order; the host finishes the FP64 values.
+__global__ void k_spmm_output_packed(const float* __restrict__ x0, const float* __restrict__ x1,
+                                     const BatchCtl* __restrict__ ctl, int32_t stride,
+                                     const ColState* __restrict__ cols, const int32_t* perm, int32_t n, int32_t j,
+                                     char* __restrict__ out) {
+    const int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0) *reinterpret_cast<double*>(out) = cols[j].sum;
+    if (r >= n) return;
+    const float* x = ctl->cur ? x1 : x0;
+    const int32_t v = perm ? perm[r] : r;
+    reinterpret_cast<float*>(out + kPackedHeader)[v] = x[(size_t)r * stride + j];
+}
+
 __global__ void k_set_tslot(int32_t* __restrict__ tslot, const int32_t* __restrict__ rows,
                             const int32_t* __restrict__ slots, int32_t count) {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1069,7 +1083,7 @@ int batch_open(Graph& g, std::unique_ptr<Batch>& out) {
     // reduce parts: about 64 hub rows + partial slots each
     b->n_red = (uint32_t)std::min<int64_t>(1024, std::max<int64_t>(1, (b->n_hrows + b->grid + 63) / 64));
     EGX_TRY(b->red_part.alloc((size_t)b->n_red * 3 * kCols * 8));
-    EGX_TRY(b->out.alloc((size_t)n * 8));
+    EGX_TRY(b->out.alloc(std::max((size_t)n * 8, kPackedHeader + (size_t)n * 4)));  // FP64 or packed result
     EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
     out = std::move(b);
     return EGX_SUCC;
@@ -1530,7 +1544,7 @@ int run_until_done(egx::Batch& b, int32_t max_launches, int32_t check_every, dou
 
 }  // namespace
 
-int egx::batch_result_enqueue(egx_batch* bh, int32_t j, double* host_out, hipEvent_t done) {
+int egx::batch_result_enqueue(egx_batch* bh, int32_t j, void* host_out, hipEvent_t done) {
     using namespace egx;
     if (!bh || !bh->b) return EGX_UNSUPPORTED_GRAPH;
     BatchDevice dg(bh->b->device);
@@ -1544,13 +1558,18 @@ int egx::batch_result_enqueue(egx_batch* bh, int32_t j, double* host_out, hipEve
     }
     // b.out is reused by the next result on this stream only after this copy
     EGX_TRY(flush_init(b));
-    k_spmm_output<<<nblk(g.n, 256), 256, 0, s>>>(b.X[0].as<float>(), b.X[1].as<float>(), b.ctl.as<BatchCtl>(),
-                                                  b.width, b.cols.as<ColState>(),
-                                                  g.reordered ? g.perm.as<int32_t>() : nullptr, g.n, j,
-                                                  b.out.as<double>());
+    // b.out holds n doubles: room for the header and n floats (n >= 1)
+    const size_t bytes = kPackedHeader + (size_t)g.n * 4;
+    if (bytes > b.out.bytes) {
+        set_last_error("batch result buffer too small for the packed result");
+        return EGX_UNSUPPORTED_GRAPH;
+    }
+    k_spmm_output_packed<<<nblk(g.n, 256), 256, 0, s>>>(b.X[0].as<float>(), b.X[1].as<float>(),
+                                                         b.ctl.as<BatchCtl>(), b.width, b.cols.as<ColState>(),
+                                                         g.reordered ? g.perm.as<int32_t>() : nullptr, g.n, j,
+                                                         b.out.as<char>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-    if (host_out)
-        EGX_HIP_TRY(hipMemcpyAsync(host_out, b.out.p, (size_t)g.n * 8, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
+    if (host_out) EGX_HIP_TRY(hipMemcpyAsync(host_out, b.out.p, bytes, hipMemcpyDeviceToHost, s), EGX_DEVICE_ERR);
     if (done) EGX_HIP_TRY(hipEventRecord(done, s), EGX_DEVICE_ERR);
     return EGX_SUCC;
 }
