@@ -91,6 +91,26 @@ def test_results_alone_equal_results_in_the_mix(pool):
     assert st["peak_active_columns"] > 1  # requests did share the device
 
 
+def test_packed_delivery_equals_device_output(pool):
+    """The server copies a finished column packed (its FP32 iterate and FP64
+    sum) and finishes the FP64 values on the host; egx_batch_result finishes
+    them on the device (k_spmm_output).  Same bits, weighted and unweighted,
+    global and personalised."""
+    graphs, _ = pool
+    reqs = [(0, (), 0.85), (1, (), 0.9), (2, (3, 17, 40), 0.85), (1, (5,), 0.85), (0, tuple(range(0, 64, 4)), 0.9)]
+    with native.Server(graphs) as srv:
+        served = [srv.wait(srv.submit(gi, s, alpha=a, max_iter=100, tol=1e-6)) for gi, s, a in reqs]
+    for (gi, s, a), (out, info) in zip(reqs, served):
+        with graphs[gi].batch() as b:
+            b.assign(0, s, alpha=a, max_iter=100, tol=1e-6)
+            b.run(max_launches=100)
+            ref = b.result(0)
+            it = int(b.poll()["iterations"][0])
+        assert info["iterations"] == it, (gi, s, a)
+        assert np.array_equal(out.view(np.uint64), ref.view(np.uint64)), (gi, s, a, np.abs(out - ref).max())
+    _say(f"{len(reqs)} served results equal the device-finished ones bit for bit")
+
+
 def test_served_results_match_oracle(pool):
     graphs, csr = pool
     with native.Server(graphs) as srv:
