@@ -342,6 +342,25 @@ inline void unpack_result(const void* packed, int32_t n, double* out) {
     }
 }
 
+// One request's assignment for batch_assign_many.
+struct BatchAssign {
+    int32_t slot;
+    const int32_t* seeds;  // caller node ids (null: global PageRank)
+    int32_t nseeds;
+    double alpha;
+    int32_t max_iter;
+    double tol;
+};
+
+// Assign several slots of a batch in one update (one copy and one launch)
+// after widening the batch once, to its highest slot.  status[i] receives
+// assignment i's code; one that fails its checks (slot range, seed range)
+// is skipped and changes nothing.  Returns a code for the update itself.
+int batch_assign_many(egx_batch* b, int32_t count, const BatchAssign* reqs, int* status);
+
+// Release several slots in one update.
+int batch_release_many(egx_batch* b, int32_t count, const int32_t* slots);
+
 // Close the batch session egx_graph_ppr keeps on the handle, if any.
 void drop_ppr_batch(egx_graph* gh);
 }  // namespace egx

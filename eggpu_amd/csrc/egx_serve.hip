@@ -313,6 +313,11 @@ struct egx_server {
             pending.clear();
         }
         std::vector<std::shared_ptr<Request>> keep;
+        struct Picks {
+            Session* s;
+            std::vector<std::pair<int32_t, std::shared_ptr<Request>>> slots;
+        };
+        std::vector<Picks> picks;
         for (auto& r : take) {
             Session* s = session_for(r->graph);
             if (!s) {
@@ -348,16 +353,38 @@ struct egx_server {
                 keep.push_back(r);
                 continue;
             }
-            int rc = egx_batch_assign(s->b, j, r->seeds.empty() ? nullptr : r->seeds.data(), (int32_t)r->seeds.size(),
-                                      r->alpha, r->max_iter, r->tol);
-            if (rc != EGX_SUCC) {
-                finish(r, rc);
-                continue;
-            }
-            r->t_start = Clock::now();
+            // the slot is taken (busy: the session cannot be closed) until the
+            // session's assignments of this pass are made together below
             s->slot[(size_t)j] = r;
             s->busy += 1;
-            s->last_used = r->t_start;
+            auto pick = std::find_if(picks.begin(), picks.end(), [&](const Picks& p) { return p.s == s; });
+            if (pick == picks.end()) pick = picks.insert(picks.end(), Picks{s, {}});
+            pick->slots.emplace_back(j, r);
+        }
+        // one update per session: one copy and one launch for all its new columns
+        for (Picks& p : picks) {
+            std::vector<egx::BatchAssign> reqs;
+            reqs.reserve(p.slots.size());
+            for (const auto& [j, r] : p.slots)
+                reqs.push_back(egx::BatchAssign{j, r->seeds.empty() ? nullptr : r->seeds.data(),
+                                                (int32_t)r->seeds.size(), r->alpha, r->max_iter, r->tol});
+            std::vector<int> st(reqs.size(), EGX_SUCC);
+            const int rc = egx::batch_assign_many(p.s->b, (int32_t)reqs.size(), reqs.data(), st.data());
+            const Clock::time_point now = Clock::now();
+            for (size_t i = 0; i < p.slots.size(); ++i) {
+                const auto& [j, r] = p.slots[i];
+                const int code = (rc != EGX_SUCC && st[i] == EGX_SUCC) ? rc : st[i];
+                if (code != EGX_SUCC) {
+                    p.s->slot[(size_t)j] = nullptr;
+                    p.s->busy -= 1;
+                    finish(r, code);  // and the requests that rode on it
+                    continue;
+                }
+                r->t_start = now;
+                p.s->last_used = now;
+            }
+        }
+        if (!picks.empty()) {
             const int64_t act = active_columns();
             std::lock_guard<std::mutex> lk(mu);
             stats.peak_active_columns = std::max<int64_t>(stats.peak_active_columns, act);
@@ -392,6 +419,7 @@ struct egx_server {
             if (s.slot[(size_t)j]) hi = j + 1;
         int32_t width = 8;
         while (width < hi) width <<= 1;
+        std::vector<std::pair<int32_t, Delivery>> finished;
         for (int32_t j = 0; j < kSlots; ++j) {
             auto& r = s.slot[(size_t)j];
             if (!r || !done[j]) continue;
@@ -406,23 +434,39 @@ struct egx_server {
                 st = take_staging(s.graph, &d.staged, &d.ev);
                 if (st == EGX_SUCC) st = egx::batch_result_enqueue(s.b, j, d.staged, d.ev);
             }
-            // stream order: the release applies after the result has been read
-            if (st == EGX_SUCC) st = egx_batch_release(s.b, j);
             if (st == EGX_SUCC) {
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    deliveries.push_back(std::move(d));
-                }
-                cv_deliver.notify_one();
+                finished.emplace_back(j, std::move(d));
             } else {
-                if (d.ev) (void)hipEventSynchronize(d.ev);  // the copy may be queued: let it land first
-                give_back(s.graph, d.staged, d.ev);
-                finish(r, st);
+                drop_delivery(s, d, st);
             }
             r = nullptr;
             s.busy -= 1;
             s.last_used = Clock::now();
         }
+        if (finished.empty()) return;
+        // one update releases every finished column (stream order: after
+        // their results have been read)
+        std::vector<int32_t> js;
+        js.reserve(finished.size());
+        for (const auto& f : finished) js.push_back(f.first);
+        const int st = egx::batch_release_many(s.b, (int32_t)js.size(), js.data());
+        if (st != EGX_SUCC) {
+            for (auto& f : finished) drop_delivery(s, f.second, st);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto& f : finished) deliveries.push_back(std::move(f.second));
+        }
+        cv_deliver.notify_all();
+    }
+
+    // A result that will not be delivered: its copy lands first, its buffer
+    // goes back to the pool, and its request fails with `status`.
+    void drop_delivery(Session& s, Delivery& d, int status) {
+        if (d.ev) (void)hipEventSynchronize(d.ev);
+        give_back(s.graph, d.staged, d.ev);
+        finish(d.r, status);
     }
 
     // Sessions progress independently: a session whose step has finished is

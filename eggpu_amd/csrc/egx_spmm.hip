@@ -796,13 +796,14 @@ __global__ void k_spmm_init_cols(float* __restrict__ x0, float* __restrict__ x1,
 }
 
 // A queued change of the batch state (teleport entries, slot table entries,
-// one column state, the any-active flag), applied in stream order.
+// column states, the any-active flag), applied in stream order.  One update
+// may carry the assignments or releases of many columns: one copy and one
+// launch for all of them.
 struct BatchUpdate {
     int32_t n_ptab;       // ptab[p_idx[i]] = p_val[i]
     int32_t n_tslot;      // tslot[t_idx[i]] = t_val[i]
-    int32_t col;          // >= 0: cols[col] = cs
+    int32_t n_cols;       // cols[col_idx[i]] = cs[i]
     int32_t set_any;      // 1: ctl->any_active = 1
-    ColState cs;
 };
 constexpr int kMaxUpdateEntries = 256;
 struct BatchUpdateBuf {
@@ -811,17 +812,19 @@ struct BatchUpdateBuf {
     float p_val[kMaxUpdateEntries];
     int32_t t_idx[kMaxUpdateEntries];
     int32_t t_val[kMaxUpdateEntries];
+    int32_t col_idx[kCols];
+    ColState cs[kCols];  // last: a copy stops after the n_cols in use
 };
 
 // Entries are applied in list order by one thread: a list may write one
-// address twice (a slot freed and re-used within one update), and the later
-// entry must win.
+// address twice (a slot freed and re-used within one update, a column
+// released and assigned again), and the later entry must win.
 __global__ void k_batch_update(const BatchUpdateBuf* __restrict__ u, float* __restrict__ ptab,
                                int32_t* __restrict__ tslot, ColState* __restrict__ cols, BatchCtl* __restrict__ ctl) {
     if (threadIdx.x != 0) return;
     for (int i = 0; i < u->h.n_ptab; ++i) ptab[u->p_idx[i]] = u->p_val[i];
     for (int i = 0; i < u->h.n_tslot; ++i) tslot[u->t_idx[i]] = u->t_val[i];
-    if (u->h.col >= 0) cols[u->h.col] = u->h.cs;
+    for (int i = 0; i < u->h.n_cols; ++i) cols[u->col_idx[i]] = u->cs[i];
     if (u->h.set_any) ctl->any_active = 1;
 }
 
@@ -938,28 +941,37 @@ struct Batch {
     size_t arena_used = 0;
     DevBuf upd;               // device copy of the update being applied
     double last_kernel_seconds = 0.0;
-    // a captured run of `step_iters` iterations at `step_width`, replayed by
-    // later steps of the same shape (kernel arguments never change between
-    // iterations; the buffer parity lives on the device)
+    // captured runs of `iters` iterations, one per width (8, 16, 32, 64),
+    // replayed by later steps of the same shape: a session narrows and
+    // widens as its columns finish and new ones arrive, and a capture costs
+    // far more than a replay.  The kernel arguments depend only on the width
+    // and the X buffers (the buffer parity lives on the device).
     bool use_graph = true;
-    hipGraph_t step_graph = nullptr;
-    hipGraphExec_t step_exec = nullptr;
-    int32_t step_width = 0, step_iters = 0;
-    const void* step_x0 = nullptr;
-    const void* step_x1 = nullptr;
+    struct StepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int32_t iters = 0;
+        const void* x0 = nullptr;
+        const void* x1 = nullptr;
+    };
+    StepGraph steps[4];
 
-    void drop_step_graph() {
-        if (step_exec) (void)hipGraphExecDestroy(step_exec);
-        if (step_graph) (void)hipGraphDestroy(step_graph);
-        step_exec = nullptr;
-        step_graph = nullptr;
-        step_width = step_iters = 0;
+    static int width_slot(int32_t w) { return w <= 8 ? 0 : w <= 16 ? 1 : w <= 32 ? 2 : 3; }
+
+    void drop_step_graph(StepGraph& e) {
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+        if (e.graph) (void)hipGraphDestroy(e.graph);
+        e = StepGraph{};
+    }
+    // every captured step (the stream must be idle: a replay may be running)
+    void drop_step_graphs() {
+        for (StepGraph& e : steps) drop_step_graph(e);
     }
 
     ~Batch() {
         if (device >= 0) (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
-        drop_step_graph();
+        drop_step_graphs();
         if (pinned) (void)hipHostFree(pinned);
         if (arena) (void)hipHostFree(arena);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1095,7 +1107,7 @@ int batch_poll(Batch& b);
 // staging arena wraps).
 int batch_queue_update(Batch& b, const BatchUpdateBuf& u) {
     hipStream_t s = b.stream;
-    const size_t bytes = sizeof(BatchUpdateBuf);
+    const size_t bytes = offsetof(BatchUpdateBuf, cs) + (size_t)u.h.n_cols * sizeof(ColState);
     if (b.arena_used + bytes > kArenaBytes) {
         EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);  // earlier copies out of the arena are done
         b.arena_used = 0;
@@ -1110,14 +1122,32 @@ int batch_queue_update(Batch& b, const BatchUpdateBuf& u) {
     return EGX_SUCC;
 }
 
-// Record ptab[t][j] = v in the host mirror and in update u.
-// Apply what u holds so far and start it over (when an entry list is full).
+// Apply what u holds so far and start it over (when a list is full).  The
+// parts land back to back in stream order, before any later step.
 int flush_update(Batch& b, BatchUpdateBuf& u) {
     EGX_TRY(batch_queue_update(b, u));
     u.h.n_ptab = 0;
     u.h.n_tslot = 0;
+    u.h.n_cols = 0;
     return EGX_SUCC;
 }
+
+// Queue u if it holds anything.
+int finish_update(Batch& b, BatchUpdateBuf& u) {
+    if (u.h.n_ptab == 0 && u.h.n_tslot == 0 && u.h.n_cols == 0 && !u.h.set_any) return EGX_SUCC;
+    return batch_queue_update(b, u);
+}
+
+// Record cols[j] = cs in update u.
+int stage_col(Batch& b, BatchUpdateBuf& u, int32_t j, const ColState& cs) {
+    if (u.h.n_cols >= kCols) EGX_TRY(flush_update(b, u));
+    u.col_idx[u.h.n_cols] = j;
+    u.cs[u.h.n_cols] = cs;
+    u.h.n_cols += 1;
+    return EGX_SUCC;
+}
+
+// Record ptab[t][j] = v in the host mirror and in update u.
 
 int stage_ptab(Batch& b, BatchUpdateBuf& u, int32_t t, int32_t j, float v) {
     if (u.h.n_ptab >= kMaxUpdateEntries) EGX_TRY(flush_update(b, u));
@@ -1138,8 +1168,7 @@ int stage_tslot(Batch& b, BatchUpdateBuf& u, int32_t row, int32_t t) {
 
 BatchUpdateBuf empty_update() {
     BatchUpdateBuf u;
-    std::memset(&u, 0, sizeof(u));
-    u.h.col = -1;
+    std::memset(&u.h, 0, sizeof(u.h));
     return u;
 }
 
@@ -1178,6 +1207,7 @@ int batch_set_width(Batch& b, int32_t want) {
         k_spmm_repack<<<nblk(total, 256), 256, 0, s>>>(b.X[b.cur].as<float>(), b.width, nx[0].as<float>(), want, b.n);
         EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
         EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+        b.drop_step_graphs();  // they address the old X buffers (and the stream is idle)
         b.X[0] = std::move(nx[0]);
         b.X[1] = std::move(nx[1]);
         b.cap_cols = want;
@@ -1204,26 +1234,26 @@ int32_t batch_needed_width(const Batch& b) {
     return width_for(need);
 }
 
-// Assign column j: seeds (original node ids) or none (global PageRank).
-// Queued in stream order behind any iterations already enqueued: the host
-// waits only when the X layout has to widen.
-int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, double alpha, int32_t max_iter,
-                 double tol) {
+// The checks of an assignment that come before any state changes.
+int check_assign(const Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds) {
     if (j < 0 || j >= kCols) {
         set_last_error("batch slot out of range [0, 64)");
         return EGX_UNSUPPORTED_GRAPH;
     }
-    Graph& g = *b.g;
-    hipStream_t s = b.stream;
     if (nseeds > 0 && !seeds) return EGX_UNSUPPORTED_GRAPH;
     for (int32_t i = 0; i < nseeds; ++i) {
-        if (seeds[i] < 0 || seeds[i] >= g.n) {
+        if (seeds[i] < 0 || seeds[i] >= b.g->n) {
             set_last_error("seed node out of range");
             return EGX_UNSUPPORTED_GRAPH;
         }
     }
-    if (b.width <= j) EGX_TRY(batch_set_width(b, width_for(j + 1)));
-    BatchUpdateBuf u = empty_update();
+    return EGX_SUCC;
+}
+
+// Column j's assignment staged in update u (checked, within the width).
+int assign_into(Batch& b, BatchUpdateBuf& u, int32_t j, const int32_t* seeds, int32_t nseeds, double alpha,
+                int32_t max_iter, double tol) {
+    Graph& g = *b.g;
     // drop the old seeds of column j from the table
     for (int32_t v : b.seeds[(size_t)j]) {
         const int32_t t = b.slot_of_row(v);
@@ -1276,17 +1306,28 @@ int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, doub
     const float x0 = (float)(1.0 / (double)g.n);
     cs.dang = (double)g.n_dangling * (double)x0;
     b.host_cols[(size_t)j] = cs;
-    u.h.col = j;
-    u.h.cs = cs;
+    EGX_TRY(stage_col(b, u, j, cs));
     u.h.set_any = 1;
-    EGX_TRY(batch_queue_update(b, u));
     b.init_pending |= 1ull << j;  // x0 is written with the other new columns' (flush_init)
     return EGX_SUCC;
 }
 
-int batch_release(Batch& b, int32_t j) {
-    if (j < 0 || j >= kCols) return EGX_UNSUPPORTED_GRAPH;
+// Assign column j: seeds (original node ids) or none (global PageRank).
+// Queued in stream order behind any iterations already enqueued: the host
+// waits only when the X layout has to widen.
+int batch_assign(Batch& b, int32_t j, const int32_t* seeds, int32_t nseeds, double alpha, int32_t max_iter,
+                 double tol) {
+    EGX_TRY(check_assign(b, j, seeds, nseeds));
+    if (b.width <= j) EGX_TRY(batch_set_width(b, width_for(j + 1)));
     BatchUpdateBuf u = empty_update();
+    EGX_TRY(assign_into(b, u, j, seeds, nseeds, alpha, max_iter, tol));
+    return finish_update(b, u);
+}
+
+// Column j's release staged in update u: its seeds leave the table and it
+// goes inactive.
+int release_into(Batch& b, BatchUpdateBuf& u, int32_t j) {
+    if (j < 0 || j >= kCols) return EGX_UNSUPPORTED_GRAPH;
     for (int32_t v : b.seeds[(size_t)j]) {
         const int32_t t = b.slot_of_row(v);
         if (t >= 0) EGX_TRY(stage_ptab(b, u, t, j, 0.0f));
@@ -1295,9 +1336,13 @@ int batch_release(Batch& b, int32_t j) {
     ColState cs = b.host_cols[(size_t)j];
     cs.active = 0;
     b.host_cols[(size_t)j].active = 0;
-    u.h.col = j;
-    u.h.cs = cs;
-    return batch_queue_update(b, u);
+    return stage_col(b, u, j, cs);
+}
+
+int batch_release(Batch& b, int32_t j) {
+    BatchUpdateBuf u = empty_update();
+    EGX_TRY(release_into(b, u, j));
+    return finish_update(b, u);
 }
 
 int enqueue_iterations(Batch& b, const SpmmArgs& a, int32_t iters, hipStream_t s);
@@ -1363,11 +1408,11 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
     a.xbuf[0] = b.X[0].as<float>();
     a.xbuf[1] = b.X[1].as<float>();
     if (b.use_graph && iters >= 2) {
-        if (!(b.step_exec && b.step_width == b.width && b.step_iters == iters && b.step_x0 == b.X[0].p &&
-              b.step_x1 == b.X[1].p)) {
-            // the previous replay may still be running: let it finish before its exec goes away
-            if (b.step_exec) EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
-            b.drop_step_graph();
+        Batch::StepGraph& sg = b.steps[Batch::width_slot(b.width)];
+        if (!(sg.exec && sg.iters == iters && sg.x0 == b.X[0].p && sg.x1 == b.X[1].p)) {
+            // a replay of this exec may still be running: let it finish before the exec goes away
+            if (sg.exec) EGX_HIP_TRY(hipStreamSynchronize(s), EGX_DEVICE_ERR);
+            b.drop_step_graph(sg);
             // captured on the device's capture stream, replayed on `s` below
             CaptureStreams& cap = capture_streams();
             std::unique_lock<std::mutex> lk(cap.mu);
@@ -1383,15 +1428,14 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
                 return rc;
             }
             EGX_HIP_TRY(e, EGX_DEVICE_ERR);
-            b.step_graph = graph;
-            EGX_HIP_TRY(hipGraphInstantiate(&b.step_exec, b.step_graph, nullptr, nullptr, 0), EGX_DEVICE_ERR);
-            b.step_width = b.width;
-            b.step_iters = iters;
-            b.step_x0 = b.X[0].p;
-            b.step_x1 = b.X[1].p;
+            sg.graph = graph;
+            EGX_HIP_TRY(hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0), EGX_DEVICE_ERR);
+            sg.iters = iters;
+            sg.x0 = b.X[0].p;
+            sg.x1 = b.X[1].p;
         }
         if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev0, s), EGX_DEVICE_ERR);
-        EGX_HIP_TRY(hipGraphLaunch(b.step_exec, s), EGX_DEVICE_ERR);
+        EGX_HIP_TRY(hipGraphLaunch(sg.exec, s), EGX_DEVICE_ERR);
         if (record_time) EGX_HIP_TRY(hipEventRecord(b.ev1, s), EGX_DEVICE_ERR);
         return EGX_SUCC;
     }
@@ -1574,6 +1618,43 @@ int egx::batch_result_enqueue(egx_batch* bh, int32_t j, void* host_out, hipEvent
     return EGX_SUCC;
 }
 
+int egx::batch_assign_many(egx_batch* bh, int32_t count, const BatchAssign* reqs, int* status) {
+    using namespace egx;
+    if (!bh || !bh->b || count < 0 || (count > 0 && (!reqs || !status))) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    Batch& b = *bh->b;
+    int32_t hi = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const BatchAssign& r = reqs[i];
+        status[i] = check_assign(b, r.slot, r.seeds, r.nseeds);
+        if (status[i] == EGX_SUCC) hi = std::max(hi, r.slot + 1);
+    }
+    if (hi > b.width) EGX_TRY(batch_set_width(b, width_for(hi)));
+    BatchUpdateBuf u = empty_update();
+    for (int32_t i = 0; i < count; ++i) {
+        if (status[i] != EGX_SUCC) continue;
+        const BatchAssign& r = reqs[i];
+        status[i] = assign_into(b, u, r.slot, r.seeds, r.nseeds, r.alpha, r.max_iter, r.tol);
+        if (status[i] == EGX_DEVICE_ERR) {
+            for (int32_t k = i + 1; k < count; ++k) status[k] = EGX_DEVICE_ERR;
+            return EGX_DEVICE_ERR;
+        }
+    }
+    return finish_update(b, u);
+}
+
+int egx::batch_release_many(egx_batch* bh, int32_t count, const int32_t* slots) {
+    using namespace egx;
+    if (!bh || !bh->b || count < 0 || (count > 0 && !slots)) return EGX_UNSUPPORTED_GRAPH;
+    BatchDevice dg(bh->b->device);
+    if (!dg.ok) return EGX_DEVICE_ERR;
+    Batch& b = *bh->b;
+    BatchUpdateBuf u = empty_update();
+    for (int32_t i = 0; i < count; ++i) EGX_TRY(release_into(b, u, slots[i]));
+    return finish_update(b, u);
+}
+
 extern "C" {
 
 int egx_batch_open(egx_graph* gh, egx_batch** out) {
@@ -1695,16 +1776,23 @@ int egx_graph_ppr(egx_graph* gh, int32_t k, const int32_t* seed_offsets, const i
         BatchDevice dg(bh->b->device);
         if (!dg.ok) return EGX_DEVICE_ERR;
         // columns past k from a previous call leave the session
+        std::vector<int32_t> gone;
         for (int32_t j = k; j < egx::kCols; ++j)
-            if (bh->b->host_cols[(size_t)j].active) EGX_TRY(egx::batch_release(*bh->b, j));
+            if (bh->b->host_cols[(size_t)j].active) gone.push_back(j);
+        if (!gone.empty()) EGX_TRY(egx::batch_release_many(bh, (int32_t)gone.size(), gone.data()));
         EGX_TRY(egx::batch_set_width(*bh->b, egx::width_for(k)));  // one layout for all k columns
     }
+    // the k columns in one update
+    std::vector<egx::BatchAssign> reqs((size_t)k);
     for (int32_t j = 0; j < k; ++j) {
         const int32_t b0 = seed_offsets[j], b1 = seed_offsets[j + 1];
         if (b1 < b0) return EGX_UNSUPPORTED_GRAPH;
-        EGX_TRY(egx_batch_assign(bh, j, seeds ? seeds + b0 : nullptr, b1 - b0, alphas ? alphas[j] : alpha,
-                                 max_iter, tol));
+        reqs[(size_t)j] = egx::BatchAssign{j, seeds ? seeds + b0 : nullptr, b1 - b0, alphas ? alphas[j] : alpha,
+                                           max_iter, tol};
     }
+    std::vector<int> st((size_t)k, EGX_SUCC);
+    EGX_TRY(egx::batch_assign_many(bh, k, reqs.data(), st.data()));
+    for (int32_t j = 0; j < k; ++j) EGX_TRY(st[(size_t)j]);
     // at tol 0 every column runs max_iter iterations: one completion check
     const int32_t launches = std::max(1, max_iter);
     EGX_TRY(egx_batch_run(bh, launches, tol > 0.0 ? 0 : launches, kernel_seconds));
