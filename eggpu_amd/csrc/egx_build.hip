@@ -470,6 +470,10 @@ __global__ void k_pb_groups(const uint32_t* __restrict__ soff, const uint32_t* _
 // 256K-slot scatter and 128K-slot sum items -> these sizes): 0.125 -> 0.047,
 // 0.163 -> 0.083, 0.243 -> 0.184, 0.609 -> 0.611.
 constexpr int64_t kPbScatterItems = 1024;
+// weighted graphs: their scatter keeps 2 groups per lane per batch, and
+// shorter items balance better (R-MAT 24: 0.65-0.67 ms per pair at 2048-4096
+// items against 0.70 at 1024; unweighted within noise)
+constexpr int64_t kPbScatterItemsW = 4096;
 constexpr int64_t kPbScatterMin = 32768;
 constexpr int64_t kPbSumItems = 512;
 constexpr int64_t kPbSumMin = 16384;
@@ -627,7 +631,8 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
 
     // scatter work: contiguous group ranges of at most scat_slots slots per chunk
     // (a graph laid out for shared launches takes the largest items: the set fills the device)
-    const int32_t scat_groups = (int32_t)(pb_item_slots(slots, cfg.blocked_shared ? 1 : kPbScatterItems,
+    const int64_t scat_items = g.weighted ? kPbScatterItemsW : kPbScatterItems;
+    const int32_t scat_groups = (int32_t)(pb_item_slots(slots, cfg.blocked_shared ? 1 : scat_items,
                                                         kPbScatterMin, kPbScatterItem) /
                                           kPbVec);
     const int32_t sum_slots =
