@@ -377,11 +377,16 @@ __global__ void k_pb_gather_w(const int32_t* __restrict__ arc, const float* __re
     w[p] = (e >= 0 && (int64_t)e < m) ? val[e] : 0.0f;
 }
 
-// Stretch mask of every kPbVec-slot group (bit j: slot j starts a stretch of
-// equal rows; padding slots, row -1, join the stretch before them) and its
-// value-slot count; entry `groups` of pc is 0 for the scan total.
+// Stretch mask of every kPbVec-slot group and its term count; entry
+// `groups` of pc is 0 for the scan total.  Bit j >= 1: slot j starts a
+// stretch of equal rows (padding slots, row -1, join the stretch before
+// them).  Bit 0: the group's last stretch is a term of its own; with
+// `merge` (weighted graphs) it is clear when the next group of the same
+// kPbWindow-group window of the chunk starts with the same row, and the
+// stretch's sum then goes into the next group's first term.
 static_assert(kPbVec == 8, "a group is two int4 loads of rows");
-__global__ void k_pb_masks(const int32_t* __restrict__ srow, int64_t groups, uint8_t* __restrict__ mask,
+__global__ void k_pb_masks(const int32_t* __restrict__ srow, int64_t groups, const uint32_t* __restrict__ soff,
+                           int64_t S, int64_t B, bool merge, uint8_t* __restrict__ mask,
                            uint32_t* __restrict__ pc) {
     const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gi > groups) return;
@@ -392,10 +397,29 @@ __global__ void k_pb_masks(const int32_t* __restrict__ srow, int64_t groups, uin
     const int4 lo = reinterpret_cast<const int4*>(srow)[2 * gi];
     const int4 hi = reinterpret_cast<const int4*>(srow)[2 * gi + 1];
     const int32_t r[kPbVec] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-    uint32_t mk = 1;
+    uint32_t mk = 0;
+    int32_t last = r[0];  // the group's last real row
 #pragma unroll
-    for (int j = 1; j < kPbVec; ++j)
+    for (int j = 1; j < kPbVec; ++j) {
         if (r[j] >= 0 && r[j] != r[j - 1]) mk |= 1u << j;
+        if (r[j] >= 0) last = r[j];
+    }
+    bool cont = false;
+    const int64_t nx = gi + 1;
+    if (merge && nx < groups && last >= 0) {
+        // the chunk of the group: the last c with soff[c * B] <= its first slot
+        const int64_t s0 = gi * kPbVec;
+        int64_t lo_c = 0, hi_c = S;
+        while (lo_c + 1 < hi_c) {
+            const int64_t mid = (lo_c + hi_c) >> 1;
+            if ((int64_t)soff[mid * B] <= s0) lo_c = mid;
+            else hi_c = mid;
+        }
+        const int64_t c0 = (int64_t)soff[lo_c * B] / kPbVec;
+        const int64_t c1 = (int64_t)soff[(lo_c + 1) * B] / kPbVec;
+        if (nx < c1 && (nx - c0) % kPbWindow != 0) cont = srow[nx * kPbVec] == last;
+    }
+    if (!cont) mk |= 1u;
     mask[gi] = (uint8_t)mk;
     pc[gi] = (uint32_t)__popc(mk);
 }
@@ -440,15 +464,19 @@ __global__ void k_pb_groups(const uint32_t* __restrict__ soff, const uint32_t* _
     const int64_t v = (k % B) * S + k / B;
     const int64_t o = (int64_t)voff[v] + gs[gi] - gs[soff[k] / kPbVec];
     const uint32_t mk = mask[gi];
-    if (!(mk & 1u) || o + __popc(mk) > (int64_t)voff[v + 1]) {
+    if (o + __popc(mk) > (int64_t)voff[v + 1]) {
         *bad = 1;
         return;
     }
     vgrp[gi] = (int32_t)o;
     const int64_t r0 = (k % B) * kPbRows;
+    // the rows of the group's terms: its stretches (slot 0 and every mask
+    // bit j >= 1) in order, the first popcount(mk) of them (the last stretch
+    // has none when bit 0 is clear)
     int64_t q = o;
-    for (int j = 0; j < kPbVec; ++j)
-        if (mk & (1u << j)) {
+    const int64_t end = o + __popc(mk);
+    for (int j = 0; j < kPbVec && q < end; ++j)
+        if (j == 0 || (mk & (1u << j))) {
             const int64_t d = (int64_t)srow[s0 + j] - r0;
             if (d < 0 || d >= kPbRows) {
                 *bad = 1;
@@ -597,8 +625,10 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     DevBuf gs, voff;
     EGX_TRY(g.pb_mask.alloc((size_t)groups));
     EGX_TRY(gs.alloc((size_t)(groups + 1) * 4));
-    k_pb_masks<<<nblocks(groups + 1, tb), tb, 0, s>>>(srow.as<int32_t>(), groups, g.pb_mask.as<uint8_t>(),
-                                                      gs.as<uint32_t>());
+    // merged stretches for the weighted kernels (the solve's weight choice)
+    const bool merge = g.weighted && (!g.adjacency || g.adjacency_weights);
+    k_pb_masks<<<nblocks(groups + 1, tb), tb, 0, s>>>(srow.as<int32_t>(), groups, soff.as<uint32_t>(), S, B, merge,
+                                                      g.pb_mask.as<uint8_t>(), gs.as<uint32_t>());
     EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
     EGX_TRY(exclusive_scan_u32(gs.as<uint32_t>(), gs.as<uint32_t>(), groups + 1, s));
     EGX_TRY(voff.alloc((size_t)(runs + 1) * 4));
@@ -632,9 +662,10 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     // scatter work: contiguous group ranges of at most scat_slots slots per chunk
     // (a graph laid out for shared launches takes the largest items: the set fills the device)
     const int64_t scat_items = g.weighted ? kPbScatterItemsW : kPbScatterItems;
-    const int32_t scat_groups = (int32_t)(pb_item_slots(slots, cfg.blocked_shared ? 1 : scat_items,
-                                                        kPbScatterMin, kPbScatterItem) /
-                                          kPbVec);
+    // whole windows: an item starts on its chunk's window grid
+    const int32_t scat_groups = std::max<int32_t>(
+        kPbWindow, (int32_t)(pb_item_slots(slots, cfg.blocked_shared ? 1 : scat_items, kPbScatterMin, kPbScatterItem) /
+                             kPbVec / kPbWindow * kPbWindow));
     const int32_t sum_slots =
         (int32_t)pb_item_slots(nvals, cfg.blocked_shared ? 1 : kPbSumItems, kPbSumMin, kPbItem);
     std::vector<PbScatter> scat;
@@ -712,6 +743,7 @@ int build_blocked(Graph& g, const BinCfg& cfg, bool layout_changed) {
     g.pb_sum_size.resize(sums.size());
     for (size_t i = 0; i < sums.size(); ++i) g.pb_sum_size[i] = sums[i].e1 - sums[i].e0;
     g.pb_shared = cfg.blocked_shared;
+    g.pb_merged = merge;
     g.pb_ready = true;
     return EGX_SUCC;
 }

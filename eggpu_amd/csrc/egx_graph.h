@@ -110,6 +110,12 @@ struct BinCfg {
 constexpr int32_t kPbChunk = 32768;
 constexpr int32_t kPbRows = 2048;
 constexpr int32_t kPbVec = 8;
+// Groups per scatter window: a wavefront's 64 lanes take 64 consecutive
+// groups; in the layouts of weighted graphs a row's stretch may run on
+// across the groups of one window (one term), never across windows.
+// Windows start at the chunk's first group plus a multiple of kPbWindow
+// (scatter items are cut on that grid).
+constexpr int32_t kPbWindow = 64;
 constexpr int32_t kPbScatterItem = 262144;
 constexpr int32_t kPbItem = 131072;
 // PageRank graphs with at least this many arcs start with bins.blocked = 1
@@ -162,6 +168,9 @@ struct Graph {
     // per sum item), for ordering them in multi-graph launches
     std::vector<int32_t> pb_scatter_size, pb_sum_size;
     int32_t pb_shared = -1;  // item sizing the blocked layout was built with (BinCfg::blocked_shared)
+    // the blocked layout merges stretches across groups (weighted solves);
+    // only the kernels built for that merge read it
+    bool pb_merged = false;
     double alpha_vals = 0.0;   // alpha baked into it_val
     double alpha_orig = 0.0;   // alpha baked into in_val
     int32_t max_in_degree = 0;

@@ -433,27 +433,45 @@ __device__ __forceinline__ int32_t wave_excl_scan_small(uint32_t x, int32_t& tot
     return off;
 }
 
-// The stretch terms of one group (mask bit j: a stretch starts at slot j),
-// written to o[0], o[1], ...  Weighted (W): slot j's term is the FP32
-// product of its arc value and the staged x, as in the pull kernel.
-template <bool W, typename P>
-__device__ __forceinline__ void pb_emit(const float* sc, pb_u32x4 u, pb_f32x4 wa, pb_f32x4 wb, uint32_t mk, P* o) {
+// The kPbVec staged sources of one group, times their arc values when
+// weighted (W: the FP32 product, as in the pull kernel).
+template <bool W>
+__device__ __forceinline__ void pb_group_values(const float* sc, pb_u32x4 u, pb_f32x4 wa, pb_f32x4 wb,
+                                                float (&v)[kPbVec]) {
     const uint32_t w[4] = {u.x, u.y, u.z, u.w};
-    const float wt[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
-    float v[8];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
         v[2 * h] = sc[w[h] & 0xffffu];
         v[2 * h + 1] = sc[w[h] >> 16];
     }
     if (W) {
+        const float wt[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
 #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = wt[j] * v[j];
+        for (int k = 0; k < kPbVec; ++k) v[k] = wt[k] * v[k];
     }
-    double run = (double)v[0];
-    int32_t k = 0;
+}
+
+// FP64 sum of the group's last stretch (from the last of mask bits 1..7,
+// or slot 0, to the end).
+__device__ __forceinline__ double pb_tail(const float (&v)[kPbVec], uint32_t mk) {
+    double t = (double)v[0];
 #pragma unroll
-    for (int j = 1; j < 8; ++j) {
+    for (int j = 1; j < kPbVec; ++j) t = ((mk >> j) & 1u) ? (double)v[j] : t + (double)v[j];
+    return t;
+}
+
+// The terms of one group, written to o[0], o[1], ...: one per stretch
+// (mask bit j >= 1: a stretch starts at slot j), summed in FP64.  Merged
+// layouts (weighted graphs): the first stretch takes `carry`, the sum that
+// the groups before carried into its row, and the last stretch has no term
+// of its own when mask bit 0 is clear (it goes into the next group's first
+// term).  Otherwise bit 0 is always set and there is no carry.
+template <bool Merge, typename P>
+__device__ __forceinline__ void pb_emit(const float (&v)[kPbVec], uint32_t mk, double carry, P* o) {
+    int32_t k = 0;
+    double run = Merge ? carry + (double)v[0] : (double)v[0];
+#pragma unroll
+    for (int j = 1; j < kPbVec; ++j) {
         if ((mk >> j) & 1u) {
             o[k++] = (float)run;
             run = (double)v[j];
@@ -461,39 +479,68 @@ __device__ __forceinline__ void pb_emit(const float* sc, pb_u32x4 u, pb_f32x4 wa
             run += (double)v[j];
         }
     }
-    o[k] = (float)run;
+    if (!Merge || (mk & 1u)) o[k] = (float)run;
 }
 
 // The same terms, written without branches for the LDS staging area: every
 // slot writes the running sum of its stretch to the stretch's entry o[k], so
 // a stretch's entry is written once per slot and, since a lane's LDS writes
 // land in program order, ends with the stretch's full sum (the value pb_emit
-// writes).
-template <bool W>
-__device__ __forceinline__ void pb_emit_stage(const float* sc, pb_u32x4 u, pb_f32x4 wa, pb_f32x4 wb, uint32_t mk,
-                                              float* o) {
-    const uint32_t w[4] = {u.x, u.y, u.z, u.w};
-    const float wt[8] = {wa.x, wa.y, wa.z, wa.w, wb.x, wb.y, wb.z, wb.w};
-    float v[8];
-#pragma unroll
-    for (int h = 0; h < 4; ++h) {
-        v[2 * h] = sc[w[h] & 0xffffu];
-        v[2 * h + 1] = sc[w[h] >> 16];
-    }
-    if (W) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = wt[j] * v[j];
-    }
-    double run = (double)v[0];
-    o[0] = (float)run;
+// writes).  Entries past the group's popcount(mk) terms are never written.
+template <bool Merge>
+__device__ __forceinline__ void pb_emit_stage(const float (&v)[kPbVec], uint32_t mk, double carry, float* o) {
+    const int32_t nt = __builtin_popcount(mk);
     int32_t k = 0;
+    double run = Merge ? carry + (double)v[0] : (double)v[0];
+    if (!Merge || nt > 0) o[0] = (float)run;
 #pragma unroll
-    for (int j = 1; j < 8; ++j) {
+    for (int j = 1; j < kPbVec; ++j) {
         const uint32_t st = (mk >> j) & 1u;
         k += (int32_t)st;
         run = st ? (double)v[j] : run + (double)v[j];
-        o[k] = (float)run;
+        if (!Merge || k < nt) o[k] = (float)run;
     }
+}
+
+// A DPP move of a double (two 32-bit moves; lanes without a source keep 0).
+template <int Ctrl>
+__device__ __forceinline__ double pb_dpp(double x) {
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, Ctrl, 0xf, 0xf, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), Ctrl, 0xf, 0xf, true);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// One step of the chain scan: lanes with `take` compose their map with the
+// one DPP control Ctrl brings (S = a + b * S_before).
+template <int Ctrl>
+__device__ __forceinline__ void pb_chain_step(double& a, int& b, bool take) {
+    const double an = pb_dpp<Ctrl>(a);
+    const int bn = __builtin_amdgcn_mov_dpp(b, Ctrl, 0xf, 0xf, true);
+    if (take) {
+        a = b ? a + an : a;
+        b = b & bn;
+    }
+}
+
+// The stretch sums that flow right across a wavefront's 64 consecutive
+// groups (one window): lane l sends S_l = a_l + b_l * S_{l-1}, with a_l the
+// sum of its last stretch when that stretch continues into the next group
+// (mask bit 0 clear) and b_l = 1 when the whole group is that one stretch.
+// An inclusive scan of the affine maps by DPP row shifts and row broadcasts
+// (no LDS), in a fixed order; returns S_{l-1}, the carry lane l adds to its
+// first term (0 for lane 0).
+__device__ __forceinline__ double pb_window_carry(double a, bool bb, int l) {
+    int b = bb ? 1 : 0;
+    const int rl = l & 15;
+    pb_chain_step<0x111>(a, b, rl >= 1);  // row_shr:1
+    pb_chain_step<0x112>(a, b, rl >= 2);  // row_shr:2
+    pb_chain_step<0x114>(a, b, rl >= 4);  // row_shr:4
+    pb_chain_step<0x118>(a, b, rl >= 8);  // row_shr:8
+    pb_chain_step<0x142>(a, b, (l & 31) >= 16);  // row_bcast:15
+    pb_chain_step<0x143>(a, b, l >= 32);  // row_bcast:31
+    const double c = pb_dpp<0x138>(a);  // wave_shr:1
+    return l > 0 ? c : 0.0;
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -554,17 +601,23 @@ __device__ __forceinline__ void pb_scatter_batch(const PbScatterBatch<W>& cur, i
         const uint32_t mkq = ok ? cur.mk[q] : 0u;
         const pb_f32x4 waq = W ? cur.wa[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
         const pb_f32x4 wbq = W ? cur.wb[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
+        float v[kPbVec];
+        pb_group_values<W>(sc, cur.u[q], waq, wbq, v);
+        const bool goes_on = ok && !(mkq & 1u);
+        // weighted graphs merge stretches across the window (the unweighted
+        // kernel measured no faster for it: its registers are full)
+        const double carry = W ? pb_window_carry(goes_on ? pb_tail(v, mkq) : 0.0, goes_on && (mkq >> 1) == 0u, l) : 0.0;
         int32_t total = 0;
         const int32_t off = wave_excl_scan_small((uint32_t)__builtin_popcount(mkq), total);
         const int32_t first = __builtin_amdgcn_readfirstlane(vgq - off);  // lane 0: every lane runs here
         const bool contig = total > 0 && total <= kPbStage && __all(!ok || vgq - off == first);
         if (contig) {
-            if (ok) pb_emit_stage<W>(sc, cur.u[q], waq, wbq, mkq, stg + off);
+            if (ok) pb_emit_stage<W>(v, mkq, carry, stg + off);
             wave_lds_sync();
             for (int32_t j = l; j < total; j += kWave) vals[first + j] = stg[j];
             wave_lds_sync();
         } else if (ok) {
-            pb_emit<W>(sc, cur.u[q], waq, wbq, mkq, vals + vgq);
+            pb_emit<W>(v, mkq, carry, vals + vgq);
         }
     }
 }
@@ -1171,7 +1224,7 @@ int solve_typed(Graph& g, const SolveParams& p, double* host_out, SolveStats* st
     // blocked layout (weighted: with its per-slot arc values); split
     // sub-bins go through the hub-row reduce
     constexpr bool kBlockable = std::is_same<T, float>::value;
-    const bool blocked = kBlockable && g.pb_ready && (!W || g.pb_w.p != nullptr);
+    const bool blocked = kBlockable && g.pb_ready && (!W || g.pb_w.p != nullptr) && g.pb_merged == W;
     PbArgs pb{};
     if (blocked) {
         if (!p.bufs) {
@@ -1348,7 +1401,7 @@ int solve_multi_typed(std::vector<Graph*>& gs, const SolveParams& p, double* con
     if constexpr (kBlockable) {
         for (int32_t i = 0; i < count; ++i) {
             Graph& g = *gs[(size_t)i];
-            if (!g.pb_ready || (W && g.pb_w.p == nullptr)) continue;
+            if (!g.pb_ready || (W && g.pb_w.p == nullptr) || g.pb_merged != W) continue;
             is_pb[(size_t)i] = 1;
             IterArgs<T>& a = table[(size_t)i];
             total -= a.grid;
