@@ -502,25 +502,22 @@ __device__ __forceinline__ void pb_emit_stage(const float (&v)[kPbVec], uint32_t
     }
 }
 
-// A DPP move of a double (two 32-bit moves; lanes without a source keep 0).
-template <int Ctrl>
+// A DPP move of a double (two 32-bit moves): lanes without a source, and
+// the rows RowMask leaves out, read 0.
+template <int Ctrl, int RowMask>
 __device__ __forceinline__ double pb_dpp(double x) {
     const uint64_t u = (uint64_t)__double_as_longlong(x);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)u, Ctrl, 0xf, 0xf, true);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(u >> 32), Ctrl, 0xf, 0xf, true);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, Ctrl, RowMask, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), Ctrl, RowMask, 0xf, false);
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// One step of the chain scan: lanes with `take` compose their map with the
-// one DPP control Ctrl brings (S = a + b * S_before).
-template <int Ctrl>
-__device__ __forceinline__ void pb_chain_step(double& a, int& b, bool take) {
-    const double an = pb_dpp<Ctrl>(a);
-    const int bn = __builtin_amdgcn_mov_dpp(b, Ctrl, 0xf, 0xf, true);
-    if (take) {
-        a = b ? a + an : a;
-        b = b & bn;
-    }
+// One step of the chain scan: the lanes whose bit of B is set add the a
+// that DPP control Ctrl brings them (0 where it brings none).
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ void pb_chain_step(double& a, uint64_t B) {
+    const double an = pb_dpp<Ctrl, RowMask>(a);
+    a = __builtin_amdgcn_inverse_ballot_w64(B) ? a + an : a;
 }
 
 // The stretch sums that flow right across a wavefront's 64 consecutive
@@ -528,19 +525,27 @@ __device__ __forceinline__ void pb_chain_step(double& a, int& b, bool take) {
 // sum of its last stretch when that stretch continues into the next group
 // (mask bit 0 clear) and b_l = 1 when the whole group is that one stretch.
 // An inclusive scan of the affine maps by DPP row shifts and row broadcasts
-// (no LDS), in a fixed order; returns S_{l-1}, the carry lane l adds to its
-// first term (0 for lane 0).
-__device__ __forceinline__ double pb_window_carry(double a, bool bb, int l) {
-    int b = bb ? 1 : 0;
-    const int rl = l & 15;
-    pb_chain_step<0x111>(a, b, rl >= 1);  // row_shr:1
-    pb_chain_step<0x112>(a, b, rl >= 2);  // row_shr:2
-    pb_chain_step<0x114>(a, b, rl >= 4);  // row_shr:4
-    pb_chain_step<0x118>(a, b, rl >= 8);  // row_shr:8
-    pb_chain_step<0x142>(a, b, (l & 31) >= 16);  // row_bcast:15
-    pb_chain_step<0x143>(a, b, l >= 32);  // row_bcast:31
-    const double c = pb_dpp<0x138>(a);  // wave_shr:1
-    return l > 0 ? c : 0.0;
+// (no LDS), in a fixed order, with the b of all lanes kept as one wave mask
+// in scalar registers; returns S_{l-1}, the carry lane l adds to its first
+// term (0 for lane 0).
+__device__ __forceinline__ double pb_window_carry(double a, bool b) {
+    uint64_t B = __ballot(b);
+    // the lanes (l % 16) < d of every row, which a row shift by d leaves as they are
+    constexpr uint64_t kRowHead1 = 0x0001000100010001ull, kRowHead2 = 0x0003000300030003ull;
+    constexpr uint64_t kRowHead4 = 0x000f000f000f000full, kRowHead8 = 0x00ff00ff00ff00ffull;
+    pb_chain_step<0x111, 0xf>(a, B);  // row_shr:1
+    B &= (B << 1) | kRowHead1;
+    pb_chain_step<0x112, 0xf>(a, B);  // row_shr:2
+    B &= (B << 2) | kRowHead2;
+    pb_chain_step<0x114, 0xf>(a, B);  // row_shr:4
+    B &= (B << 4) | kRowHead4;
+    pb_chain_step<0x118, 0xf>(a, B);  // row_shr:8
+    B &= (B << 8) | kRowHead8;
+    pb_chain_step<0x142, 0xa>(a, B);  // row_bcast:15: rows 1 and 3 take lanes 15 and 47
+    B &= 0x0000ffff0000ffffull | (((B >> 15) & 1u) ? 0x00000000ffff0000ull : 0ull) |
+         (((B >> 47) & 1u) ? 0xffff000000000000ull : 0ull);
+    pb_chain_step<0x143, 0xc>(a, B);  // row_bcast:31: rows 2 and 3 take lane 31
+    return pb_dpp<0x138, 0xf>(a);     // wave_shr:1
 }
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -601,22 +606,33 @@ __device__ __forceinline__ void pb_scatter_batch(const PbScatterBatch<W>& cur, i
         const uint32_t mkq = ok ? cur.mk[q] : 0u;
         const pb_f32x4 waq = W ? cur.wa[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
         const pb_f32x4 wbq = W ? cur.wb[q] : pb_f32x4{0.f, 0.f, 0.f, 0.f};
-        float v[kPbVec];
-        pb_group_values<W>(sc, cur.u[q], waq, wbq, v);
-        const bool goes_on = ok && !(mkq & 1u);
         // weighted graphs merge stretches across the window (the unweighted
-        // kernel measured no faster for it: its registers are full)
-        const double carry = W ? pb_window_carry(goes_on ? pb_tail(v, mkq) : 0.0, goes_on && (mkq >> 1) == 0u, l) : 0.0;
+        // kernel measured no faster for it: its registers are full); the
+        // values are read where they are summed, so that the unweighted
+        // kernel keeps its registers
+        double carry = 0.0;
+        if constexpr (W) {
+            float v[kPbVec];
+            pb_group_values<W>(sc, cur.u[q], waq, wbq, v);
+            const bool goes_on = ok && !(mkq & 1u);
+            carry = pb_window_carry(goes_on ? pb_tail(v, mkq) : 0.0, goes_on && (mkq >> 1) == 0u);
+        }
         int32_t total = 0;
         const int32_t off = wave_excl_scan_small((uint32_t)__builtin_popcount(mkq), total);
         const int32_t first = __builtin_amdgcn_readfirstlane(vgq - off);  // lane 0: every lane runs here
         const bool contig = total > 0 && total <= kPbStage && __all(!ok || vgq - off == first);
         if (contig) {
-            if (ok) pb_emit_stage<W>(v, mkq, carry, stg + off);
+            if (ok) {
+                float v[kPbVec];
+                pb_group_values<W>(sc, cur.u[q], waq, wbq, v);
+                pb_emit_stage<W>(v, mkq, carry, stg + off);
+            }
             wave_lds_sync();
             for (int32_t j = l; j < total; j += kWave) vals[first + j] = stg[j];
             wave_lds_sync();
         } else if (ok) {
+            float v[kPbVec];
+            pb_group_values<W>(sc, cur.u[q], waq, wbq, v);
             pb_emit<W>(v, mkq, carry, vals + vgq);
         }
     }
