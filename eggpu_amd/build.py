@@ -72,7 +72,7 @@ def build_host(verbose=False):
         raise RuntimeError("a C compiler (gcc) is needed for eggpu_amd's host extension")
     out = os.path.join(HERE, "_egx_host" + sysconfig.get_config_var("EXT_SUFFIX"))
     tmp = out + ".tmp"
-    cmd = [cc, "-O2", "-Wall", "-Wextra", "-shared", "-fPIC", "-I", sysconfig.get_paths()["include"],
+    cmd = [cc, "-O2", "-Wall", "-Wextra", "-shared", "-fPIC", "-pthread", "-I", sysconfig.get_paths()["include"],
            os.path.join(CSRC, "egx_host.c"), "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

@@ -25,9 +25,21 @@
  *     A 64-bit fingerprint of the node keys' hashes, their order, their
  *     degrees and their first neighbours' hashes (splitmix64 finaliser over
  *     each position, summed and xored), and the number of adjacency entries.
+ *   int_arcs(adj, n, drop_self, threads) -> (V, E, m, fingerprint, arcs) | None
+ *     arcs() and stamp() in one pass for the common case: the adjacency's
+ *     keys are the ints 0..n-1 in order, every neighbour key is an int and
+ *     there are no weights.  V and E are bytearrays of int32 (E holds at
+ *     least m columns); the fingerprint and arc count are stamp()'s.  The
+ *     rows are split over up to `threads` POSIX threads by arc count.  The
+ *     calling thread keeps the GIL throughout, so no Python code runs, and
+ *     no dict changes, while the workers read; they only read dict entries
+ *     and int values (no references taken, no allocation, no Python
+ *     errors).  None when a key falls outside that case: the caller then
+ *     uses arcs() and stamp(), which raise the proper errors.
  */
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -272,6 +284,12 @@ static uint64_t mix(uint64_t x) {
     return x;
 }
 
+/* the per-position term of stamp()'s fingerprint */
+static uint64_t stamp_term(uint64_t h, uint64_t d, uint64_t i, uint64_t h1) {
+    return mix(h * 0x9E3779B97F4A7C15ull + d * 0xD6E8FEB86659FD93ull + i * 0xA0761D6478BD642Full +
+               h1 * 0xE7037ED1A0B428DBull);
+}
+
 static PyObject* stamp(PyObject* self, PyObject* args) {
     (void)self;
     PyObject* adj;
@@ -298,8 +316,7 @@ static PyObject* stamp(PyObject* self, PyObject* args) {
             h1 = PyObject_Hash(v);
             if (h1 == -1 && PyErr_Occurred()) return NULL;
         }
-        const uint64_t x = mix((uint64_t)h * 0x9E3779B97F4A7C15ull + (uint64_t)d * 0xD6E8FEB86659FD93ull +
-                               (uint64_t)i * 0xA0761D6478BD642Full + (uint64_t)h1 * 0xE7037ED1A0B428DBull);
+        const uint64_t x = stamp_term((uint64_t)h, (uint64_t)d, (uint64_t)i, (uint64_t)h1);
         sum += x;
         xr ^= mix(x + (uint64_t)i);
         arcs_total += d;
@@ -307,6 +324,142 @@ static PyObject* stamp(PyObject* self, PyObject* args) {
     }
     const uint64_t fp = mix(sum ^ (xr * 0x9E3779B97F4A7C15ull)) ^ (uint64_t)i;
     return Py_BuildValue("(Kn)", (unsigned long long)fp, arcs_total);
+}
+
+typedef struct {
+    PyObject* const* rows; /* the neighbour dict of each row (borrowed) */
+    const int64_t* off;    /* row i's columns go to E[off[i]...] */
+    int32_t* E;
+    int32_t* kept;         /* columns kept per row */
+    Py_ssize_t r0, r1, n;
+    int drop_self;
+    uint64_t sum, xr;
+    int bad;
+} IntArcsPart;
+
+static void* int_arcs_part(void* arg) {
+    IntArcsPart* p = (IntArcsPart*)arg;
+    for (Py_ssize_t i = p->r0; i < p->r1; ++i) {
+        int32_t* e = p->E + p->off[i];
+        Py_ssize_t q = 0, k = 0;
+        PyObject *v, *data;
+        uint64_t h1 = 0;
+        int first = 1;
+        while (PyDict_Next(p->rows[i], &q, &v, &data)) {
+            if (!PyLong_CheckExact(v)) {
+                p->bad = 1;
+                return NULL;
+            }
+            int of = 0;
+            const long long c = PyLong_AsLongLongAndOverflow(v, &of);
+            if (of || c < 0 || c >= p->n) {
+                p->bad = 1;
+                return NULL;
+            }
+            if (first) h1 = (uint64_t)c; /* hash(c) == c for 0 <= c < 2^61 - 1 */
+            first = 0;
+            if (p->drop_self && c == i) continue;
+            e[k++] = (int32_t)c;
+        }
+        p->kept[i] = (int32_t)k;
+        const uint64_t x = stamp_term((uint64_t)i, (uint64_t)(p->off[i + 1] - p->off[i]), (uint64_t)i, h1);
+        p->sum += x;
+        p->xr ^= mix(x + (uint64_t)i);
+    }
+    return NULL;
+}
+
+#define INT_ARCS_MAX_THREADS 32
+
+static PyObject* int_arcs(PyObject* self, PyObject* args) {
+    (void)self;
+    PyObject* adj;
+    Py_ssize_t n;
+    int drop_self, threads;
+    if (!PyArg_ParseTuple(args, "Onpi", &adj, &n, &drop_self, &threads)) return NULL;
+    if (!PyDict_Check(adj) || n < 0 || PyDict_GET_SIZE(adj) != n) Py_RETURN_NONE;
+    PyObject** rows = (PyObject**)malloc(sizeof(PyObject*) * (size_t)(n > 0 ? n : 1));
+    int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    int32_t* kept = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    PyObject *Vb = NULL, *Eb = NULL, *res = NULL;
+    if (!rows || !off || !kept) {
+        PyErr_NoMemory();
+        goto out;
+    }
+    /* rows in adjacency order, each keyed by its own index */
+    Py_ssize_t pos = 0, i = 0;
+    PyObject *u, *nbrs;
+    off[0] = 0;
+    while (PyDict_Next(adj, &pos, &u, &nbrs)) {
+        int of = 0;
+        if (i >= n || !PyLong_CheckExact(u) || PyLong_AsLongLongAndOverflow(u, &of) != (long long)i || of ||
+            !PyDict_Check(nbrs)) {
+            res = Py_None;
+            goto out;
+        }
+        rows[i] = nbrs;
+        off[i + 1] = off[i] + PyDict_GET_SIZE(nbrs);
+        ++i;
+    }
+    const int64_t total = off[n];
+    if (i != n || total > INT32_MAX - 1) {
+        res = Py_None;
+        goto out;
+    }
+    Vb = PyByteArray_FromStringAndSize(NULL, (Py_ssize_t)(n + 1) * 4);
+    Eb = PyByteArray_FromStringAndSize(NULL, (Py_ssize_t)(total > 0 ? total : 1) * 4);
+    if (!Vb || !Eb) goto out;
+    int32_t* V = (int32_t*)PyByteArray_AS_STRING(Vb);
+    int32_t* E = (int32_t*)PyByteArray_AS_STRING(Eb);
+    int T = threads < 1 ? 1 : (threads > INT_ARCS_MAX_THREADS ? INT_ARCS_MAX_THREADS : threads);
+    if (total < 65536 || n < 2 * T) T = 1;
+    IntArcsPart parts[INT_ARCS_MAX_THREADS];
+    pthread_t tid[INT_ARCS_MAX_THREADS];
+    int started[INT_ARCS_MAX_THREADS];
+    Py_ssize_t r = 0;
+    for (int t = 0; t < T; ++t) {
+        /* rows [r, r1): the first row whose arcs start at or past (t + 1) / T of them */
+        const int64_t goal = total * (t + 1) / T;
+        Py_ssize_t r1 = r;
+        if (t == T - 1) r1 = n;
+        else
+            while (r1 < n && off[r1] < goal) ++r1;
+        parts[t] = (IntArcsPart){rows, off, E, kept, r, r1, n, drop_self, 0, 0, 0};
+        r = r1;
+    }
+    for (int t = 1; t < T; ++t) started[t] = pthread_create(&tid[t], NULL, int_arcs_part, &parts[t]) == 0;
+    int_arcs_part(&parts[0]);
+    for (int t = 1; t < T; ++t) {
+        if (started[t]) pthread_join(tid[t], NULL);
+        else int_arcs_part(&parts[t]); /* no thread: this one runs it */
+    }
+    uint64_t sum = 0, xr = 0x243F6A8885A308D3ull;
+    for (int t = 0; t < T; ++t) {
+        if (parts[t].bad) {
+            res = Py_None;
+            goto out;
+        }
+        sum += parts[t].sum;
+        xr ^= parts[t].xr;
+    }
+    /* rows that dropped a self-loop leave gaps: close them, in order */
+    int64_t m = 0;
+    V[0] = 0;
+    for (Py_ssize_t k = 0; k < n; ++k) {
+        if (m != off[k] && kept[k] > 0) memmove(E + m, E + off[k], (size_t)kept[k] * 4);
+        m += kept[k];
+        V[k + 1] = (int32_t)m;
+    }
+    const uint64_t fp = mix(sum ^ (xr * 0x9E3779B97F4A7C15ull)) ^ (uint64_t)n;
+    res = Py_BuildValue("(OOnKn)", Vb, Eb, (Py_ssize_t)m, (unsigned long long)fp, (Py_ssize_t)total);
+out:
+    free(rows);
+    free(off);
+    free(kept);
+    Py_XDECREF(Vb);
+    Py_XDECREF(Eb);
+    if (res == Py_None) Py_RETURN_NONE;
+    return res;
 }
 
 /* identity(nodes) -> bool: nodes is a list whose i-th item is the int i
@@ -335,6 +488,7 @@ static PyMethodDef methods[] = {
     {"arcs", arcs, METH_VARARGS, "adjacency dicts -> out-CSR arrays"},
     {"sort_rows", sort_rows, METH_VARARGS, "sort each CSR row's columns ascending, in place"},
     {"stamp", stamp, METH_VARARGS, "fingerprint of node keys, order, degrees, first neighbours; arc count"},
+    {"int_arcs", int_arcs, METH_VARARGS, "arcs() and stamp() in one threaded pass for int-keyed, unweighted adjacencies"},
     {NULL, NULL, 0, NULL},
 };
 

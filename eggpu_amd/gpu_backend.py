@@ -64,6 +64,8 @@ _RESULT_CACHE_MAX_NODES = _count("EASYGRAPH_GPU_RESULT_CACHE_MAX_NODES", 2000000
 _CONTEXT_CACHE_OFF = _flag("EASYGRAPH_GPU_DISABLE_GRAPH_CONTEXT_CACHE")
 _DEVICE_GRAPH_CACHE_OFF = _flag("EASYGRAPH_GPU_DISABLE_CPP_GRAPH_CACHE")
 _VALIDATE_EVERY = _count("EASYGRAPH_GPU_CTX_VALIDATE_EVERY", 32, 1)
+# threads of the C walk of int-keyed adjacencies (one below 64K arcs)
+_HOST_THREADS = _count("EASYGRAPH_GPU_HOST_THREADS", min(8, len(os.sched_getaffinity(0))), 1)
 # the dict-returning native route: the PageRank switch, else the global one
 # when that is set, else off
 _PR_PREFER_DICT = _flag(
@@ -292,7 +294,10 @@ def _context_current(G, ctx):
     if len(_adj_dict(G)) != ctx.get("node_count"):
         return False
     ctx["uses"] = ctx.get("uses", 0) + 1
-    return ctx["uses"] % _VALIDATE_EVERY != 0 or ctx.get("stamp") == _graph_stamp(G)
+    if ctx["uses"] % _VALIDATE_EVERY != 0:
+        return True
+    # a context whose adjacency was never read has no stamp: it is renewed
+    return ctx.get("stamp") is not None and ctx["stamp"] == _graph_stamp(G)
 
 
 def _new_context(G):
@@ -303,8 +308,10 @@ def _new_context(G):
                 "node_to_idx": None, "node_count": int(len(G)), "mutation_generation": None, "stamp": None,
                 "uses": 0, "result_cache": {}}
     nodes = _node_list(G)
+    # the stamp is taken by the first read of the adjacency (in the same C
+    # pass when it can be), so that it describes what the device graph holds
     ctx = {"ctx_id": next(_ctx_ids), "directed": directed, "nodes": nodes, "node_count": len(nodes),
-           "mutation_generation": getattr(G, "_mutation_generation", None), "stamp": _graph_stamp(G),
+           "mutation_generation": getattr(G, "_mutation_generation", None), "stamp": None,
            "uses": 0, "result_cache": {}}
     # nodes 0..n-1 in order are their own indices: no index dict is needed
     ctx["node_to_idx"] = None if _identity_labels(ctx) else dict(zip(nodes, range(len(nodes))))
@@ -379,6 +386,21 @@ def _walk_py(adj, index, n, drop_self, weight):
     return V, E, W
 
 
+def _int_arcs(adj, prepared, index, n, drop_self):
+    """(V, E) from the threaded C walk when the rows are keyed 0..n-1 and
+    every neighbour is an int (no weights), recording the context's stamp
+    from the same pass when it has none; None otherwise."""
+    if _host is None or index is not None or type(adj) is not dict or not hasattr(_host, "int_arcs"):
+        return None
+    got = _host.int_arcs(adj, n, drop_self, _HOST_THREADS)
+    if got is None:
+        return None
+    Vb, Eb, m, fp, slots = got
+    if prepared.get("stamp") is None:
+        prepared["stamp"] = (bool(prepared["directed"]), len(adj), slots, fp)
+    return np.frombuffer(Vb, dtype=np.int32), np.frombuffer(Eb, dtype=np.int32)[:m]
+
+
 def _out_arcs(G, prepared, weight, sort):
     """(V, E, W) of the out-arcs: rows in node order, columns in adjacency
     order or, with sort, ascending; directed graphs keep self-loops
@@ -388,9 +410,17 @@ def _out_arcs(G, prepared, weight, sort):
     n = len(prepared["nodes"])
     index = None if _identity_labels(prepared) else prepared["node_to_idx"]
     drop_self = not prepared["directed"]
-    if _host is not None and type(adj) is dict:
+    got = _int_arcs(adj, prepared, index, n, drop_self) if weight is None else None
+    if got is not None:
+        V, E = got
+        W = None
+    elif _host is not None and type(adj) is dict:
         stamp = prepared.get("stamp")
-        cap = stamp[2] if stamp and prepared.get("uses", 0) == 0 else sum(map(len, adj.values()))
+        if stamp is None:
+            stamp = prepared["stamp"] = _graph_stamp(G)
+            cap = stamp[2]
+        else:
+            cap = sum(map(len, adj.values()))
         if cap >= 2**31 - 1:
             raise ValueError("adjacency entry count exceeds the signed 32-bit EGGPU CSR limit")
         V = np.empty(n + 1, dtype=np.int32)
@@ -432,6 +462,9 @@ def adjacency_csr(G, prepared):
     adj = _adj_dict(G)
     n = len(prepared["nodes"])
     index = None if _identity_labels(prepared) else prepared["node_to_idx"]
+    got = _int_arcs(adj, prepared, index, n, False)
+    if got is not None:
+        return got
     if _host is not None and type(adj) is dict:
         cap = sum(map(len, adj.values()))
         V = np.empty(n + 1, dtype=np.int32)

@@ -83,6 +83,57 @@ def test_c_reader_rejects_unknown_neighbours_and_short_buffers():
         host.arcs({0: {1: {}}, 1: {0: {}}, 2: {}}, None, 3, False, V, np.empty(1, np.int32), None, None)
 
 
+def _int_adjacency(n, arcs_per_row, directed, seed=5):
+    rng = np.random.default_rng(seed)
+    G = DiGraph() if directed else Graph()
+    G.add_nodes_from(range(n))
+    deg = rng.integers(0, 2 * arcs_per_row, n)
+    deg[::97] = 40 * arcs_per_row  # a few long rows to balance across threads
+    src = np.repeat(np.arange(n), deg)
+    dst = rng.integers(0, n, len(src))
+    dst[::50] = src[::50]  # self-loops, which undirected graphs drop
+    G.add_edges_from(zip(src.tolist(), dst.tolist()))
+    return G
+
+
+@pytest.mark.parametrize("directed", [True, False])
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_int_reader_matches_arcs_and_stamp(directed, threads):
+    G = _int_adjacency(3000, 24, directed)
+    adj, n = G._adj, len(G._adj)
+    Vb, Eb, m, fp, slots = host.int_arcs(adj, n, not directed, threads)
+    cap = sum(map(len, adj.values()))
+    V, E = np.empty(n + 1, np.int32), np.empty(cap, np.int32)
+    m0 = host.arcs(adj, None, n, not directed, V, E, None, None)
+    assert cap >= 65536 and m == m0
+    assert np.array_equal(np.frombuffer(Vb, np.int32), V)
+    assert np.array_equal(np.frombuffer(Eb, np.int32)[:m], E[:m0])
+    assert (fp, slots) == host.stamp(adj)
+    # the route through the backend records that stamp on the new context
+    prep = gpu_backend._prepare_graph_cache(G)
+    assert prep["stamp"] is None
+    V2, E2, _ = gpu_backend._out_arcs(G, prep, None, sort=False)
+    assert np.array_equal(V2, V) and np.array_equal(E2, E[:m0])
+    assert prep["stamp"] == gpu_backend._graph_stamp(G)
+
+
+def test_threaded_int_reader_declines_other_keys():
+    G = _int_adjacency(200, 4, True)
+    adj, n = G._adj, len(G._adj)
+    assert host.int_arcs(adj, n, False, 4) is not None
+    assert host.int_arcs(adj, n + 1, False, 4) is None  # row count differs
+    adj[7]["x"] = {}  # a neighbour that is not an int
+    assert host.int_arcs(adj, n, False, 4) is None
+    del adj[7]["x"]
+    adj[7][n + 5] = {}  # an int outside the node range
+    assert host.int_arcs(adj, n, False, 4) is None
+    del adj[7][n + 5]
+    H = DiGraph()
+    H.add_nodes_from([1, 0, 2])  # keys not in index order
+    H.add_edges_from([(0, 1), (1, 2)])
+    assert host.int_arcs(H._adj, 3, False, 2) is None
+
+
 def test_stamp_sees_first_neighbour_rewiring():
     G = DiGraph()
     G.add_edges_from([(0, 1), (1, 2), (2, 0)])
