@@ -30,7 +30,8 @@
  *     keys are the ints 0..n-1 in order, every neighbour key is an int and
  *     there are no weights.  V and E are bytearrays of int32 (E holds at
  *     least m columns); the fingerprint and arc count are stamp()'s.  The
- *     rows are split over up to `threads` POSIX threads by arc count.  The
+ *     rows are split over up to `threads` threads (a pool kept for the
+ *     process) by arc count.  The
  *     calling thread keeps the GIL throughout, so no Python code runs, and
  *     no dict changes, while the workers read; they only read dict entries
  *     and int values (no references taken, no allocation, no Python
@@ -40,6 +41,7 @@
 #define PY_SSIZE_T_CLEAN
 #include <Python.h>
 #include <pthread.h>
+#include <unistd.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -326,50 +328,200 @@ static PyObject* stamp(PyObject* self, PyObject* args) {
     return Py_BuildValue("(Kn)", (unsigned long long)fp, arcs_total);
 }
 
-typedef struct {
-    PyObject* const* rows; /* the neighbour dict of each row (borrowed) */
-    const int64_t* off;    /* row i's columns go to E[off[i]...] */
-    int32_t* E;
-    int32_t* kept;         /* columns kept per row */
-    Py_ssize_t r0, r1, n;
-    int drop_self;
-    uint64_t sum, xr;
-    int bad;
-} IntArcsPart;
+#define INT_ARCS_MAX_THREADS 32
 
-static void* int_arcs_part(void* arg) {
-    IntArcsPart* p = (IntArcsPart*)arg;
-    for (Py_ssize_t i = p->r0; i < p->r1; ++i) {
-        int32_t* e = p->E + p->off[i];
-        Py_ssize_t q = 0, k = 0;
-        PyObject *v, *data;
-        uint64_t h1 = 0;
-        int first = 1;
-        while (PyDict_Next(p->rows[i], &q, &v, &data)) {
-            if (!PyLong_CheckExact(v)) {
-                p->bad = 1;
-                return NULL;
-            }
-            int of = 0;
-            const long long c = PyLong_AsLongLongAndOverflow(v, &of);
-            if (of || c < 0 || c >= p->n) {
-                p->bad = 1;
-                return NULL;
-            }
-            if (first) h1 = (uint64_t)c; /* hash(c) == c for 0 <= c < 2^61 - 1 */
-            first = 0;
-            if (p->drop_self && c == i) continue;
-            e[k++] = (int32_t)c;
-        }
-        p->kept[i] = (int32_t)k;
-        const uint64_t x = stamp_term((uint64_t)i, (uint64_t)(p->off[i + 1] - p->off[i]), (uint64_t)i, h1);
-        p->sum += x;
-        p->xr ^= mix(x + (uint64_t)i);
+/* A pool of worker threads for int_arcs, made on first use and kept for the
+ * process (creating a thread costs tens of microseconds, a cold R-MAT 14
+ * walk a few hundred).  run_parallel(T, fn, args, size) runs fn on args[0]
+ * here and on args[1..T-1] in pool threads, and returns once all are done.
+ * A process forked from this one starts a pool of its own. */
+typedef void (*PoolFn)(void*);
+typedef struct {
+    pthread_mutex_t mu;
+    pthread_cond_t go, done;
+    pid_t pid;
+    int workers;      /* threads started */
+    unsigned long gen; /* job generation */
+    PoolFn fn;
+    char* args;
+    size_t size;
+    int jobs;         /* workers taking part in the current job */
+    int left;         /* of those, still running */
+} WalkPool;
+
+static WalkPool walk_pool = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, PTHREAD_COND_INITIALIZER, 0, 0, 0,
+                             NULL, NULL, 0, 0, 0};
+
+typedef struct {
+    int id;              /* 1.. */
+    unsigned long gen0;  /* the job generation when the thread was made */
+} PoolSeat;
+
+static PoolSeat pool_seats[INT_ARCS_MAX_THREADS];
+
+static void* pool_worker(void* arg) {
+    const int id = ((PoolSeat*)arg)->id;
+    WalkPool* P = &walk_pool;
+    pthread_mutex_lock(&P->mu);
+    /* a job posted after this thread was made is one it takes part in */
+    unsigned long seen = ((PoolSeat*)arg)->gen0;
+    for (;;) {
+        while (P->gen == seen) pthread_cond_wait(&P->go, &P->mu);
+        seen = P->gen;
+        if (id >= P->jobs) continue;
+        PoolFn fn = P->fn;
+        void* a = P->args + (size_t)id * P->size;
+        pthread_mutex_unlock(&P->mu);
+        fn(a);
+        pthread_mutex_lock(&P->mu);
+        if (--P->left == 0) pthread_cond_signal(&P->done);
     }
     return NULL;
 }
 
-#define INT_ARCS_MAX_THREADS 32
+/* the number of pool threads that can take part (at most want - 1) */
+static int pool_ready(int want) {
+    WalkPool* P = &walk_pool;
+    if (P->pid != getpid()) { /* first use, or a forked child: no threads here */
+        pthread_mutex_init(&P->mu, NULL);
+        pthread_cond_init(&P->go, NULL);
+        pthread_cond_init(&P->done, NULL);
+        P->pid = getpid();
+        P->workers = 0;
+        P->gen = 0;
+    }
+    while (P->workers < want - 1 && P->workers < INT_ARCS_MAX_THREADS - 1) {
+        pthread_t t;
+        pthread_attr_t at;
+        pool_seats[P->workers].id = P->workers + 1;
+        pool_seats[P->workers].gen0 = P->gen; /* gen changes only in run_parallel, on this thread */
+        pthread_attr_init(&at);
+        pthread_attr_setdetachstate(&at, PTHREAD_CREATE_DETACHED);
+        const int rc = pthread_create(&t, &at, pool_worker, &pool_seats[P->workers]);
+        pthread_attr_destroy(&at);
+        if (rc != 0) break;
+        ++P->workers;
+    }
+    return P->workers;
+}
+
+static void run_parallel(int T, PoolFn fn, void* args, size_t size) {
+    WalkPool* P = &walk_pool;
+    if (T > 1) {
+        pthread_mutex_lock(&P->mu);
+        P->fn = fn;
+        P->args = (char*)args;
+        P->size = size;
+        P->jobs = T;
+        P->left = T - 1;
+        ++P->gen;
+        pthread_cond_broadcast(&P->go);
+        pthread_mutex_unlock(&P->mu);
+    }
+    fn(args);
+    if (T > 1) {
+        pthread_mutex_lock(&P->mu);
+        while (P->left > 0) pthread_cond_wait(&P->done, &P->mu);
+        pthread_mutex_unlock(&P->mu);
+    }
+}
+
+/* What the parts of one int_arcs call share.  Phase 1 (rows split evenly):
+ * check each row's key and neighbour dict and take its size.  The calling
+ * thread then turns the sizes into offsets, allocates E and splits the rows
+ * by arc count.  Phase 2: read each row's neighbours into E. */
+typedef struct {
+    PyObject* const* keys; /* the adjacency's keys and neighbour dicts, borrowed */
+    PyObject* const* rows;
+    int64_t* off;          /* row i's columns go to E[off[i]...] */
+    int32_t* E;
+    int32_t* kept;         /* columns kept per row */
+    Py_ssize_t n;
+    int drop_self, T;
+    Py_ssize_t cut[INT_ARCS_MAX_THREADS + 1]; /* phase 2 row ranges */
+} IntArcsJob;
+
+typedef struct {
+    IntArcsJob* job;
+    int t;
+    uint64_t sum, xr;
+    int bad;
+} IntArcsPart;
+
+static void int_arcs_sizes(void* arg) {
+    IntArcsPart* p = (IntArcsPart*)arg;
+    IntArcsJob* j = p->job;
+    const Py_ssize_t r0 = j->n * p->t / j->T, r1 = j->n * (p->t + 1) / j->T;
+    for (Py_ssize_t i = r0; i < r1; ++i) {
+        PyObject* u = j->keys[i];
+        int of = 0;
+        if (!PyLong_CheckExact(u) || PyLong_AsLongLongAndOverflow(u, &of) != (long long)i || of ||
+            !PyDict_Check(j->rows[i])) {
+            p->bad = 1;
+            return;
+        }
+        j->off[i + 1] = PyDict_GET_SIZE(j->rows[i]);
+    }
+}
+
+static void int_arcs_fill(void* arg) {
+    IntArcsPart* p = (IntArcsPart*)arg;
+    IntArcsJob* j = p->job;
+    for (Py_ssize_t i = j->cut[p->t]; i < j->cut[p->t + 1]; ++i) {
+        int32_t* e = j->E + j->off[i];
+        Py_ssize_t q = 0, k = 0;
+        PyObject *v, *data;
+        uint64_t h1 = 0;
+        int first = 1;
+        while (PyDict_Next(j->rows[i], &q, &v, &data)) {
+            if (!PyLong_CheckExact(v)) {
+                p->bad = 1;
+                return;
+            }
+            int of = 0;
+            const long long c = PyLong_AsLongLongAndOverflow(v, &of);
+            if (of || c < 0 || c >= j->n) {
+                p->bad = 1;
+                return;
+            }
+            if (first) h1 = (uint64_t)c; /* hash(c) == c for 0 <= c < 2^61 - 1 */
+            first = 0;
+            if (j->drop_self && c == i) continue;
+            e[k++] = (int32_t)c;
+        }
+        j->kept[i] = (int32_t)k;
+        const uint64_t x = stamp_term((uint64_t)i, (uint64_t)(j->off[i + 1] - j->off[i]), (uint64_t)i, h1);
+        p->sum += x;
+        p->xr ^= mix(x + (uint64_t)i);
+    }
+}
+
+/* between the phases: offsets, E, the split by arcs; returns 0, 1 (a key
+ * outside the case) or 2 (allocation failed, exception set) */
+static int int_arcs_setup(IntArcsJob* j, IntArcsPart* parts, PyObject** Vb, PyObject** Eb, int64_t* total) {
+    for (int t = 0; t < j->T; ++t)
+        if (parts[t].bad) return 1;
+    const Py_ssize_t n = j->n;
+    j->off[0] = 0;
+    for (Py_ssize_t k = 0; k < n; ++k) j->off[k + 1] += j->off[k];
+    *total = j->off[n];
+    if (*total > INT32_MAX - 1) return 1;
+    *Vb = PyByteArray_FromStringAndSize(NULL, (Py_ssize_t)(n + 1) * 4);
+    *Eb = PyByteArray_FromStringAndSize(NULL, (Py_ssize_t)(*total > 0 ? *total : 1) * 4);
+    if (!*Vb || !*Eb) return 2;
+    j->E = (int32_t*)PyByteArray_AS_STRING(*Eb);
+    j->cut[0] = 0;
+    Py_ssize_t r = 0;
+    for (int t = 0; t < j->T; ++t) {
+        /* up to the first row whose arcs start at or past (t + 1) / T of them */
+        const int64_t goal = *total * (t + 1) / j->T;
+        if (t == j->T - 1) r = n;
+        else
+            while (r < n && j->off[r] < goal) ++r;
+        j->cut[t + 1] = r;
+    }
+    return 0;
+}
 
 static PyObject* int_arcs(PyObject* self, PyObject* args) {
     (void)self;
@@ -378,71 +530,65 @@ static PyObject* int_arcs(PyObject* self, PyObject* args) {
     int drop_self, threads;
     if (!PyArg_ParseTuple(args, "Onpi", &adj, &n, &drop_self, &threads)) return NULL;
     if (!PyDict_Check(adj) || n < 0 || PyDict_GET_SIZE(adj) != n) Py_RETURN_NONE;
-    PyObject** rows = (PyObject**)malloc(sizeof(PyObject*) * (size_t)(n > 0 ? n : 1));
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    PyObject** keys = (PyObject**)malloc(sizeof(PyObject*) * nn);
+    PyObject** rows = (PyObject**)malloc(sizeof(PyObject*) * nn);
     int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
-    int32_t* kept = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t* kept = (int32_t*)malloc(sizeof(int32_t) * nn);
     PyObject *Vb = NULL, *Eb = NULL, *res = NULL;
-    if (!rows || !off || !kept) {
+    IntArcsJob job;
+    IntArcsPart parts[INT_ARCS_MAX_THREADS];
+    memset(&job, 0, sizeof(job));
+    if (!keys || !rows || !off || !kept) {
         PyErr_NoMemory();
         goto out;
     }
-    /* rows in adjacency order, each keyed by its own index */
+    /* the adjacency's entries in order; the parts check them */
     Py_ssize_t pos = 0, i = 0;
     PyObject *u, *nbrs;
-    off[0] = 0;
-    while (PyDict_Next(adj, &pos, &u, &nbrs)) {
-        int of = 0;
-        if (i >= n || !PyLong_CheckExact(u) || PyLong_AsLongLongAndOverflow(u, &of) != (long long)i || of ||
-            !PyDict_Check(nbrs)) {
-            res = Py_None;
-            goto out;
-        }
+    while (i < n && PyDict_Next(adj, &pos, &u, &nbrs)) {
+        keys[i] = u;
         rows[i] = nbrs;
-        off[i + 1] = off[i] + PyDict_GET_SIZE(nbrs);
         ++i;
     }
-    const int64_t total = off[n];
-    if (i != n || total > INT32_MAX - 1) {
+    if (i != n) {
         res = Py_None;
         goto out;
     }
-    Vb = PyByteArray_FromStringAndSize(NULL, (Py_ssize_t)(n + 1) * 4);
-    Eb = PyByteArray_FromStringAndSize(NULL, (Py_ssize_t)(total > 0 ? total : 1) * 4);
-    if (!Vb || !Eb) goto out;
-    int32_t* V = (int32_t*)PyByteArray_AS_STRING(Vb);
-    int32_t* E = (int32_t*)PyByteArray_AS_STRING(Eb);
     int T = threads < 1 ? 1 : (threads > INT_ARCS_MAX_THREADS ? INT_ARCS_MAX_THREADS : threads);
-    if (total < 65536 || n < 2 * T) T = 1;
-    IntArcsPart parts[INT_ARCS_MAX_THREADS];
-    pthread_t tid[INT_ARCS_MAX_THREADS];
-    int started[INT_ARCS_MAX_THREADS];
-    Py_ssize_t r = 0;
-    for (int t = 0; t < T; ++t) {
-        /* rows [r, r1): the first row whose arcs start at or past (t + 1) / T of them */
-        const int64_t goal = total * (t + 1) / T;
-        Py_ssize_t r1 = r;
-        if (t == T - 1) r1 = n;
-        else
-            while (r1 < n && off[r1] < goal) ++r1;
-        parts[t] = (IntArcsPart){rows, off, E, kept, r, r1, n, drop_self, 0, 0, 0};
-        r = r1;
+    if (n < 4096) T = 1; /* few rows: not worth the threads */
+    if (T > 1) T = 1 + pool_ready(T);
+    job.keys = keys;
+    job.rows = rows;
+    job.off = off;
+    job.kept = kept;
+    job.n = n;
+    job.drop_self = drop_self;
+    job.T = T;
+    for (int t = 0; t < T; ++t) parts[t] = (IntArcsPart){&job, t, 0, 0, 0};
+    /* the calling thread keeps the GIL through both phases */
+    run_parallel(T, int_arcs_sizes, parts, sizeof(IntArcsPart));
+    int64_t total = 0;
+    int bad = int_arcs_setup(&job, parts, &Vb, &Eb, &total);
+    if (bad == 2) goto out; /* the allocation's exception is set */
+    if (bad) {
+        res = Py_None;
+        goto out;
     }
-    for (int t = 1; t < T; ++t) started[t] = pthread_create(&tid[t], NULL, int_arcs_part, &parts[t]) == 0;
-    int_arcs_part(&parts[0]);
-    for (int t = 1; t < T; ++t) {
-        if (started[t]) pthread_join(tid[t], NULL);
-        else int_arcs_part(&parts[t]); /* no thread: this one runs it */
-    }
+    run_parallel(T, int_arcs_fill, parts, sizeof(IntArcsPart));
     uint64_t sum = 0, xr = 0x243F6A8885A308D3ull;
     for (int t = 0; t < T; ++t) {
-        if (parts[t].bad) {
-            res = Py_None;
-            goto out;
-        }
+        bad |= parts[t].bad;
         sum += parts[t].sum;
         xr ^= parts[t].xr;
     }
+    if (bad) {
+        res = Py_None;
+        goto out;
+    }
     /* rows that dropped a self-loop leave gaps: close them, in order */
+    int32_t* V = (int32_t*)PyByteArray_AS_STRING(Vb);
+    int32_t* E = job.E;
     int64_t m = 0;
     V[0] = 0;
     for (Py_ssize_t k = 0; k < n; ++k) {
@@ -453,6 +599,7 @@ static PyObject* int_arcs(PyObject* self, PyObject* args) {
     const uint64_t fp = mix(sum ^ (xr * 0x9E3779B97F4A7C15ull)) ^ (uint64_t)n;
     res = Py_BuildValue("(OOnKn)", Vb, Eb, (Py_ssize_t)m, (unsigned long long)fp, (Py_ssize_t)total);
 out:
+    free(keys);
     free(rows);
     free(off);
     free(kept);

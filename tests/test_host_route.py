@@ -117,6 +117,28 @@ def test_threaded_int_reader_matches_arcs_and_stamp(directed, threads):
     assert prep["stamp"] == gpu_backend._graph_stamp(G)
 
 
+def _walk_in_child(conn):
+    G = _int_adjacency(3000, 24, True)
+    got = host.int_arcs(G._adj, len(G._adj), False, 8)
+    conn.send(got is not None and got[2] == sum(map(len, G._adj.values())))
+
+
+def test_threaded_int_reader_pool_survives_fork():
+    import multiprocessing as mp
+
+    G = _int_adjacency(3000, 24, True)
+    assert host.int_arcs(G._adj, len(G._adj), False, 8) is not None  # the pool exists here
+    ctx = mp.get_context("fork")
+    a, b = ctx.Pipe()
+    p = ctx.Process(target=_walk_in_child, args=(b,))
+    p.start()
+    p.join(60)
+    alive = p.is_alive()
+    if alive:
+        p.kill()
+    assert not alive and p.exitcode == 0 and a.poll(1) and a.recv()
+
+
 def test_threaded_int_reader_declines_other_keys():
     G = _int_adjacency(200, 4, True)
     adj, n = G._adj, len(G._adj)
