@@ -626,16 +626,10 @@ constexpr int kW64OldRows = 4;  // old-value rows per window of wave_stream64
 // Hub segments (a launch of their own: their register needs differ from
 // the row kernel's).  Block b is segment b; its partial slots are zeroed.
 template <int C>
-__global__ __launch_bounds__(kRowThreads) void k_spmm_hub(SpmmArgs a) {
+__device__ __forceinline__ void hub_block(const SpmmArgs& a, uint32_t b) {
     __shared__ double sh[kRowWaves][kCols];
-    if (!a.ctl->any_active) return;
-    const int cur = a.ctl->cur;
-    // selects, not an index: a runtime index would move the argument block to scratch
-    a.x_in = cur ? a.xbuf[1] : a.xbuf[0];
-    a.x_out = cur ? a.xbuf[0] : a.xbuf[1];
     const int lane = lane_id();
     const int w = wave_id();
-    const uint32_t b = blockIdx.x;
     // four wavefronts take contiguous quarters, combined in order
     const HubSeg sg = a.hsegs[b];
     const int32_t len = sg.end - sg.beg;
@@ -654,16 +648,10 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_hub(SpmmArgs a) {
 // Non-hub rows: block b (partial slot nblk_h + b) holds 4 wavefronts of the
 // merge-path split.
 template <int C>
-__global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
+__device__ __forceinline__ void rows_block(const SpmmArgs& a, uint32_t b) {
     __shared__ double sh[3][kRowWaves][kCols];
-    if (!a.ctl->any_active) return;
-    const int cur = a.ctl->cur;
-    // selects, not an index: a runtime index would move the argument block to scratch
-    a.x_in = cur ? a.xbuf[1] : a.xbuf[0];
-    a.x_out = cur ? a.xbuf[0] : a.xbuf[1];
     const int lane = lane_id();
     const int w = wave_id();
-    const uint32_t b = blockIdx.x;
     const bool owner = lane < C;  // lanes that write a column's partials
     const ColState cs = a.cols[lane % C];
     const double cj = (1.0 - cs.alpha) + cs.alpha * cs.dang;
@@ -694,6 +682,20 @@ __global__ __launch_bounds__(kRowThreads) void k_spmm_rows(SpmmArgs a) {
             a.partials[(slot * 3 + q) * kCols + lane] = t;
         }
     }
+}
+
+// One iteration's row sums in one launch: blocks [0, nblk_h) take the hub
+// segments, the rest the other rows, so that the two overlap (as two
+// launches the hub blocks ran alone, and each launch boundary cost a gap).
+template <int C>
+__global__ __launch_bounds__(kRowThreads) void k_spmm_iter(SpmmArgs a) {
+    if (!a.ctl->any_active) return;
+    const int cur = a.ctl->cur;
+    // selects, not an index: a runtime index would move the argument block to scratch
+    a.x_in = cur ? a.xbuf[1] : a.xbuf[0];
+    a.x_out = cur ? a.xbuf[0] : a.xbuf[1];
+    if (blockIdx.x < a.nblk_h) hub_block<C>(a, blockIdx.x);
+    else rows_block<C>(a, blockIdx.x - a.nblk_h);
 }
 
 // Reduce, in two launches.  Part r (of n_red workgroups, 4 wavefronts,
@@ -1477,23 +1479,13 @@ int batch_step(Batch& b, int32_t iters, bool record_time) {
 
 // `iters` iterations (rows + reduce launches) at the batch's current width.
 int enqueue_iterations(Batch& b, const SpmmArgs& a, int32_t iters, hipStream_t s) {
-    const uint32_t rgrid = b.grid - b.nblk_h;
     for (int32_t i = 0; i < iters; ++i) {
-        if (b.nblk_h > 0) {
+        if (b.grid > 0) {
             switch (b.width) {
-                case 8: k_spmm_hub<8><<<b.nblk_h, kRowThreads, 0, s>>>(a); break;
-                case 16: k_spmm_hub<16><<<b.nblk_h, kRowThreads, 0, s>>>(a); break;
-                case 32: k_spmm_hub<32><<<b.nblk_h, kRowThreads, 0, s>>>(a); break;
-                default: k_spmm_hub<64><<<b.nblk_h, kRowThreads, 0, s>>>(a); break;
-            }
-            EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
-        }
-        if (rgrid > 0) {
-            switch (b.width) {
-                case 8: k_spmm_rows<8><<<rgrid, kRowThreads, 0, s>>>(a); break;
-                case 16: k_spmm_rows<16><<<rgrid, kRowThreads, 0, s>>>(a); break;
-                case 32: k_spmm_rows<32><<<rgrid, kRowThreads, 0, s>>>(a); break;
-                default: k_spmm_rows<64><<<rgrid, kRowThreads, 0, s>>>(a); break;
+                case 8: k_spmm_iter<8><<<b.grid, kRowThreads, 0, s>>>(a); break;
+                case 16: k_spmm_iter<16><<<b.grid, kRowThreads, 0, s>>>(a); break;
+                case 32: k_spmm_iter<32><<<b.grid, kRowThreads, 0, s>>>(a); break;
+                default: k_spmm_iter<64><<<b.grid, kRowThreads, 0, s>>>(a); break;
             }
             EGX_CHECK_LAUNCH(EGX_DEVICE_ERR);
         }
